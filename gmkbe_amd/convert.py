@@ -1,0 +1,111 @@
+"""Edge list -> ``.graph.bin`` conversion with the reference converter's relabel.
+
+Restates preprocess/graph_conver.cpp:500-817 (``Convert``):
+
+1. every raw ID that occurs in either column is compacted by rank
+   (graph_conver.cpp:566-584);
+2. IDs seen in the first column become left vertices, IDs seen in the second
+   column right vertices, each numbered by compacted rank (graph_conver.cpp:586-613);
+   an ID used in both columns yields one left and one right vertex;
+3. duplicate edges are dropped per adjacency list (graph_conver.cpp:706-713);
+4. each side is relabelled by sorting on (degree_bucket(d'), d', old id) with
+   d' = deg - 2 on the left and deg - 1 on the right, computed in uint32, so a
+   left vertex of degree 1 wraps to 0xFFFFFFFF and sorts last
+   (graph_conver.cpp:715-733; degree_bucket at graph_conver.cpp:443-451);
+5. lists are rebuilt under the new IDs and sorted (graph_conver.cpp:746-777).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from .graphio import Graph
+
+
+def degree_bucket(d: np.ndarray) -> np.ndarray:
+    """Number of times 32 must be multiplied in before it reaches d, capped at 8
+    (graph_conver.cpp:443-451)."""
+    d = np.asarray(d, dtype=np.uint64)
+    b = np.zeros(d.shape, dtype=np.int64)
+    limit = np.uint64(32)
+    for _ in range(8):
+        more = d > limit
+        b += more
+        limit = limit * np.uint64(32)
+    return b
+
+
+def parse_edge_text(path: str) -> tuple[np.ndarray, np.ndarray]:
+    """Read ``u v`` lines; lines starting with '%' or '#' are skipped
+    (graph_conver.cpp:529-549).  Only the first two integers of a line count."""
+    us, vs = [], []
+    with open(path, "rb") as f:
+        for line in f:
+            s = line.lstrip()
+            if not s or s[:1] in (b"%", b"#"):
+                continue
+            parts = s.split()
+            if len(parts) < 2:
+                continue
+            us.append(int(parts[0]))
+            vs.append(int(parts[1]))
+    return np.asarray(us, dtype=np.uint32), np.asarray(vs, dtype=np.uint32)
+
+
+def convert_edges(u: np.ndarray, v: np.ndarray) -> Graph:
+    u = np.asarray(u, dtype=np.uint32)
+    v = np.asarray(v, dtype=np.uint32)
+    if u.size == 0:
+        raise ValueError("empty edge list")
+    ids = np.unique(np.concatenate([u, v]))
+    cu = np.searchsorted(ids, u)
+    cv = np.searchsorted(ids, v)
+    nid = ids.size
+    seen_src = np.zeros(nid, dtype=bool)
+    seen_dst = np.zeros(nid, dtype=bool)
+    seen_src[cu] = True
+    seen_dst[cv] = True
+    src_rank = np.cumsum(seen_src) - 1
+    dst_rank = np.cumsum(seen_dst) - 1
+    n_left = int(seen_src.sum())
+    n_right = int(seen_dst.sum())
+    n = n_left + n_right
+    lu = src_rank[cu].astype(np.int64)
+    rv = dst_rank[cv].astype(np.int64) + n_left
+    # unique undirected pairs (a left-right pair listed twice collapses)
+    key = np.unique(lu * n + rv)
+    lu, rv = key // n, key % n
+    deg = np.zeros(n, dtype=np.int64)
+    np.add.at(deg, lu, 1)
+    np.add.at(deg, rv, 1)
+
+    def order(start: int, count: int, sub: int) -> np.ndarray:
+        old = np.arange(start, start + count, dtype=np.int64)
+        dprime = ((deg[old] - sub) & 0xFFFFFFFF).astype(np.uint64)
+        return old[np.lexsort((old, dprime, degree_bucket(dprime)))]
+
+    left_nodes = order(0, n_left, 2)
+    right_nodes = order(n_left, n_right, 1)
+    gmap = np.empty(n, dtype=np.int64)
+    gmap[left_nodes] = np.arange(n_left)
+    gmap[right_nodes] = n_left + np.arange(n_right)
+    a = gmap[lu]
+    b = gmap[rv]
+    src = np.concatenate([a, b])
+    dst = np.concatenate([b, a])
+    o = np.lexsort((dst, src))
+    src, dst = src[o], dst[o]
+    off = np.zeros(n + 1, dtype=np.int64)
+    np.add.at(off, src + 1, 1)
+    off = np.cumsum(off)
+    return Graph(n_left, n, off.astype(np.uint32), dst.astype(np.uint32))
+
+
+def convert_file(path: str) -> Graph:
+    return convert_edges(*parse_edge_text(path))
+
+
+def write_edge_text(path: str, u: np.ndarray, v: np.ndarray) -> None:
+    """``u v`` per line (the converter's input format)."""
+    with open(path, "w") as f:
+        f.write("\n".join(f"{a} {b}" for a, b in zip(np.asarray(u).tolist(), np.asarray(v).tolist())))
+        f.write("\n")

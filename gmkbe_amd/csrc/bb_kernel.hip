@@ -1,0 +1,527 @@
+// Branch-and-bound state expansion for maximal k-biplex enumeration on gfx950.
+//
+// One wave64 expands one search state.  A state lives in HBM as bitsets over the
+// local vertex IDs of its partition:
+//     [header][S_L C_L X_L][S_R C_R X_R]      (each set W words of 64 bits)
+// S = vertices taken, C = candidates that can still be added, X = vertices that
+// can be added to S but are excluded from this subtree.  Invariant: for every
+// v in C or X, S + v is a k-biplex.  A state with C = X = {} and both sides of S
+// at least lb is a maximal k-biplex of its partition.
+//
+// The branching and pruning rules are those of BPPivot's branchNew
+// (baselines/MaximalBiPlex/src/biplex/biplex.cpp:781-1082): one-pass degree
+// pruning of C (biplex.cpp:695-713), the size bound, the non-degree bound
+// (biplex.cpp:840-866), the binary branch on a candidate missed by the S vertex
+// with most misses (biplex.cpp:912-965) and otherwise pivot branching
+// (biplex.cpp:970-1071).  The per-vertex counters BPPivot keeps incrementally
+// (degC, nonNbrS) are recomputed here from the bitsets with AND + popcount over
+// adjacency rows, which is what the wave does in parallel: lane i owns vertices
+// i, i+64, ...  Ties in every argmax/argmin go to the smallest local ID.
+//
+// Children of a state are written to a contiguous block claimed with one
+// atomicAdd; when the block would cross the buffer's end, the claimed slots
+// inside the buffer are marked as holes (skipped when read) and the parent is
+// written to the retry list unchanged, to be expanded again in a later batch.
+#include "gmk_device.h"
+
+namespace gmk {
+
+struct Stats {
+  u64 states, intersections, bytes, leaves;
+};
+
+template <int W>
+struct __align__(16) Smem {
+  u64 set[6 * W];         // S_L C_L X_L S_R C_R X_R
+  u64 full[2][W];         // members with ms >= k
+  u64 k1[2][W];           // members with ms == k - 1
+  u64 bad[2][W];          // S members with non-degree >= k (blocks X pivots)
+  u64 cand[W];            // pivot candidates (on side cand_side)
+  u64 sat[W];             // S members saturated by the include being built
+  u64 andsat[W];          // AND of the rows of sat
+  uint16_t deg[2][64 * W];
+  uint8_t ms[2][64 * W];
+};
+
+__device__ __forceinline__ u64* setp(u64* base, int side, int which, int W) {
+  return base + (side * 3 + which) * W;
+}
+
+// Stats for all members of side a: deg = |N(v) & C_b|, ms = |S_b \ N(v)|.
+template <int W>
+__device__ void side_stats(Smem<W>& sm, const PartDesc& P, const u64* rows, int a, Stats& st) {
+  const int lane = threadIdx.x;
+  const u32 na = a ? P.nr : P.nl;
+  const u32 wa = a ? P.wr : P.wl;
+  const u32 wb = a ? P.wl : P.wr;
+  const u64* rowsA = rows + (a ? P.rows_r : P.rows_l);
+  const u64* Sa = setp(sm.set, a, 0, W);
+  const u64* Ca = setp(sm.set, a, 1, W);
+  const u64* Xa = setp(sm.set, a, 2, W);
+  const u64* Sb = setp(sm.set, a ^ 1, 0, W);
+  const u64* Cb = setp(sm.set, a ^ 1, 1, W);
+  u32 tests = 0;
+  for (u32 i = 0; i < wa; ++i) {
+    const u32 v = i * 64 + lane;
+    const u64 memb = Sa[i] | Ca[i] | Xa[i];
+    const bool m = v < na && ((memb >> lane) & 1ull);
+    u32 d = 0, s = 0;
+    if (m) {
+      const u64* row = rowsA + (u64)v * wb;
+      for (u32 w = 0; w < wb; ++w) {
+        const u64 r = row[w];
+        d += __popcll(r & Cb[w]);
+        s += __popcll(Sb[w] & ~r);
+      }
+      ++tests;
+    }
+    sm.deg[a][v] = (uint16_t)d;
+    sm.ms[a][v] = (uint8_t)(s > 255 ? 255 : s);
+  }
+  st.intersections += tests;
+  st.bytes += (u64)tests * wb * 8;
+}
+
+// Recompute deg only (after C changed).
+template <int W>
+__device__ void side_deg(Smem<W>& sm, const PartDesc& P, const u64* rows, int a, Stats& st) {
+  const int lane = threadIdx.x;
+  const u32 na = a ? P.nr : P.nl;
+  const u32 wa = a ? P.wr : P.wl;
+  const u32 wb = a ? P.wl : P.wr;
+  const u64* rowsA = rows + (a ? P.rows_r : P.rows_l);
+  const u64* Sa = setp(sm.set, a, 0, W);
+  const u64* Ca = setp(sm.set, a, 1, W);
+  const u64* Xa = setp(sm.set, a, 2, W);
+  const u64* Cb = setp(sm.set, a ^ 1, 1, W);
+  u32 tests = 0;
+  for (u32 i = 0; i < wa; ++i) {
+    const u32 v = i * 64 + lane;
+    const u64 memb = Sa[i] | Ca[i] | Xa[i];
+    const bool m = v < na && ((memb >> lane) & 1ull);
+    u32 d = 0;
+    if (m) {
+      const u64* row = rowsA + (u64)v * wb;
+      for (u32 w = 0; w < wb; ++w) d += __popcll(row[w] & Cb[w]);
+      ++tests;
+    }
+    sm.deg[a][v] = (uint16_t)d;
+  }
+  st.intersections += tests;
+  st.bytes += (u64)tests * wb * 8;
+}
+
+template <int W>
+__device__ __forceinline__ u32 popc_set(const u64* s, u32 w) {
+  u32 c = 0;
+  for (u32 i = 0; i < w; ++i) c += __popcll(s[i]);
+  return c;
+}
+
+// Emit S as a result record (global IDs ascending) and fold its fingerprint.
+template <int W>
+__device__ void emit_state(Smem<W>& sm, const PartDesc& P, const u32* maps, const ExpandArgs& a) {
+  const int lane = threadIdx.x;
+  const u64* SL = setp(sm.set, 0, 0, W);
+  const u64* SR = setp(sm.set, 1, 0, W);
+  const u32 nl = popc_set<W>(SL, P.wl), nr = popc_set<W>(SR, P.wr);
+  u64 slot = 0;
+  if (lane == 0) slot = atomicAdd((unsigned long long*)a.res_count, 1ull);
+  slot = wave_bcast_u64(slot, 0);
+  if (slot >= a.limit) {
+    if (lane == 0) atomicOr(a.flags, FLAG_LIMIT);
+    return;
+  }
+  if (slot + 1 == a.limit && lane == 0) atomicOr(a.flags, FLAG_LIMIT);
+  const u64 len = 2ull + nl + nr;
+  u64 pos = 0;
+  if (a.keep && lane == 0) pos = atomicAdd((unsigned long long*)a.res_words, (unsigned long long)len);
+  pos = wave_bcast_u64(pos, 0);
+  const bool write = a.keep && pos + len <= a.res_cap;
+  if (a.keep && !write && lane == 0) atomicOr(a.flags, FLAG_RES_OVERFLOW);
+  u64 h = 0;
+  for (int side = 0; side < 2; ++side) {
+    const u64* Sx = side ? SR : SL;
+    const u32 wx = side ? P.wr : P.wl;
+    const u32* map = maps + (side ? P.map_r : P.map_l);
+    const u64 off = pos + 1 + (side ? 1 + nl : 0);
+    u32 before = 0;
+    for (u32 i = 0; i < wx; ++i) {
+      const u64 word = Sx[i];
+      if ((word >> lane) & 1ull) {
+        const u32 rank = before + __popcll(word & ((1ull << lane) - 1ull));
+        const u32 g = map[i * 64 + lane];
+        h += mix64(g);
+        if (write) a.res[off + rank] = g;
+      }
+      before += __popcll(word);
+    }
+  }
+  h = wave_sum_u64(h);
+  if (lane == 0) {
+    if (write) {
+      a.res[pos] = nl;
+      a.res[pos + 1 + nl] = nr;
+    }
+    const u64 fp = fmix64(h + (u64)nl * FP_A + (u64)nr * FP_B);
+    u64* fs = a.fp + (blockIdx.x % NSTRIPE) * 2;
+    atomicAdd((unsigned long long*)&fs[0], (unsigned long long)fp);
+    atomicXor((unsigned long long*)&fs[1], (unsigned long long)fp);
+  }
+}
+
+
+// Write one child.  The child is the parent with `moved` (on side ms_side)
+// moved from C to X, then (when inc != NONE) vertex inc of side us included.
+template <int W>
+__device__ void write_child(Smem<W>& sm, const PartDesc& P, const u64* rows, u32 k, u64* out, u32 pid,
+                            u32 inc, int us, int mv_side, const u64* moved /* W words or null */,
+                            u32 moved_single /* used when moved == null, NONE = none */) {
+  const int lane = threadIdx.x;
+  constexpr int NWS = 6 * W;
+  const int os = us ^ 1;
+  const u32 wos = os ? P.wr : P.wl;
+  const u64* rowU = nullptr;
+  bool ufull = false;
+  if (inc != NONE) {
+    rowU = rows + (us ? P.rows_r : P.rows_l) + (u64)inc * (us ? P.wl : P.wr);
+    ufull = sm.ms[us][inc] >= k;
+    // saturated S members of the other side: ms == k-1 and not adjacent to inc
+    if (lane < W) {
+      u64 s = 0;
+      if ((u32)lane < wos && k > 0) s = setp(sm.set, os, 0, W)[lane] & ~rowU[lane] & sm.k1[os][lane];
+      sm.sat[lane] = s;
+    }
+    wave_barrier();
+    // AND of the saturated members' rows (over side us)
+    const u32 wus = us ? P.wr : P.wl;
+    if (lane < W) {
+      u64 acc = ~0ull;
+      for (u32 i = 0; i < wos; ++i) {
+        u64 s = sm.sat[i];
+        while (s) {
+          const u32 b = __ffsll((long long)s) - 1;
+          s &= s - 1;
+          const u32 v = i * 64 + b;
+          if ((u32)lane < wus) acc &= rows[(os ? P.rows_r : P.rows_l) + (u64)v * wus + lane];
+        }
+      }
+      sm.andsat[lane] = acc;
+    }
+    wave_barrier();
+  }
+  if (lane == 0) out[0] = pid;
+  for (int j = lane; j < NWS; j += 64) {
+    const int side = j / (3 * W);
+    const int which = (j / W) % 3;
+    const int w = j % W;
+    u64 val = sm.set[j];
+    // moves C -> X
+    if (side == mv_side && which != 0) {
+      u64 mv = 0;
+      if (moved) mv = moved[w];
+      else if (moved_single != NONE && (moved_single >> 6) == (u32)w) mv = 1ull << (moved_single & 63);
+      val = which == 1 ? (val & ~mv) : (val | mv);
+    }
+    if (inc != NONE) {
+      if (side == os && which != 0) {
+        const u64 r = (u32)w < wos ? rowU[w] : 0;
+        const u64 kill = ~r & (ufull ? ~0ull : sm.full[os][w]);
+        val &= ~kill;
+      }
+      if (side == us) {
+        const u64 bit = ((inc >> 6) == (u32)w) ? (1ull << (inc & 63)) : 0;
+        if (which == 0) val |= bit;
+        else val &= ~bit & sm.andsat[w];
+      }
+    }
+    out[1 + j] = val;
+  }
+}
+
+template <int W>
+__global__ __launch_bounds__(64) void bb_expand(ExpandArgs a) {
+  __shared__ Smem<W> sm;
+  const int lane = threadIdx.x;
+  const u32 sidx = blockIdx.x;
+  if (sidx >= a.n_in) return;
+  constexpr int STRIDE = 1 + 6 * W;
+  const u64* in = a.in + (u64)sidx * STRIDE;
+  const u32 pid = (u32)in[0];
+  if (pid == INVALID_PID) return;  // hole left by a child block that did not fit
+  const PartDesc P = a.parts[pid];
+  const u32 k = a.k;
+  const u32 lb = P.lb;
+  Stats st = {0, 0, 0, 0};
+  for (int j = lane; j < 6 * W; j += 64) sm.set[j] = in[1 + j];
+  wave_barrier();
+  if (__hip_atomic_load(a.flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & FLAG_LIMIT) return;
+  st.states = 1;
+  st.bytes += STRIDE * 8;
+
+  side_stats<W>(sm, P, a.rows, 0, st);
+  side_stats<W>(sm, P, a.rows, 1, st);
+  wave_barrier();
+
+  u32 nS[2], nC[2], nX[2];
+  auto count_sets = [&]() {
+    nS[0] = popc_set<W>(setp(sm.set, 0, 0, W), P.wl);
+    nC[0] = popc_set<W>(setp(sm.set, 0, 1, W), P.wl);
+    nX[0] = popc_set<W>(setp(sm.set, 0, 2, W), P.wl);
+    nS[1] = popc_set<W>(setp(sm.set, 1, 0, W), P.wr);
+    nC[1] = popc_set<W>(setp(sm.set, 1, 1, W), P.wr);
+    nX[1] = popc_set<W>(setp(sm.set, 1, 2, W), P.wr);
+  };
+  count_sets();
+
+  // One-pass degree pruning of C (biplex.cpp:695-713).  Decisions use the
+  // degrees before any removal; removal only lowers degrees, so every removed
+  // vertex is one the sequential pass would also be allowed to remove.
+  bool pruned_any = false;
+  for (int a_ = 0; a_ < 2; ++a_) {
+    const u32 na = a_ ? P.nr : P.nl, wa = a_ ? P.wr : P.wl;
+    u64* Ca = setp(sm.set, a_, 1, W);
+    for (u32 i = 0; i < wa; ++i) {
+      const u32 v = i * 64 + lane;
+      bool inC = v < na && ((Ca[i] >> lane) & 1ull);
+      bool rm = false;
+      if (inC) {
+        const int64_t d = (int64_t)sm.deg[a_][v] + ((int64_t)nS[a_ ^ 1] - (int64_t)sm.ms[a_][v]) + (int64_t)k;
+        rm = d < (int64_t)lb;
+      }
+      const u64 rmw = __ballot(rm);
+      wave_barrier();
+      if (rmw) {
+        if (lane == 0) Ca[i] &= ~rmw;
+        pruned_any = true;
+      }
+      wave_barrier();
+    }
+  }
+  if (pruned_any) {
+    count_sets();
+    side_deg<W>(sm, P, a.rows, 0, st);
+    side_deg<W>(sm, P, a.rows, 1, st);
+    wave_barrier();
+  }
+
+  u32 nchild = 0;
+  bool emit = false;
+  // kind: 0 = none, 1 = binary (include u / exclude u), 2 = pivot
+  int kind = 0;
+  u32 bin_u = NONE;
+  int bin_side = 0;
+  u32 pivot = NONE;
+  int pside = 0;
+  bool pivot_in_c = false;
+
+  if (nS[0] + nC[0] < lb || nS[1] + nC[1] < lb) goto finish;
+  if (nC[0] == 0 && nC[1] == 0) {
+    if (nX[0] == 0 && nX[1] == 0 && nS[0] >= lb && nS[1] >= lb) emit = true;
+    st.leaves = 1;
+    goto finish;
+  }
+  {
+    // Non-degree of S members: |C_b| - deg + ms (biplex.cpp:847-866).
+    int64_t nd[2];
+    u32 sp[2];
+    for (int a_ = 0; a_ < 2; ++a_) {
+      const u32 na = a_ ? P.nr : P.nl, wa = a_ ? P.wr : P.wl;
+      const u64* Sa = setp(sm.set, a_, 0, W);
+      u64 best = 0;  // packed (nd + 1) << 32 | ~id ; 0 = none
+      for (u32 i = 0; i < wa; ++i) {
+        const u32 v = i * 64 + lane;
+        const bool m = v < na && ((Sa[i] >> lane) & 1ull);
+        bool bad = false;
+        if (m) {
+          const u32 d = nC[a_ ^ 1] - sm.deg[a_][v] + sm.ms[a_][v];
+          const u64 key = ((u64)(d + 1) << 32) | (u64)(0xFFFFFFFFu - v);
+          best = key > best ? key : best;
+          bad = d >= k;
+        }
+        const u64 bw = __ballot(bad);
+        if (lane == 0) sm.bad[a_][i] = bw;
+      }
+      best = wave_max_u64(best);
+      if (best == 0) { nd[a_] = INT64_MIN / 4; sp[a_] = NONE; }
+      else { nd[a_] = (int64_t)(best >> 32) - 1; sp[a_] = 0xFFFFFFFFu - (u32)(best & 0xFFFFFFFFu); }
+      if (nS[a_] > 0 && (int64_t)(nS[a_ ^ 1] + nC[a_ ^ 1]) - nd[a_] < (int64_t)lb - (int64_t)k) goto finish;
+    }
+    // full / k1 bitsets over all members (used by the include filter)
+    for (int a_ = 0; a_ < 2; ++a_) {
+      const u32 na = a_ ? P.nr : P.nl, wa = a_ ? P.wr : P.wl;
+      for (u32 i = 0; i < (u32)W; ++i) {
+        const u32 v = i * 64 + lane;
+        const bool ok = i < wa && v < na;
+        const u32 m = ok ? sm.ms[a_][v] : 0;
+        const u64 f = __ballot(ok && m >= k);
+        const u64 q = __ballot(ok && k > 0 && m == k - 1);
+        if (lane == 0) { sm.full[a_][i] = f; sm.k1[a_][i] = q; }
+      }
+    }
+    wave_barrier();
+
+    if (nd[0] > (int64_t)k || nd[1] > (int64_t)k) {
+      // Binary branch (biplex.cpp:912-965).
+      const int side = nd[0] > nd[1] ? 0 : 1;
+      const int os = side ^ 1;
+      const u32 n_os = os ? P.nr : P.nl, w_os = os ? P.wr : P.wl;
+      const u64* rowSp = a.rows + (side ? P.rows_r : P.rows_l) + (u64)sp[side] * w_os;
+      const u64* Cos = setp(sm.set, os, 1, W);
+      u64 best = ~0ull;  // minimise (d, 255 - ms, id)
+      for (u32 i = 0; i < w_os; ++i) {
+        const u32 v = i * 64 + lane;
+        const u64 r = rowSp[i];
+        const bool c = v < n_os && ((Cos[i] >> lane) & 1ull) && !((r >> lane) & 1ull);
+        if (c) {
+          const u32 d = sm.deg[os][v] + nS[side] - sm.ms[os][v];
+          const u64 key = ((u64)d << 40) | ((u64)(255 - sm.ms[os][v]) << 32) | v;
+          best = key < best ? key : best;
+        }
+      }
+      best = wave_min_u64(best);
+      if (best != ~0ull) {
+        kind = 1;
+        bin_u = (u32)(best & 0xFFFFFFFFu);
+        bin_side = os;
+        nchild = 2;
+      }
+    } else {
+      // Pivot branching (biplex.cpp:970-1071).
+      int64_t dp[2];
+      u32 pv[2];
+      bool pin[2];
+      for (int a_ = 0; a_ < 2; ++a_) {
+        const u32 na = a_ ? P.nr : P.nl, wa = a_ ? P.wr : P.wl;
+        const u32 wb = a_ ? P.wl : P.wr;
+        const u64* Ca = setp(sm.set, a_, 1, W);
+        const u64* Xa = setp(sm.set, a_, 2, W);
+        const u64* rowsA = a.rows + (a_ ? P.rows_r : P.rows_l);
+        u64 bc = 0, bx = 0;  // (deg + 1) << 32 | ~id
+        for (u32 i = 0; i < wa; ++i) {
+          const u32 v = i * 64 + lane;
+          if (v >= na) continue;
+          if ((Ca[i] >> lane) & 1ull) {
+            const u64 key = ((u64)(sm.deg[a_][v] + 1) << 32) | (u64)(0xFFFFFFFFu - v);
+            bc = key > bc ? key : bc;
+          }
+          if ((Xa[i] >> lane) & 1ull) {
+            const u64* row = rowsA + (u64)v * wb;
+            bool elig = true;
+            for (u32 w = 0; w < wb; ++w)
+              if (sm.bad[a_ ^ 1][w] & ~row[w]) { elig = false; break; }
+            if (elig) {
+              const u64 key = ((u64)(sm.deg[a_][v] + 1) << 32) | (u64)(0xFFFFFFFFu - v);
+              bx = key > bx ? key : bx;
+            }
+          }
+        }
+        bc = wave_max_u64(bc);
+        bx = wave_max_u64(bx);
+        // an X vertex wins only with a strictly larger degree
+        if (bx != 0 && (bc == 0 || (bx >> 32) > (bc >> 32))) {
+          dp[a_] = (int64_t)(bx >> 32) - 1; pv[a_] = 0xFFFFFFFFu - (u32)(bx & 0xFFFFFFFFu); pin[a_] = false;
+        } else if (bc != 0) {
+          dp[a_] = (int64_t)(bc >> 32) - 1; pv[a_] = 0xFFFFFFFFu - (u32)(bc & 0xFFFFFFFFu); pin[a_] = true;
+        } else {
+          dp[a_] = -1; pv[a_] = NONE; pin[a_] = false;
+        }
+      }
+      pside = ((int64_t)nC[1] - dp[0] < (int64_t)nC[0] - dp[1]) ? 0 : 1;
+      pivot = pv[pside];
+      pivot_in_c = pin[pside];
+      if (pivot != NONE) {
+        const int os = pside ^ 1;
+        const u32 w_os = os ? P.wr : P.wl;
+        const u64* rowP = a.rows + (pside ? P.rows_r : P.rows_l) + (u64)pivot * w_os;
+        const u64* Cos = setp(sm.set, os, 1, W);
+        if (lane < W) sm.cand[lane] = (u32)lane < w_os ? (Cos[lane] & ~rowP[lane]) : 0;
+        wave_barrier();
+        kind = 2;
+        nchild = popc_set<W>(sm.cand, w_os) + (pivot_in_c ? 1 : 0);
+      }
+    }
+  }
+
+finish:
+  if (emit) emit_state<W>(sm, P, a.maps, a);
+  if (nchild > 0) {
+    u32 base = 0;
+    if (lane == 0) base = atomicAdd(a.out_n, nchild);
+    base = (u32)wave_bcast_u64(base, 0);
+    if ((u64)base + nchild > a.out_cap) {
+      // no room: mark the claimed slots below the cap as holes and requeue the
+      // parent unchanged
+      for (u32 s2 = base + lane; s2 < a.out_cap && s2 < base + nchild; s2 += 64)
+        a.out[(u64)s2 * STRIDE] = INVALID_PID;
+      // no room: requeue the parent unchanged
+      u32 r = 0;
+      if (lane == 0) r = atomicAdd(a.retry_n, 1u);
+      r = (u32)wave_bcast_u64(r, 0);
+      u64* o = a.retry + (u64)r * STRIDE;
+      for (int j = lane; j < STRIDE; j += 64) o[j] = in[j];
+      st.bytes += STRIDE * 8;
+    } else if (kind == 1) {
+      u64* o = a.out + (u64)base * STRIDE;
+      write_child<W>(sm, P, a.rows, k, o, pid, bin_u, bin_side, bin_side, nullptr, NONE);
+      wave_barrier();
+      write_child<W>(sm, P, a.rows, k, o + STRIDE, pid, NONE, bin_side, bin_side, nullptr, bin_u);
+      st.bytes += 2ull * STRIDE * 8;
+    } else {
+      const int os = pside ^ 1;
+      const u32 w_os = os ? P.wr : P.wl;
+      // running prefix of candidates already branched on (moved to X)
+      __shared__ u64 prefix[W];
+      if (lane < W) prefix[lane] = 0;
+      wave_barrier();
+      u32 c = 0;
+      for (u32 i = 0; i < w_os; ++i) {
+        u64 s = sm.cand[i];
+        while (s) {
+          const u32 b = __ffsll((long long)s) - 1;
+          s &= s - 1;
+          const u32 u = i * 64 + b;
+          write_child<W>(sm, P, a.rows, k, a.out + (u64)(base + c) * STRIDE, pid, u, os, os, prefix, NONE);
+          wave_barrier();
+          if (lane == 0) prefix[i] |= 1ull << b;
+          wave_barrier();
+          ++c;
+        }
+      }
+      if (pivot_in_c) {
+        write_child<W>(sm, P, a.rows, k, a.out + (u64)(base + c) * STRIDE, pid, pivot, pside, os, sm.cand, NONE);
+        ++c;
+      }
+      st.bytes += (u64)c * STRIDE * 8;
+    }
+  }
+  const u64 tsum = wave_sum_u64(st.intersections);
+  const u64 bsum = wave_sum_u64(st.bytes);
+  if (lane == 0) {
+    u64* sp = a.stat + (sidx % NSTRIPE) * 4;
+    atomicAdd((unsigned long long*)&sp[0], (unsigned long long)st.states);
+    atomicAdd((unsigned long long*)&sp[1], (unsigned long long)tsum);
+    atomicAdd((unsigned long long*)&sp[2], (unsigned long long)st.leaves);
+    atomicAdd((unsigned long long*)&sp[3], (unsigned long long)bsum);
+  }
+}
+
+
+}  // namespace gmk
+
+namespace gmk {
+// Host-side launcher for the expansion kernel (one wave per state).
+hipError_t launch_bb_expand(int W, u32 n_states, hipStream_t s, const ExpandArgs& a) {
+  if (n_states == 0) return hipSuccess;
+  dim3 grid(n_states), block(64);
+  switch (W) {
+    case 1: hipLaunchKernelGGL(bb_expand<1>, grid, block, 0, s, a); break;
+    case 2: hipLaunchKernelGGL(bb_expand<2>, grid, block, 0, s, a); break;
+    case 4: hipLaunchKernelGGL(bb_expand<4>, grid, block, 0, s, a); break;
+    case 8: hipLaunchKernelGGL(bb_expand<8>, grid, block, 0, s, a); break;
+    case 16: hipLaunchKernelGGL(bb_expand<16>, grid, block, 0, s, a); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+}  // namespace gmk

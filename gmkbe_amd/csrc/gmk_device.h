@@ -1,0 +1,99 @@
+// Shared device-side definitions of the enumerator's HIP kernels (gfx950, wave64).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace gmk {
+
+typedef uint32_t u32;
+typedef uint64_t u64;
+
+constexpr u32 NONE = 0xFFFFFFFFu;
+constexpr u32 INVALID_PID = 0xFFFFFFFFu;
+constexpr u32 FLAG_LIMIT = 1u;
+constexpr u32 FLAG_RES_OVERFLOW = 2u;
+constexpr u32 FLAG_TABLE_FULL = 4u;
+constexpr int NSTRIPE = 64;  // striped counters (stat and fingerprint atomics)
+
+// Fingerprint constants: fp(L, R) = fmix64(sum mix64(id) + |L| A + |R| B).
+constexpr u64 FP_A = 0x2545F4914F6CDD1Dull;
+constexpr u64 FP_B = 0x9E6C63D0676A9A99ull;
+
+__host__ __device__ __forceinline__ u64 mix64(u64 x) {
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+__host__ __device__ __forceinline__ u64 fmix64(u64 h) {
+  h ^= h >> 33; h *= 0xff51afd7ed558ccdull;
+  h ^= h >> 33; h *= 0xc4ceb9fe1a85ec53ull;
+  h ^= h >> 33; return h;
+}
+
+// A partition: a local universe of nl left and nr right vertices (local IDs
+// ordered like their global IDs) with bitset adjacency rows in the row pool.
+struct PartDesc {
+  u32 nl, nr;   // local universe sizes
+  u32 wl, wr;   // 64-bit words per left / right bitset
+  u64 rows_l;   // word offset in the row pool of the nl left rows (wr words each)
+  u64 rows_r;   // word offset of the nr right rows (wl words each)
+  u64 map_l;    // u32 offset in the map pool: local -> global, left
+  u64 map_r;    // same, right
+  u32 lb;       // both result sides must reach lb
+  u32 seed;     // seed vertex (global ID) or NONE for a global partition
+};
+
+struct ExpandArgs {
+  const u64* in;        // input states (stride 1 + 6W words)
+  u32 n_in;
+  u64* out;             // children
+  u32 out_cap;
+  u32* out_n;
+  u64* retry;           // parents whose children did not fit
+  u32* retry_n;
+  const PartDesc* parts;
+  const u64* rows;
+  const u32* maps;
+  u32 k;
+  int keep;             // write result records
+  u32* res;             // result record stream
+  u64 res_cap;          // capacity in words
+  u64* res_words;       // words used
+  u64* res_count;       // results emitted (claims; may exceed limit)
+  u64 limit;
+  u64* fp;              // [NSTRIPE][2] fingerprint sum / xor
+  u64* stat;            // [NSTRIPE][4] states, intersections, leaves, bytes
+  u32* flags;
+};
+
+__device__ __forceinline__ void wave_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+__device__ __forceinline__ u64 shfl_xor_u64(u64 v, int m) {
+  const u32 lo = __shfl_xor((int)(u32)v, m, 64);
+  const u32 hi = __shfl_xor((int)(u32)(v >> 32), m, 64);
+  return ((u64)hi << 32) | lo;
+}
+__device__ __forceinline__ u64 wave_bcast_u64(u64 v, int src) {
+  const u32 lo = __shfl((int)(u32)v, src, 64);
+  const u32 hi = __shfl((int)(u32)(v >> 32), src, 64);
+  return ((u64)hi << 32) | lo;
+}
+__device__ __forceinline__ u64 wave_sum_u64(u64 v) {
+  for (int m = 32; m > 0; m >>= 1) v += shfl_xor_u64(v, m);
+  return v;
+}
+__device__ __forceinline__ u64 wave_max_u64(u64 v) {
+  for (int m = 32; m > 0; m >>= 1) { const u64 o = shfl_xor_u64(v, m); v = o > v ? o : v; }
+  return v;
+}
+__device__ __forceinline__ u64 wave_min_u64(u64 v) {
+  for (int m = 32; m > 0; m >>= 1) { const u64 o = shfl_xor_u64(v, m); v = o < v ? o : v; }
+  return v;
+}
+
+}  // namespace gmk

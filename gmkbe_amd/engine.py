@@ -1,0 +1,150 @@
+"""Python host interface over the C ABI of libgmk.so (include/gmk.h).
+
+Mirrors the reference's process interface for the enumeration path: a graph in
+.graph.bin CSR form, k (-k), a result limit (-r) and a device (-d), producing the
+set of maximal k-biplexes with both sides at least theta.  The HIP library is
+required: importing succeeds without it, but every call raises when the library
+is missing or no GPU is present -- there is no CPU path.
+"""
+from __future__ import annotations
+
+import ctypes
+import dataclasses
+import os
+
+import numpy as np
+
+from .graphio import Graph
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "_lib", "libgmk.so")
+
+MODES = {"plain": 0, "batched": 1, "learned": 2}
+PARTS = {"auto": 0, "global": 1, "seeded": 2}
+STATUS = {0: "ok", -1: "invalid argument", -2: "io error", -3: "device error", -4: "out of device memory",
+          -5: "no graph", -6: "overflow", -7: "unsupported"}
+
+
+class Params(ctypes.Structure):
+    _fields_ = [("k", ctypes.c_uint32), ("theta", ctypes.c_uint32), ("limit", ctypes.c_uint64),
+                ("mode", ctypes.c_int32), ("partition", ctypes.c_int32), ("keep_results", ctypes.c_int32),
+                ("verbose", ctypes.c_int32), ("result_capacity_words", ctypes.c_uint64)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("count", ctypes.c_uint64), ("fp_sum", ctypes.c_uint64), ("fp_xor", ctypes.c_uint64),
+                ("states", ctypes.c_uint64), ("intersections", ctypes.c_uint64), ("algo_bytes", ctypes.c_uint64),
+                ("partitions", ctypes.c_uint64), ("launches", ctypes.c_uint64), ("dedup_inserts", ctypes.c_uint64),
+                ("dedup_probes", ctypes.c_uint64), ("dedup_dups", ctypes.c_uint64), ("ms", ctypes.c_double),
+                ("ms_expand", ctypes.c_double), ("hit_limit", ctypes.c_int32), ("pad", ctypes.c_int32)]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_ if name != "pad"}
+
+
+class GmkError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def load_library():
+    """Loads libgmk.so; raises GmkError when it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise GmkError(f"HIP library not built: {LIB_PATH} (run __graft_entry__.build())")
+    L = ctypes.CDLL(LIB_PATH)
+    P = ctypes.POINTER
+    u32p = P(ctypes.c_uint32)
+    L.gmk_create.argtypes = [ctypes.c_int, P(ctypes.c_void_p)]
+    L.gmk_create.restype = ctypes.c_int
+    L.gmk_destroy.argtypes = [ctypes.c_void_p]
+    L.gmk_destroy.restype = None
+    L.gmk_last_error.argtypes = [ctypes.c_void_p]
+    L.gmk_last_error.restype = ctypes.c_char_p
+    L.gmk_load_graph_bin.argtypes = [ctypes.c_void_p, ctypes.c_char_p, u32p, u32p, P(ctypes.c_uint64)]
+    L.gmk_load_graph_bin.restype = ctypes.c_int
+    L.gmk_set_graph_csr.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, u32p, u32p]
+    L.gmk_set_graph_csr.restype = ctypes.c_int
+    L.gmk_enumerate.argtypes = [ctypes.c_void_p, P(Params), P(Stats)]
+    L.gmk_enumerate.restype = ctypes.c_int
+    L.gmk_fetch_results.argtypes = [ctypes.c_void_p, u32p, ctypes.c_uint64, P(ctypes.c_uint64)]
+    L.gmk_fetch_results.restype = ctypes.c_int
+    _lib = L
+    return L
+
+
+@dataclasses.dataclass
+class Result:
+    stats: dict
+    words: np.ndarray | None  # record stream [|L|, L.., |R|, R..] per result
+
+    @property
+    def count(self) -> int:
+        return int(self.stats["count"])
+
+
+class Engine:
+    """One device context (gmk_create).  Not thread-safe; one run at a time."""
+
+    def __init__(self, device: int = 0):
+        self._L = load_library()
+        h = ctypes.c_void_p()
+        rc = self._L.gmk_create(device, ctypes.byref(h))
+        if rc != 0:
+            raise GmkError(f"gmk_create(device={device}) failed: {STATUS.get(rc, rc)}")
+        self._h = h
+        self.graph: Graph | None = None
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._L.gmk_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, rc: int, what: str) -> None:
+        if rc != 0:
+            msg = self._L.gmk_last_error(self._h)
+            raise GmkError(f"{what}: {STATUS.get(rc, rc)}: {msg.decode() if msg else ''}")
+
+    def set_graph(self, g: Graph) -> None:
+        off = np.ascontiguousarray(g.off, dtype=np.uint32)
+        adj = np.ascontiguousarray(g.adj, dtype=np.uint32)
+        p = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))
+        self._check(self._L.gmk_set_graph_csr(self._h, g.n, g.n_left, p(off), p(adj)), "gmk_set_graph_csr")
+        self.graph = g
+
+    def load_graph_bin(self, path: str) -> tuple[int, int, int]:
+        n, nl, nnz = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint64()
+        self._check(self._L.gmk_load_graph_bin(self._h, path.encode(), ctypes.byref(n), ctypes.byref(nl),
+                                               ctypes.byref(nnz)), f"gmk_load_graph_bin({path})")
+        return n.value, nl.value, nnz.value
+
+    def enumerate(self, k: int = 1, theta: int = 1, limit: int = 0, mode: str = "plain",
+                  partition: str = "auto", keep: bool = True, capacity_words: int = 0) -> Result:
+        prm = Params(k, theta, limit, MODES[mode], PARTS[partition], int(keep), 0, capacity_words)
+        st = Stats()
+        self._check(self._L.gmk_enumerate(self._h, ctypes.byref(prm), ctypes.byref(st)), "gmk_enumerate")
+        words = None
+        if keep:
+            nw = ctypes.c_uint64()
+            self._check(self._L.gmk_fetch_results(self._h, None, 0, ctypes.byref(nw)), "gmk_fetch_results")
+            words = np.empty(int(nw.value), dtype=np.uint32)
+            if nw.value:
+                self._check(self._L.gmk_fetch_results(self._h, words.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
+                                                      nw.value, ctypes.byref(nw)), "gmk_fetch_results")
+        return Result(st.as_dict(), words)

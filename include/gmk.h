@@ -1,0 +1,104 @@
+/* gmk.h - C ABI of the MI355X maximal k-biplex enumerator (libgmk.so).
+ *
+ * The reference (GMkBPE) has no plugin or FFI layer: its boundary is the
+ * `gmkbpe` process (CLI, .graph.bin input, result lines and time records) and
+ * the internal seam enumerate_GPU / Generate_GPU.  Each entry point below names
+ * the reference code it replaces.  All calls return GMK_OK or a negative
+ * status; nothing calls exit().  A context owns one device and its buffers;
+ * calls on one context must not overlap (different contexts may run in
+ * parallel on different threads).
+ */
+#ifndef GMK_H
+#define GMK_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  GMK_OK = 0,
+  GMK_EINVAL = -1,     /* bad argument */
+  GMK_EIO = -2,        /* file missing / malformed .graph.bin */
+  GMK_EDEVICE = -3,    /* HIP runtime error (message in gmk_last_error) */
+  GMK_ENOMEM = -4,     /* device allocation failed */
+  GMK_ENOGRAPH = -5,   /* no graph loaded */
+  GMK_EOVERFLOW = -6,  /* a fixed-capacity buffer overflowed (message says which) */
+  GMK_EUNSUPPORTED = -7
+};
+
+/* Work order of the enumeration.
+ *   GMK_MODE_PLAIN   one partition per launch sequence (the reference's
+ *                    one-candidate-at-a-time structure, src/main.cu:1078-1506)
+ *   GMK_MODE_BATCHED all partitions share one device work stack drained by one
+ *                    persistent launch with work stealing
+ *   GMK_MODE_LEARNED as BATCHED, partitions ordered and split by the cost model
+ *                    loaded with gmk_set_cost_model (falls back to the degree
+ *                    heuristic when none is loaded) */
+enum { GMK_MODE_PLAIN = 0, GMK_MODE_BATCHED = 1, GMK_MODE_LEARNED = 2 };
+
+/* Partition scheme.
+ *   GMK_PART_AUTO    global partition when both sides fit one bitset tile,
+ *                    else seeded
+ *   GMK_PART_GLOBAL  the whole graph is one partition (small graphs)
+ *   GMK_PART_SEEDED  per-seed 2-hop partitions for the results with both sides
+ *                    >= 2k+1, plus the thin enumerators for the rest */
+enum { GMK_PART_AUTO = 0, GMK_PART_GLOBAL = 1, GMK_PART_SEEDED = 2 };
+
+typedef struct gmk_ctx gmk_ctx;
+
+typedef struct {
+  uint32_t k;            /* -k, default 1 (src/main.cu:1726) */
+  uint32_t theta;        /* size lower bound on both sides, default 1 (src/main.cu:1585) */
+  uint64_t limit;        /* -r, 0 = no limit (src/main.cu:1721) */
+  int32_t mode;          /* GMK_MODE_* */
+  int32_t partition;     /* GMK_PART_* */
+  int32_t keep_results;  /* 1: keep the record stream for gmk_fetch_results */
+  int32_t verbose;       /* -m */
+  uint64_t result_capacity_words; /* 0 = automatic */
+} gmk_params;
+
+typedef struct {
+  uint64_t count;          /* results emitted (== limit when the limit was hit) */
+  uint64_t fp_sum, fp_xor; /* order-free digest of the emitted set */
+  uint64_t states;         /* search states expanded */
+  uint64_t intersections;  /* bitset row intersections (AND + popcount) */
+  uint64_t algo_bytes;     /* algorithmic bytes read + written by the kernels */
+  uint64_t partitions;     /* partitions built */
+  uint64_t launches;       /* kernel launches */
+  uint64_t dedup_inserts, dedup_probes, dedup_dups;
+  double ms;               /* enumeration window (after graph upload) */
+  double ms_expand;        /* device time inside the expansion kernel */
+  int32_t hit_limit;
+  int32_t pad;
+} gmk_stats;
+
+/* Create a context on HIP device `device` (-d, src/main.cu:1765-1766). */
+int gmk_create(int device, gmk_ctx** out);
+void gmk_destroy(gmk_ctx* ctx);
+const char* gmk_last_error(gmk_ctx* ctx);
+
+/* Load a .graph.bin file and upload its CSR (replaces load_graph_from_bin,
+ * data/src/tmp_load.h:213-295, and the H2D copies at src/main.cu:1680-1690). */
+int gmk_load_graph_bin(gmk_ctx* ctx, const char* path, uint32_t* n, uint32_t* n_left, uint64_t* nnz);
+
+/* Upload a CSR held by the caller (copied; the caller keeps ownership).
+ * off has n+1 entries; vertices [0, n_left) are left, [n_left, n) right;
+ * neighbour lists must be sorted ascending and point to the other side. */
+int gmk_set_graph_csr(gmk_ctx* ctx, uint32_t n, uint32_t n_left, const uint32_t* off, const uint32_t* adj);
+
+/* Enumerate maximal k-biplexes (L, R) with |L|, |R| >= theta (replaces
+ * enumerate_GPU, src/main.cu:999-1580, with Generate_GPU and its callees). */
+int gmk_enumerate(gmk_ctx* ctx, const gmk_params* params, gmk_stats* stats);
+
+/* Copy the record stream of the last gmk_enumerate run with keep_results = 1:
+ * u32 words [|L|, L ascending, |R|, R ascending] per result, global IDs,
+ * records in emission order.  *nwords receives the stream length; when
+ * cap_words is smaller nothing is copied and GMK_EOVERFLOW is returned. */
+int gmk_fetch_results(gmk_ctx* ctx, uint32_t* words, uint64_t cap_words, uint64_t* nwords);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GMK_H */

@@ -1,0 +1,146 @@
+"""Regenerates the golden fixtures in tests/golden/ from the reference's own CPU
+tools, compiled from their sources by oracle/ref.mk into oracle/_ref/.
+
+Run in the build container only (it reads /root/reference and runs the
+reference binaries there):
+
+    make -f oracle/ref.mk && python tests/golden/make_golden.py
+
+Outputs (data only):
+  graphs/<name>.graph.bin   written by the reference converter (graph_convert)
+                            from the edge list this script writes
+  golden.json               per (graph, k, theta): result count and digests of
+                            the reference BPPivot's printed result set
+  sets/<case>.u32           the full record stream for small cases
+"""
+from __future__ import annotations
+
+import json
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+from gmkbe_amd import convert, gen, graphio  # noqa: E402
+import oracle_lib  # noqa: E402
+
+REF = "/root/reference"
+BIN = os.path.join(ROOT, "oracle", "_ref")
+
+
+def ref_convert(name: str, u: np.ndarray, v: np.ndarray) -> str:
+    os.makedirs(os.path.join(HERE, "graphs"), exist_ok=True)
+    out = os.path.join(HERE, "graphs", name)
+    with tempfile.TemporaryDirectory() as td:
+        txt = os.path.join(td, name + ".txt")
+        convert.write_edge_text(txt, u, v)
+        subprocess.run([os.path.join(BIN, "graph_convert"), "-i", txt, "-o", out], check=True,
+                       capture_output=True)
+    return out + ".graph.bin"
+
+
+def strip_header_edges(path: str) -> tuple[np.ndarray, np.ndarray]:
+    """BPPivot's text fixtures start with a count line; the converter would read
+    it as an edge, so it is dropped."""
+    lines = open(path).read().split("\n")[1:]
+    us, vs = [], []
+    for ln in lines:
+        p = ln.split()
+        if len(p) >= 2:
+            us.append(int(p[0])); vs.append(int(p[1]))
+    return np.asarray(us, dtype=np.uint32), np.asarray(vs, dtype=np.uint32)
+
+
+def example_graph_edges(path: str) -> tuple[np.ndarray, np.ndarray]:
+    """SGMD22-MBPE/Example.graph: header `N nLeft m`, then one adjacency line per
+    vertex with global IDs.  Edges are taken from the left vertices' lines."""
+    rows = [list(map(int, ln.split())) for ln in open(path) if ln.strip()]
+    n, n_left = rows[0][0], rows[0][1]
+    us, vs = [], []
+    for r in rows[1:]:
+        if r[0] < n_left:
+            for w in r[1:]:
+                us.append(r[0]); vs.append(w)
+    return np.asarray(us, dtype=np.uint32), np.asarray(vs, dtype=np.uint32)
+
+
+def run_bppivot(graph_path: str, k: int, theta: int) -> np.ndarray:
+    """Runs the reference BPPivot with -o 1 and returns its result set as a
+    record stream with global IDs (right labels are id - nLeft in its output:
+    load_bipartite.hpp:58)."""
+    g = graphio.read_graph_bin(graph_path)
+    with tempfile.TemporaryDirectory() as td:
+        gp = os.path.join(td, "g.graph.bin")
+        with open(graph_path, "rb") as f, open(gp, "wb") as o:
+            o.write(f.read())
+        out = os.path.join(td, "out.txt")
+        subprocess.run(f"ulimit -s unlimited; {os.path.join(BIN, 'bppivot')} -d {gp} -q {theta} -k {k} -o 1 > {out}",
+                       shell=True, check=True, timeout=600)
+        parts: list[np.ndarray] = []
+        L = None
+        with open(out) as f:
+            for ln in f:
+                s = ln.strip()
+                if s.startswith("SL:"):
+                    L = np.sort(np.array(s[3:].replace(",", " ").split(), dtype=np.uint32))
+                elif s.startswith("SR:"):
+                    R = np.sort(np.array(s[3:].replace(",", " ").split(), dtype=np.uint32)) + np.uint32(g.n_left)
+                    parts.append(np.concatenate([[len(L)], L, [len(R)], R]).astype(np.uint32))
+    return np.concatenate(parts) if parts else np.zeros(0, dtype=np.uint32)
+
+
+def main() -> None:
+    graphs = {}
+    graphs["southern_women"] = ref_convert(
+        "southern_women", *strip_header_edges(f"{REF}/baselines/MaximalBiPlex/datas/opsahl_southernwomen.txt"))
+    graphs["moreno_crime"] = ref_convert(
+        "moreno_crime", *strip_header_edges(f"{REF}/baselines/MaximalBiPlex/datas/moreno_crime.txt"))
+    graphs["example"] = ref_convert("example", *example_graph_edges(f"{REF}/baselines/SGMD22-MBPE/Example.graph"))
+    graphs["er10x12"] = ref_convert("er10x12", *gen.er_edges(10, 12, 0.3, 11))
+    graphs["er12x10"] = ref_convert("er12x10", *gen.er_edges(12, 10, 0.45, 12))
+    graphs["er40x50"] = ref_convert("er40x50", *gen.er_edges(40, 50, 0.1, 13))
+    graphs["er200x300"] = ref_convert("er200x300", *gen.er_edges(200, 300, 0.05, gen.SEED_C1))
+    graphs["cl600x400"] = ref_convert("cl600x400", *gen.chung_lu_edges(600, 400, 3000, 2.1, 17, cap=120))
+
+    cases = [
+        ("example", 1, 1), ("example", 2, 1), ("example", 1, 2),
+        ("er10x12", 1, 1), ("er10x12", 2, 1), ("er10x12", 1, 3), ("er12x10", 1, 1), ("er12x10", 2, 2),
+        ("er12x10", 0, 1), ("er12x10", 3, 1),
+        ("southern_women", 1, 1), ("southern_women", 2, 3), ("southern_women", 1, 3), ("southern_women", 2, 5),
+        ("moreno_crime", 1, 3), ("moreno_crime", 2, 5), ("moreno_crime", 1, 2),
+        ("er40x50", 1, 1), ("er40x50", 1, 3), ("er40x50", 2, 5),
+        ("cl600x400", 1, 3), ("cl600x400", 2, 9),
+        ("er200x300", 1, 1), ("er200x300", 1, 3), ("er200x300", 2, 5),
+    ]
+    os.makedirs(os.path.join(HERE, "sets"), exist_ok=True)
+    gpath = os.path.join(HERE, "golden.json")
+    old = json.load(open(gpath)) if os.path.exists(gpath) else {"cases": []}
+    have = {(c["graph"], c["k"], c["theta"]): c for c in old["cases"]}
+    out = {"graphs": {k: os.path.relpath(v, HERE) for k, v in graphs.items()}, "cases": []}
+    for name, k, theta in cases:
+        if (name, k, theta) in have and "--force" not in sys.argv:
+            out["cases"].append(have[(name, k, theta)])
+            continue
+        words = run_bppivot(graphs[name], k, theta)
+        dg = oracle_lib.digest(words)
+        rec = {"graph": name, "k": k, "theta": theta, "count": int(dg.count), "fp_sum": str(dg.fp_sum),
+               "fp_xor": str(dg.fp_xor), "canon_lo": str(dg.canon_lo), "canon_hi": str(dg.canon_hi)}
+        if dg.count <= 20000:
+            fn = f"sets/{name}_k{k}_t{theta}.u32"
+            words.astype("<u4").tofile(os.path.join(HERE, fn))
+            rec["set"] = fn
+        out["cases"].append(rec)
+        print(name, k, theta, dg.count, flush=True)
+        with open(gpath, "w") as f:
+            json.dump(out, f, indent=1)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,114 @@
+"""ctypes bindings to the oracle (oracle/gmk_oracle.cpp).  Test infrastructure:
+only tests, smoke() and the CPU-baseline leg of bench.py import this module."""
+from __future__ import annotations
+
+import ctypes
+import os
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(ROOT, "oracle", "_build", "liboracle.so")
+
+
+class Digest(ctypes.Structure):
+    _fields_ = [("count", ctypes.c_uint64), ("fp_sum", ctypes.c_uint64), ("fp_xor", ctypes.c_uint64),
+                ("canon_lo", ctypes.c_uint64), ("canon_hi", ctypes.c_uint64)]
+
+    def as_tuple(self):
+        return (self.count, self.fp_sum, self.fp_xor)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"oracle library missing: {LIB_PATH} (run __graft_entry__.build())")
+        L = ctypes.CDLL(LIB_PATH)
+        P = ctypes.POINTER
+        u32p = P(ctypes.c_uint32)
+        L.or_bppivot.restype = ctypes.c_longlong
+        L.or_bppivot.argtypes = [ctypes.c_uint32, ctypes.c_uint32, u32p, u32p, ctypes.c_uint32, ctypes.c_uint32,
+                                 ctypes.c_uint64, ctypes.c_int, ctypes.c_int, P(u32p), P(ctypes.c_uint64),
+                                 P(Digest), P(ctypes.c_uint64)]
+        L.or_brute.restype = ctypes.c_longlong
+        L.or_brute.argtypes = [ctypes.c_uint32, ctypes.c_uint32, u32p, u32p, ctypes.c_uint32, ctypes.c_uint32,
+                               P(u32p), P(ctypes.c_uint64), P(Digest)]
+        L.or_canon_digest.restype = ctypes.c_int
+        L.or_canon_digest.argtypes = [u32p, ctypes.c_uint64, P(Digest)]
+        L.or_validate.restype = ctypes.c_longlong
+        L.or_validate.argtypes = [ctypes.c_uint32, ctypes.c_uint32, u32p, u32p, ctypes.c_uint32, ctypes.c_uint32,
+                                  u32p, ctypes.c_uint64, P(ctypes.c_int)]
+        L.or_fingerprint.restype = ctypes.c_uint64
+        L.or_fingerprint.argtypes = [u32p, ctypes.c_uint32, u32p, ctypes.c_uint32]
+        L.or_free.argtypes = [ctypes.c_void_p]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))
+
+
+def _take(words_p, nwords):
+    n = int(nwords.value)
+    arr = np.ctypeslib.as_array(words_p, shape=(max(n, 1),))[:n].copy()
+    lib().or_free(ctypes.cast(words_p, ctypes.c_void_p))
+    return arr
+
+
+def bppivot(g, k, theta, limit=0, mode=2, keep=True):
+    """Returns (records uint32 array or None, Digest, branches)."""
+    off = np.ascontiguousarray(g.off, dtype=np.uint32)
+    adj = np.ascontiguousarray(g.adj, dtype=np.uint32)
+    wp = ctypes.POINTER(ctypes.c_uint32)()
+    nw = ctypes.c_uint64(0)
+    dg = Digest()
+    br = ctypes.c_uint64(0)
+    lib().or_bppivot(g.n_left, g.n, _p(off), _p(adj), k, theta, limit, mode, int(keep),
+                     ctypes.byref(wp), ctypes.byref(nw), ctypes.byref(dg), ctypes.byref(br))
+    words = _take(wp, nw)
+    return (words if keep else None), dg, int(br.value)
+
+
+def brute(g, k, theta):
+    off = np.ascontiguousarray(g.off, dtype=np.uint32)
+    adj = np.ascontiguousarray(g.adj, dtype=np.uint32)
+    wp = ctypes.POINTER(ctypes.c_uint32)()
+    nw = ctypes.c_uint64(0)
+    dg = Digest()
+    r = lib().or_brute(g.n_left, g.n, _p(off), _p(adj), k, theta, ctypes.byref(wp), ctypes.byref(nw), ctypes.byref(dg))
+    if r < 0:
+        raise ValueError("graph too large for brute force")
+    return _take(wp, nw), dg
+
+
+def digest(words):
+    words = np.ascontiguousarray(words, dtype=np.uint32)
+    dg = Digest()
+    if lib().or_canon_digest(_p(words), words.size, ctypes.byref(dg)) != 0:
+        raise ValueError("malformed record stream")
+    return dg
+
+
+def validate(g, k, theta, words):
+    """0 when every record is a distinct maximal k-biplex with sides >= theta;
+    otherwise (record index, reason code)."""
+    off = np.ascontiguousarray(g.off, dtype=np.uint32)
+    adj = np.ascontiguousarray(g.adj, dtype=np.uint32)
+    words = np.ascontiguousarray(words, dtype=np.uint32)
+    why = ctypes.c_int(0)
+    r = lib().or_validate(g.n_left, g.n, _p(off), _p(adj), k, theta, _p(words), words.size, ctypes.byref(why))
+    return 0 if r == 0 else (int(r), int(why.value))
+
+
+def records(words):
+    """Iterate (L tuple, R tuple) over a record stream."""
+    i, n = 0, len(words)
+    while i < n:
+        nl = int(words[i]); L = tuple(int(x) for x in words[i + 1:i + 1 + nl])
+        nr = int(words[i + 1 + nl]); R = tuple(int(x) for x in words[i + 2 + nl:i + 2 + nl + nr])
+        yield L, R
+        i += 2 + nl + nr

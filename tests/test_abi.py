@@ -1,0 +1,31 @@
+"""The C-ABI library loads and exports every entry point include/gmk.h declares
+(no compute call: this runs without a GPU)."""
+import ctypes
+import os
+import re
+
+from gmkbe_amd import engine
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    txt = open(os.path.join(ROOT, "include", "gmk.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(gmk_[a-z_]+)\s*\(", txt)))
+
+
+def test_library_exports_declared_symbols():
+    lib = ctypes.CDLL(engine.LIB_PATH)
+    names = declared_symbols()
+    assert "gmk_enumerate" in names and len(names) >= 7
+    for name in names:
+        assert hasattr(lib, name), name
+
+
+def test_engine_fails_loudly_without_gpu_or_library(monkeypatch):
+    import pytest
+    monkeypatch.setattr(engine, "LIB_PATH", "/nonexistent/libgmk.so")
+    monkeypatch.setattr(engine, "_lib", None)
+    with pytest.raises(engine.GmkError):
+        engine.Engine(0)

@@ -1,0 +1,62 @@
+"""GPU parity: the HIP enumerator against the reference BPPivot's golden sets,
+brute force and the oracle, through the C ABI."""
+import numpy as np
+import pytest
+
+import golden_cases as GC
+import oracle_lib as O
+from gmkbe_amd import gen, graphio
+from gmkbe_amd.engine import Engine
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def eng():
+    e = Engine(0)
+    yield e
+    e.close()
+
+
+def _check_set(g, k, theta, res, expect_words=None, expect_case=None):
+    dg = O.digest(res.words)
+    assert dg.count == res.count
+    assert (dg.fp_sum, dg.fp_xor) == (res.stats["fp_sum"], res.stats["fp_xor"])
+    if expect_case is not None:
+        assert GC.digest_matches(expect_case, dg), (dg.count, expect_case["count"])
+    if expect_words is not None:
+        de = O.digest(expect_words)
+        assert (dg.count, dg.canon_lo, dg.canon_hi) == (de.count, de.canon_lo, de.canon_hi)
+
+
+@pytest.mark.parametrize("c", GC.cases(min_k=1), ids=GC.case_id)
+def test_global_partition_matches_reference(eng, c):
+    g = GC.graph(c["graph"])
+    print(f"[gpu] golden {GC.case_id(c)} n={g.n}", flush=True)
+    eng.set_graph(g)
+    res = eng.enumerate(k=c["k"], theta=c["theta"], partition="global")
+    _check_set(g, c["k"], c["theta"], res, expect_case=c)
+
+
+@pytest.mark.parametrize("seed", range(6))
+@pytest.mark.parametrize("k", [0, 1, 2, 3])
+def test_global_partition_matches_brute(eng, seed, k):
+    rng = np.random.default_rng(100 + seed)
+    a, b = int(rng.integers(4, 10)), int(rng.integers(4, 10))
+    u, v = gen.er_edges(a, b, float(rng.uniform(0.25, 0.7)), 100 + seed)
+    g = graphio.from_edges(a, b, u, v - a)
+    eng.set_graph(g)
+    for theta in (1, 2, 3):
+        print(f"[gpu] brute seed={seed} k={k} theta={theta} {a}x{b}", flush=True)
+        wb, _ = O.brute(g, k, theta)
+        res = eng.enumerate(k=k, theta=theta, partition="global")
+        _check_set(g, k, theta, res, expect_words=wb)
+
+
+def test_limit_counts_and_validity(eng):
+    g = GC.graph("er200x300")
+    eng.set_graph(g)
+    for lim in (1, 10, 1000, 123457):
+        res = eng.enumerate(k=1, theta=1, limit=lim, partition="global")
+        assert res.count == lim and res.stats["hit_limit"]
+        assert O.validate(g, 1, 1, res.words) == 0
