@@ -9,7 +9,7 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(PKG, "csrc")
 OUT_DIR = os.path.join(PKG, "_lib")
 LIB = os.path.join(OUT_DIR, "libgmk.so")
-SOURCES = ["bb_kernel.hip", "gmk_engine.hip", "partition.hip"]
+SOURCES = ["bb_kernel.hip", "gmk_engine.hip", "partition.hip", "thin_kernels.hip", "thin_host.hip", "seeds.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # -packed-fp32-ops: keep the compiler from emitting packed FP32 instructions.
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++20", "-fPIC", "-Wall", "-Wno-unused-function",
@@ -20,8 +20,11 @@ def build(verbose: bool = False) -> str:
     os.makedirs(OUT_DIR, exist_ok=True)
     objs = []
     for src in SOURCES:
-        obj = os.path.join(OUT_DIR, src.replace(".hip", ".o"))
-        cmd = [HIPCC, *FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
+        obj = os.path.join(OUT_DIR, os.path.splitext(src)[0] + ".o")
+        if src.endswith(".cpp"):
+            cmd = ["g++", "-O3", "-std=c++20", "-fPIC", "-Wall", "-c", os.path.join(CSRC, src), "-o", obj]
+        else:
+            cmd = [HIPCC, *FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
         subprocess.run(cmd, check=True)

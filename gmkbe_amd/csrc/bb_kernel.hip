@@ -520,6 +520,8 @@ hipError_t launch_bb_expand(int W, u32 n_states, hipStream_t s, const ExpandArgs
     case 4: hipLaunchKernelGGL(bb_expand<4>, grid, block, 0, s, a); break;
     case 8: hipLaunchKernelGGL(bb_expand<8>, grid, block, 0, s, a); break;
     case 16: hipLaunchKernelGGL(bb_expand<16>, grid, block, 0, s, a); break;
+    case 32: hipLaunchKernelGGL(bb_expand<32>, grid, block, 0, s, a); break;
+    case 64: hipLaunchKernelGGL(bb_expand<64>, grid, block, 0, s, a); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

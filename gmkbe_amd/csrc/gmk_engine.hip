@@ -19,58 +19,6 @@ namespace gmk {
 hipError_t launch_bb_expand(int W, u32 n_states, hipStream_t s, const ExpandArgs& a);
 }
 
-#define HIPCHK(ctx, call)                                                                    \
-  do {                                                                                      \
-    hipError_t e_ = (call);                                                                 \
-    if (e_ != hipSuccess) {                                                                 \
-      (ctx)->err = std::string(#call) + ": " + hipGetErrorString(e_);                       \
-      return GMK_EDEVICE;                                                                   \
-    }                                                                                       \
-  } while (0)
-
-// Device counters of one run, read back after every batch.
-struct RunCounters {
-  u32 out_n, retry_n, flags, pad;
-  u64 res_words, res_count;
-  u64 fp[NSTRIPE * 2];
-  u64 stat[NSTRIPE * 4];
-};
-
-struct DeviceVec {
-  void* p = nullptr;
-  size_t bytes = 0;
-};
-
-struct gmk_ctx {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  std::string err;
-  // graph (host copy kept for partition construction)
-  u32 n = 0, n_left = 0;
-  std::vector<u32> off, adj;
-  u32* d_off = nullptr;
-  u32* d_adj = nullptr;
-  // per-run buffers, grown on demand and reused
-  DeviceVec stack, next, retry, rows, maps, parts, res, counters;
-  RunCounters* h_cnt = nullptr;  // pinned
-  u64 res_words = 0;             // record stream length of the last run
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
-};
-
-static int ensure(gmk_ctx* c, DeviceVec& v, size_t bytes) {
-  if (v.bytes >= bytes) return GMK_OK;
-  if (v.p) (void)hipFree(v.p);
-  v.p = nullptr; v.bytes = 0;
-  size_t b = std::max<size_t>(bytes, 256);
-  hipError_t e = hipMalloc(&v.p, b);
-  if (e != hipSuccess) {
-    c->err = "hipMalloc(" + std::to_string(b) + " B) failed: " + hipGetErrorString(e);
-    return GMK_ENOMEM;
-  }
-  v.bytes = b;
-  return GMK_OK;
-}
-
 extern "C" int gmk_create(int device, gmk_ctx** out) {
   if (!out) return GMK_EINVAL;
   *out = nullptr;
@@ -93,7 +41,8 @@ extern "C" void gmk_destroy(gmk_ctx* c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
-  for (DeviceVec* v : {&c->stack, &c->next, &c->retry, &c->rows, &c->maps, &c->parts, &c->res, &c->counters})
+  for (DeviceVec* v : {&c->stack, &c->next, &c->retry, &c->rows, &c->maps, &c->parts, &c->res, &c->counters,
+                        &c->tuples, &c->arena, &c->fams, &c->prefix, &c->tcnt})
     if (v->p) (void)hipFree(v->p);
   if (c->d_off) (void)hipFree(c->d_off);
   if (c->d_adj) (void)hipFree(c->d_adj);
@@ -144,6 +93,7 @@ static int check_csr(gmk_ctx* c) {
 extern "C" int gmk_set_graph_csr(gmk_ctx* c, uint32_t n, uint32_t n_left, const uint32_t* off, const uint32_t* adj) {
   if (!c || !off || (!adj && off[n] > 0)) return GMK_EINVAL;
   c->n = n; c->n_left = n_left;
+  c->order_valid = false;
   c->off.assign(off, off + n + 1);
   c->adj.assign(adj, adj + off[n]);
   int r = check_csr(c);
@@ -209,8 +159,8 @@ extern "C" int gmk_fetch_results(gmk_ctx* c, uint32_t* words, uint64_t cap, uint
 // ---------------------------------------------------------------------------
 // Branch-and-bound over a set of partitions.
 // ---------------------------------------------------------------------------
-static int pick_w(u32 words) {
-  for (int w : {1, 2, 4, 8, 16})
+int gmk_pick_w(u32 words) {
+  for (int w : {1, 2, 4, 8, 16, 32, 64})
     if ((u32)w >= words) return w;
   return 0;
 }
@@ -328,8 +278,8 @@ extern "C" int gmk_enumerate(gmk_ctx* c, const gmk_params* prm, gmk_stats* st) {
   const u32 nr = c->n - c->n_left;
   const u32 wl = (c->n_left + 63) / 64, wr = (nr + 63) / 64;
   int part = prm->partition;
-  const int Wg = pick_w(std::max(wl, wr));
-  if (part == GMK_PART_AUTO) part = Wg ? GMK_PART_GLOBAL : GMK_PART_SEEDED;
+  const int Wg = gmk_pick_w(std::max(wl, wr));
+  if (part == GMK_PART_AUTO) part = (Wg && Wg <= 16) ? GMK_PART_GLOBAL : GMK_PART_SEEDED;
   if (part == GMK_PART_GLOBAL) {
     if (!Wg) { c->err = "graph too large for a global partition"; return GMK_EUNSUPPORTED; }
     PartitionSet ps;

@@ -1,5 +1,6 @@
 // Host-side structures shared by the engine's translation units.
 #pragma once
+#include <string>
 #include <vector>
 
 #include "gmk_device.h"
@@ -34,5 +35,68 @@ u64* push_root(PartitionSet& ps, u32 pid);
 
 }  // namespace gmk
 
+using gmk::u32;
+using gmk::u64;
+using gmk::NSTRIPE;
+
+#define HIPCHK(ctx, call)                                                                    \
+  do {                                                                                      \
+    hipError_t e_ = (call);                                                                 \
+    if (e_ != hipSuccess) {                                                                 \
+      (ctx)->err = std::string(#call) + ": " + hipGetErrorString(e_);                       \
+      return GMK_EDEVICE;                                                                   \
+    }                                                                                       \
+  } while (0)
+
+// Device counters of one run, read back after every batch.
+struct RunCounters {
+  u32 out_n, retry_n, flags, pad;
+  u64 res_words, res_count;
+  u64 fp[NSTRIPE * 2];
+  u64 stat[NSTRIPE * 4];
+};
+
+struct DeviceVec {
+  void* p = nullptr;
+  size_t bytes = 0;
+};
+
+struct gmk_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  // graph (host copy kept for partition construction)
+  u32 n = 0, n_left = 0;
+  std::vector<u32> off, adj;
+  u32* d_off = nullptr;
+  u32* d_adj = nullptr;
+  // per-run buffers, grown on demand and reused
+  DeviceVec stack, next, retry, rows, maps, parts, res, counters;
+  DeviceVec tuples, arena, fams, prefix, tcnt;  // thin-result kernels
+  std::vector<u32> order;                        // degeneracy order (seeded runs)
+  bool order_valid = false;
+  RunCounters* h_cnt = nullptr;  // pinned
+  u64 res_words = 0;             // record stream length of the last run
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+inline int ensure(gmk_ctx* c, DeviceVec& v, size_t bytes) {
+  if (v.bytes >= bytes) return GMK_OK;
+  if (v.p) (void)hipFree(v.p);
+  v.p = nullptr; v.bytes = 0;
+  size_t b = std::max<size_t>(bytes, 256);
+  hipError_t e = hipMalloc(&v.p, b);
+  if (e != hipSuccess) {
+    c->err = "hipMalloc(" + std::to_string(b) + " B) failed: " + hipGetErrorString(e);
+    return GMK_ENOMEM;
+  }
+  v.bytes = b;
+  return GMK_OK;
+}
+
+
+int gmk_pick_w(uint32_t words);
 int gmk_run_bb(gmk_ctx* c, const gmk::PartitionSet& ps, const gmk_params* prm, gmk_stats* st, gmk::RunState& rs);
 int gmk_run_seeded(gmk_ctx* c, const gmk_params* prm, gmk_stats* st, gmk::RunState& rs);
+// Thin results with the tuple on side_a (0 left, 1 right); generated side >= min_b.
+int gmk_run_thin(gmk_ctx* c, const gmk_params* prm, gmk_stats* st, gmk::RunState& rs, int side_a, uint32_t min_b);

@@ -1,0 +1,235 @@
+// Thin results: maximal k-biplexes with one side ("side A", the tuple side) of
+// at most 2k vertices.  Such a result cannot be found inside a 2-hop seed
+// partition: members of the small side need not share neighbours, and with at
+// most k vertices on a side every vertex of the other side may be a member.
+//
+// For a tuple L = (l_0 < ... < l_{j-1}) on side A (j <= 2k) the other side R of
+// every maximal result (L, R) is built from miss types.  For a vertex y of side
+// B its type is the set of tuple members it is NOT adjacent to (a j-bit mask).
+// Types with more than k bits are infeasible; type 0 vertices ("free") are in
+// every maximal R; vertices of a type m are interchangeable for L, so R is fixed
+// by a count c_m per type plus which c_m vertices of each pool are taken.  Counts
+// obey one budget per member (sum of c_m over masks containing member i <= k) and
+// are maximal (a type with spare vertices has a member whose budget is spent).
+// Vertices adjacent to no member form the far pool (type = all bits, only
+// feasible when j <= k); it is never materialised: member i of the far pool is
+// found by select over the sorted list N(L).
+//
+// A concrete R is reported iff (L, R) is maximal on side A too: no w outside L
+// can be added.  w can be added iff it misses at most k of R and is adjacent to
+// every saturated y in R (popcount(type) == k).  Candidates are the neighbours
+// of one saturated y, or, without saturated vertices and |R| > k, the
+// neighbours of the k+1 lowest-degree members of R (w misses at most k of them).
+//
+// This is the role of GMkBPE's Generate/Special path (include/for_generate.cuh:
+// 300-512, include/for_special.cuh:274-400: conY = free vertices, varY = pools,
+// k-combinations checked by CheckMaximality_for_special,
+// include/for_checkmaximality.cuh:2587-2744), generalised to |L| <= 2k.
+//
+// All functions here are plain C++ usable on the host and, compiled by hipcc,
+// on the device (one lane per tuple / per combination).
+#pragma once
+#include <cstdint>
+
+#ifdef __HIPCC__
+#include <hip/hip_runtime.h>
+#define GMK_HD __host__ __device__ __forceinline__
+#else
+#define GMK_HD inline
+#endif
+
+namespace gmk {
+namespace thin {
+
+typedef uint32_t u32;
+typedef uint64_t u64;
+
+constexpr int MAXJ = 6;       // tuple size limit (2k with k <= 3)
+constexpr int MAXT = 18;      // chosen types per family (<= j * k)
+constexpr u64 SAT = 1ull << 62;  // saturation value of combination counts
+constexpr u32 FAR = 0xFFFFFFFFu; // pool offset marking the far pool
+
+GMK_HD int popc(u32 x) {
+  int c = 0;
+  while (x) { x &= x - 1; ++c; }
+  return c;
+}
+
+// C(n, c) for c <= 3, saturating at SAT.
+GMK_HD u64 binom(u64 n, u32 c) {
+  if (c == 0) return 1;
+  if (n < c) return 0;
+  if (c == 1) return n;
+  if (c == 2) {
+    const u64 a = n % 2 == 0 ? n / 2 : n, b = n % 2 == 0 ? n - 1 : (n - 1) / 2;
+    if (a && b > SAT / a) return SAT;
+    return a * b;
+  }
+  // c == 3: n (n-1) (n-2) / 6
+  u64 x = n, y = n - 1, z = n - 2;
+  if (x % 2 == 0) x /= 2; else y /= 2;
+  if (x % 3 == 0) x /= 3; else if (y % 3 == 0) y /= 3; else z /= 3;
+  if (x && y > SAT / x) return SAT;
+  const u64 xy = x * y;
+  if (xy && z > SAT / xy) return SAT;
+  return xy * z;
+}
+
+GMK_HD u64 mul_sat(u64 a, u64 b) {
+  if (a == 0 || b == 0) return 0;
+  if (a >= SAT || b >= SAT || a > SAT / b) return SAT;
+  return a * b;
+}
+
+// Colex unrank: rank r of C(n, c) -> idx[0] < ... < idx[c-1].
+GMK_HD void unrank(u64 r, u32 c, u64 n, u32* idx) {
+  for (u32 t = c; t >= 1; --t) {
+    // largest x in [t-1, n-1] with C(x, t) <= r
+    u64 lo = t - 1, hi = n - 1;
+    while (lo < hi) {
+      const u64 mid = lo + (hi - lo + 1) / 2;
+      if (binom(mid, t) <= r) lo = mid; else hi = mid - 1;
+    }
+    idx[t - 1] = (u32)lo;
+    r -= binom(lo, t);
+    n = lo;
+  }
+}
+
+// Sorted-list membership.
+GMK_HD bool contains(const u32* a, u32 n, u32 x) {
+  u32 lo = 0, hi = n;
+  while (lo < hi) {
+    const u32 mid = (lo + hi) >> 1;
+    if (a[mid] < x) lo = mid + 1; else hi = mid;
+  }
+  return lo < n && a[lo] == x;
+}
+
+// i-th vertex of [b_lo, b_hi) that is not in the sorted list nl (size d).
+GMK_HD u32 far_select(const u32* nl, u32 d, u32 b_lo, u64 i) {
+  // p = number of members m_q with (m_q - b_lo - q) <= i
+  u32 lo = 0, hi = d;
+  while (lo < hi) {
+    const u32 mid = (lo + hi) >> 1;
+    if ((u64)(nl[mid] - b_lo) - mid <= i) lo = mid + 1; else hi = mid;
+  }
+  return (u32)(b_lo + i + lo);
+}
+
+struct Family {
+  u32 L[MAXJ];
+  u32 j;
+  u32 nfree;          // free vertices (type 0)
+  u32 ntypes;         // types with c > 0
+  u32 rsize;          // |R| = nfree + sum c
+  u64 free_off;       // arena offset of the free list (sorted)
+  u64 nl_off;         // arena offset of N(L) sorted (far-pool select)
+  u32 nl_size;
+  u32 tuple;          // index of the tuple in its batch
+  u32 mask[MAXT];
+  u32 cnt[MAXT];
+  u32 psize[MAXT];
+  u64 poff[MAXT];     // arena offset of the pool (sorted), FAR for the far pool
+  u64 total;          // number of concrete R (product of binomials), saturating
+};
+
+struct Graph {
+  const u32* off;
+  const u32* adj;
+  u32 a_lo, a_hi;  // side A (tuple side), global IDs
+  u32 b_lo, b_hi;  // side B
+};
+
+// Threat test: can some w in side A outside L be added to (L, R)?
+//   R = free (sorted, nfree) + ch (nch chosen, each with its type size tsz).
+GMK_HD bool member_of_R(const u32* freel, u32 nfree, const u32* ch, u32 nch, u32 y) {
+  if (contains(freel, nfree, y)) return true;
+  for (u32 i = 0; i < nch; ++i)
+    if (ch[i] == y) return true;
+  return false;
+}
+
+GMK_HD bool in_tuple(const u32* L, u32 j, u32 w) {
+  for (u32 i = 0; i < j; ++i)
+    if (L[i] == w) return true;
+  return false;
+}
+
+// Does w miss at most k vertices of R, and is w adjacent to every saturated one?
+GMK_HD bool addable(const Graph& g, u32 w, const u32* freel, u32 nfree, const u32* ch, const u32* ctsz, u32 nch,
+                    u32 k) {
+  const u32* nw = g.adj + g.off[w];
+  const u32 dw = g.off[w + 1] - g.off[w];
+  const u32 rs = nfree + nch;
+  if (dw + k < rs) return false;
+  u32 miss = 0;
+  for (u32 i = 0; i < nch; ++i) {
+    if (!contains(nw, dw, ch[i])) {
+      if (ctsz[i] >= k) return false;  // saturated member missed
+      if (++miss > k) return false;
+    }
+  }
+  // free vertices have type 0 (< k for k >= 1): only their number matters
+  if (nfree <= dw) {
+    for (u32 i = 0; i < nfree; ++i)
+      if (!contains(nw, dw, freel[i]) && ++miss > k) return false;
+  } else {
+    // count free vertices adjacent to w by scanning N(w)
+    u32 hit = 0;
+    for (u32 i = 0; i < dw; ++i) hit += contains(freel, nfree, nw[i]) ? 1 : 0;
+    miss += nfree - hit;
+    if (miss > k) return false;
+  }
+  return true;
+}
+
+// Returns true iff (L, R) is maximal on side A.
+GMK_HD bool side_a_maximal(const Graph& g, const u32* L, u32 j, const u32* freel, u32 nfree, const u32* ch,
+                           const u32* ctsz, u32 nch, u32 k) {
+  const u32 rs = nfree + nch;
+  // saturated chosen vertex with the smallest degree
+  u32 best = 0xFFFFFFFFu, bestdeg = 0xFFFFFFFFu;
+  for (u32 i = 0; i < nch; ++i)
+    if (ctsz[i] >= k) {
+      const u32 d = g.off[ch[i] + 1] - g.off[ch[i]];
+      if (d < bestdeg) { bestdeg = d; best = ch[i]; }
+    }
+  if (best != 0xFFFFFFFFu) {
+    for (u32 e = g.off[best]; e < g.off[best + 1]; ++e) {
+      const u32 w = g.adj[e];
+      if (in_tuple(L, j, w)) continue;
+      if (addable(g, w, freel, nfree, ch, ctsz, nch, k)) return false;
+    }
+    return true;
+  }
+  if (rs <= k) {
+    // every vertex of side A outside L can be added
+    return (u64)(g.a_hi - g.a_lo) <= j;
+  }
+  // neighbours of the k+1 lowest-degree members of R
+  u32 pick[4];
+  u32 pdeg[4];
+  u32 np = 0;
+  auto consider = [&](u32 y) {
+    const u32 d = g.off[y + 1] - g.off[y];
+    if (np < k + 1) { pick[np] = y; pdeg[np] = d; ++np; }
+    else {
+      u32 worst = 0;
+      for (u32 q = 1; q < np; ++q) if (pdeg[q] > pdeg[worst]) worst = q;
+      if (d < pdeg[worst]) { pick[worst] = y; pdeg[worst] = d; }
+    }
+  };
+  for (u32 i = 0; i < nch; ++i) consider(ch[i]);
+  for (u32 i = 0; i < nfree; ++i) consider(freel[i]);
+  for (u32 q = 0; q < np; ++q)
+    for (u32 e = g.off[pick[q]]; e < g.off[pick[q] + 1]; ++e) {
+      const u32 w = g.adj[e];
+      if (in_tuple(L, j, w)) continue;
+      if (addable(g, w, freel, nfree, ch, ctsz, nch, k)) return false;
+    }
+  return true;
+}
+
+}  // namespace thin
+}  // namespace gmk

@@ -1,0 +1,41 @@
+// Kernel arguments of the thin-result kernels (shared by host and device code).
+#pragma once
+#include "gmk_device.h"
+#include "thin.h"
+
+namespace gmk {
+
+struct ThinArgs {
+  thin::Graph g;
+  u32 k;
+  u32 min_b;            // minimum size of the generated side
+  // T1
+  const u32* tuples;    // ntuples x j
+  u32 ntuples, j;
+  u32* arena;
+  u64 arena_cap;
+  u64* arena_used;
+  thin::Family* fams;
+  u32 fam_cap;
+  u32* fam_n;
+  u32* tflags;          // family-construction overflow flags
+  u32* flags;           // run flags (limit, result overflow)
+  // T2
+  const u64* prefix;    // nfam + 1 prefix sums of family totals
+  u32 nfam;
+  u64 r0, r1;           // rank range of this launch
+  int keep;
+  int swap_sides;       // 1 when side A is the right side (records list L first)
+  u32* res;
+  u64 res_cap;
+  u64* res_words;
+  u64* res_count;
+  u64 limit;
+  u64* fp;
+  u64* stat;
+};
+
+hipError_t launch_thin_build(const ThinArgs& a, hipStream_t s);
+hipError_t launch_thin_check(const ThinArgs& a, hipStream_t s);
+
+}  // namespace gmk

@@ -1,0 +1,176 @@
+// Host orchestration of the thin-result kernels: tuples are generated seed by
+// seed (ascending IDs on the tuple side), families built on the device in
+// chunks, then every concrete combination is checked on the device, rank range
+// by rank range, until the result limit is reached.
+#include <algorithm>
+#include <cstddef>
+#include <vector>
+
+#include "gmk_host.h"
+#include "thin_args.h"
+
+using namespace gmk;
+using thin::Family;
+
+namespace {
+
+struct ThinCounters {
+  u32 fam_n, tflags;
+  u64 arena_used;
+};
+
+constexpr u32 TCH = 1u << 16;          // tuples per family-construction launch
+constexpr u64 ARENA = 1ull << 27;      // arena words (512 MiB)
+constexpr u32 FCAP = 1u << 18;         // families per launch
+constexpr u64 RCH = 1ull << 22;        // ranks per check launch
+
+struct ThinRun {
+  gmk_ctx* c;
+  const gmk_params* prm;
+  RunState* rs;
+  ThinArgs a{};
+  std::vector<u64> totals, prefix;
+  bool stop = false;
+};
+
+int run_chunk(ThinRun& T, const u32* tuples, u32 count, u32 j) {
+  gmk_ctx* c = T.c;
+  hipStream_t s = c->stream;
+  HIPCHK(c, hipMemcpyAsync(c->tuples.p, tuples, (size_t)count * j * 4, hipMemcpyHostToDevice, s));
+  HIPCHK(c, hipMemsetAsync(c->tcnt.p, 0, sizeof(ThinCounters), s));
+  ThinArgs& a = T.a;
+  a.ntuples = count;
+  a.j = j;
+  HIPCHK(c, launch_thin_build(a, s));
+  ThinCounters tc;
+  HIPCHK(c, hipMemcpyAsync(&tc, c->tcnt.p, sizeof(tc), hipMemcpyDeviceToHost, s));
+  HIPCHK(c, hipStreamSynchronize(s));
+  T.rs->launches++;
+  if (tc.tflags) {
+    if (count == 1) { c->err = "one tuple's neighbour lists exceed the arena"; return GMK_EOVERFLOW; }
+    const u32 h = count / 2;
+    int r = run_chunk(T, tuples, h, j);
+    if (r || T.stop) return r;
+    return run_chunk(T, tuples + (size_t)h * j, count - h, j);
+  }
+  const u32 nfam = tc.fam_n;
+  if (nfam == 0) return GMK_OK;
+  T.totals.resize(nfam);
+  HIPCHK(c, hipMemcpy2DAsync(T.totals.data(), 8, (char*)c->fams.p + offsetof(Family, total), sizeof(Family), 8, nfam,
+                             hipMemcpyDeviceToHost, s));
+  HIPCHK(c, hipStreamSynchronize(s));
+  T.prefix.assign(nfam + 1, 0);
+  for (u32 i = 0; i < nfam; ++i) {
+    const u64 t = T.prefix[i] + T.totals[i];
+    T.prefix[i + 1] = (t < T.prefix[i] || t > thin::SAT) ? thin::SAT : t;
+  }
+  HIPCHK(c, hipMemcpyAsync(c->prefix.p, T.prefix.data(), (nfam + 1) * 8, hipMemcpyHostToDevice, s));
+  a.nfam = nfam;
+  const u64 total = T.prefix[nfam];
+  RunCounters* dc = (RunCounters*)c->counters.p;
+  for (u64 r0 = 0; r0 < total; r0 += RCH) {
+    a.r0 = r0;
+    a.r1 = std::min(total, r0 + RCH);
+    HIPCHK(c, hipEventRecord(c->ev0, s));
+    HIPCHK(c, launch_thin_check(a, s));
+    HIPCHK(c, hipEventRecord(c->ev1, s));
+    HIPCHK(c, hipMemcpyAsync(c->h_cnt, dc, offsetof(RunCounters, fp), hipMemcpyDeviceToHost, s));
+    HIPCHK(c, hipStreamSynchronize(s));
+    float ms = 0;
+    HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    T.rs->ms_expand += ms;
+    T.rs->launches++;
+    if (c->h_cnt->flags & FLAG_RES_OVERFLOW) { c->err = "result buffer overflow (raise result_capacity_words)"; return GMK_EOVERFLOW; }
+    if (c->h_cnt->flags & FLAG_LIMIT) { T.stop = true; T.rs->hit_limit = true; return GMK_OK; }
+  }
+  return GMK_OK;
+}
+
+}  // namespace
+
+int gmk_run_thin(gmk_ctx* c, const gmk_params* prm, gmk_stats* st, RunState& rs, int side_a, uint32_t min_b) {
+  (void)st;
+  const u32 k = prm->k;
+  const u32 jmin = std::max<u32>(prm->theta, 1);
+  const u32 jmax = std::min<u32>(2 * k, thin::MAXJ);
+  if (jmin > jmax) return GMK_OK;
+  int r;
+  if ((r = ensure(c, c->tuples, (size_t)TCH * thin::MAXJ * 4))) return r;
+  if ((r = ensure(c, c->arena, ARENA * 4))) return r;
+  if ((r = ensure(c, c->fams, (size_t)FCAP * sizeof(Family)))) return r;
+  if ((r = ensure(c, c->prefix, (size_t)(FCAP + 1) * 8))) return r;
+  if ((r = ensure(c, c->tcnt, sizeof(ThinCounters)))) return r;
+  ThinRun T;
+  T.c = c; T.prm = prm; T.rs = &rs;
+  ThinArgs& a = T.a;
+  a.g.off = c->d_off;
+  a.g.adj = c->d_adj;
+  if (side_a == 0) { a.g.a_lo = 0; a.g.a_hi = c->n_left; a.g.b_lo = c->n_left; a.g.b_hi = c->n; }
+  else { a.g.a_lo = c->n_left; a.g.a_hi = c->n; a.g.b_lo = 0; a.g.b_hi = c->n_left; }
+  a.k = k;
+  a.min_b = min_b;
+  a.tuples = (const u32*)c->tuples.p;
+  a.arena = (u32*)c->arena.p;
+  a.arena_cap = ARENA;
+  ThinCounters* tc = (ThinCounters*)c->tcnt.p;
+  a.arena_used = &tc->arena_used;
+  a.fams = (Family*)c->fams.p;
+  a.fam_cap = FCAP;
+  a.fam_n = &tc->fam_n;
+  a.tflags = &tc->tflags;
+  RunCounters* dc = (RunCounters*)c->counters.p;
+  a.flags = &dc->flags;
+  a.prefix = (const u64*)c->prefix.p;
+  a.keep = rs.keep;
+  a.swap_sides = side_a;
+  a.res = (u32*)c->res.p;
+  a.res_cap = rs.res_cap;
+  a.res_words = &dc->res_words;
+  a.res_count = &dc->res_count;
+  a.limit = rs.limit;
+  a.fp = dc->fp;
+  a.stat = dc->stat;
+
+  const u32 a_lo = a.g.a_lo, a_hi = a.g.a_hi;
+  const u32* off = c->off.data();
+  // seeds per block: plain mode takes one seed at a time
+  const u32 block = prm->mode == GMK_MODE_PLAIN ? 1u : 4096u;
+  std::vector<u32> buf;
+  buf.reserve((size_t)TCH * thin::MAXJ);
+  for (u32 s0 = a_lo; s0 < a_hi && !T.stop; s0 += block) {
+    const u32 s1 = std::min(a_hi, s0 + block);
+    for (u32 j = jmin; j <= jmax && !T.stop; ++j) {
+      buf.clear();
+      u32 count = 0;
+      u64 est = 0;
+      auto flush = [&]() -> int {
+        if (count == 0) return GMK_OK;
+        int rr = run_chunk(T, buf.data(), count, j);
+        buf.clear(); count = 0; est = 0;
+        return rr;
+      };
+      for (u32 s = s0; s < s1 && !T.stop; ++s) {
+        if (j > 1 && s + (j - 1) >= a_hi) continue;
+        u32 idx[thin::MAXJ];
+        idx[0] = s;
+        for (u32 i = 1; i < j; ++i) idx[i] = s + i;
+        for (;;) {
+          u64 need = 0;
+          for (u32 i = 0; i < j; ++i) need += 2ull * (off[idx[i] + 1] - off[idx[i]]);
+          if (count == TCH || est + need > ARENA) { if ((r = flush())) return r; if (T.stop) break; }
+          buf.insert(buf.end(), idx, idx + j);
+          ++count;
+          est += need;
+          if (j == 1) break;
+          int p = (int)j - 1;
+          while (p >= 1 && idx[p] == a_hi - (j - p)) --p;
+          if (p < 1) break;
+          ++idx[p];
+          for (u32 q = p + 1; q < j; ++q) idx[q] = idx[q - 1] + 1;
+        }
+      }
+      if (!T.stop && (r = flush())) return r;
+    }
+  }
+  return GMK_OK;
+}

@@ -1,0 +1,94 @@
+// CPU model of the thin-result kernels: runs the same family construction
+// (thin_build.h) and maximality test (thin.h) sequentially, over all tuples of
+// one side.  Test infrastructure only: it lets the CPU tests check that
+// thin-L + thin-R + seed partitions cover exactly the reference's result set.
+#include <algorithm>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "../../gmkbe_amd/csrc/thin.h"
+#include "../../gmkbe_amd/csrc/thin_build.h"
+
+using namespace gmk::thin;
+
+namespace {
+struct HostArena {
+  std::vector<u32> v;
+  u64 alloc(u64 n) { const u64 at = v.size(); v.resize(at + n); return at; }
+  u32* ptr(u64 off) { return v.data() + off; }
+};
+struct HostFams {
+  std::vector<Family> f;
+  Family* next() { f.emplace_back(); return &f.back(); }
+};
+}  // namespace
+
+extern "C" long long thin_model(unsigned n, unsigned n_left, const unsigned* off, const unsigned* adj, unsigned k,
+                                unsigned theta, int side_a, unsigned min_b, unsigned** out_words,
+                                unsigned long long* out_n) {
+  Graph g;
+  g.off = off; g.adj = adj;
+  if (side_a == 0) { g.a_lo = 0; g.a_hi = n_left; g.b_lo = n_left; g.b_hi = n; }
+  else { g.a_lo = n_left; g.a_hi = n; g.b_lo = 0; g.b_hi = n_left; }
+  std::vector<u32> words;
+  long long count = 0;
+  const u32 jmin = theta > 1 ? theta : 1;
+  for (u32 s = g.a_lo; s < g.a_hi; ++s) {
+    for (u32 j = jmin; j <= 2 * k && j <= (u32)MAXJ; ++j) {
+      // all tuples (s, w_1 < ... < w_{j-1}) with w_i > s
+      u32 L[MAXJ];
+      L[0] = s;
+      std::vector<u32> idx(j, 0);
+      for (u32 i = 1; i < j; ++i) idx[i] = s + i;
+      if (j > 1 && idx[j - 1] >= g.a_hi) continue;
+      for (;;) {
+        for (u32 i = 1; i < j; ++i) L[i] = idx[i];
+        HostArena ar;
+        HostFams fo;
+        build_families(g, L, j, k, min_b, 0, ar, fo);
+        for (const Family& f : fo.f) {
+          for (u64 r = 0; r < f.total; ++r) {
+            u64 rr = r;
+            u32 ch[MAXJ * 3], ct[MAXJ * 3], nch = 0;
+            for (u32 t = 0; t < f.ntypes; ++t) {
+              const u64 radix = binom(f.psize[t], f.cnt[t]);
+              const u64 rt = rr % radix;
+              rr /= radix;
+              u32 id[3];
+              unrank(rt, f.cnt[t], f.psize[t], id);
+              for (u32 q = 0; q < f.cnt[t]; ++q) {
+                ch[nch] = f.poff[t] == FAR ? far_select(ar.ptr(f.nl_off), f.nl_size, g.b_lo, id[q]) : ar.v[f.poff[t] + id[q]];
+                ct[nch] = popc(f.mask[t]);
+                ++nch;
+              }
+            }
+            if (!side_a_maximal(g, f.L, f.j, ar.ptr(f.free_off), f.nfree, ch, ct, nch, k)) continue;
+            std::vector<u32> R(ar.ptr(f.free_off), ar.ptr(f.free_off) + f.nfree);
+            R.insert(R.end(), ch, ch + nch);
+            std::sort(R.begin(), R.end());
+            std::vector<u32> T(f.L, f.L + f.j);
+            const std::vector<u32>& left = side_a == 0 ? T : R;
+            const std::vector<u32>& right = side_a == 0 ? R : T;
+            words.push_back((u32)left.size());
+            words.insert(words.end(), left.begin(), left.end());
+            words.push_back((u32)right.size());
+            words.insert(words.end(), right.begin(), right.end());
+            ++count;
+          }
+        }
+        // next tuple
+        if (j == 1) break;
+        int p = (int)j - 1;
+        while (p >= 1 && idx[p] == g.a_hi - (j - p)) --p;
+        if (p < 1) break;
+        ++idx[p];
+        for (u32 q = p + 1; q < j; ++q) idx[q] = idx[q - 1] + 1;
+      }
+    }
+  }
+  *out_n = words.size();
+  *out_words = (u32*)malloc(sizeof(u32) * (words.size() + 1));
+  if (!words.empty()) memcpy(*out_words, words.data(), sizeof(u32) * words.size());
+  return count;
+}

@@ -112,3 +112,25 @@ def records(words):
         nr = int(words[i + 1 + nl]); R = tuple(int(x) for x in words[i + 2 + nl:i + 2 + nl + nr])
         yield L, R
         i += 2 + nl + nr
+
+
+MODEL_PATH = os.path.join(ROOT, "tests", "model", "_build", "libthinmodel.so")
+_model = None
+
+
+def thin_model(g, k, theta, side_a, min_b):
+    """CPU model of the thin-result kernels (tests/model/thin_model.cpp)."""
+    global _model
+    if _model is None:
+        _model = ctypes.CDLL(MODEL_PATH)
+        P = ctypes.POINTER
+        u32p = P(ctypes.c_uint32)
+        _model.thin_model.restype = ctypes.c_longlong
+        _model.thin_model.argtypes = [ctypes.c_uint32, ctypes.c_uint32, u32p, u32p, ctypes.c_uint32, ctypes.c_uint32,
+                                      ctypes.c_int, ctypes.c_uint32, P(u32p), P(ctypes.c_uint64)]
+    off = np.ascontiguousarray(g.off, dtype=np.uint32)
+    adj = np.ascontiguousarray(g.adj, dtype=np.uint32)
+    wp = ctypes.POINTER(ctypes.c_uint32)()
+    nw = ctypes.c_uint64(0)
+    _model.thin_model(g.n, g.n_left, _p(off), _p(adj), k, theta, side_a, min_b, ctypes.byref(wp), ctypes.byref(nw))
+    return _take(wp, nw)

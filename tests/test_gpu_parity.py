@@ -60,3 +60,32 @@ def test_limit_counts_and_validity(eng):
         res = eng.enumerate(k=1, theta=1, limit=lim, partition="global")
         assert res.count == lim and res.stats["hit_limit"]
         assert O.validate(g, 1, 1, res.words) == 0
+
+
+@pytest.mark.parametrize("mode", ["plain", "batched"])
+@pytest.mark.parametrize("c", GC.cases(min_k=1), ids=GC.case_id)
+def test_seeded_partition_matches_reference(eng, c, mode):
+    g = GC.graph(c["graph"])
+    if c["k"] > 3 and c["theta"] <= 2 * c["k"]:
+        pytest.skip("thin results need k <= 3")
+    if mode == "plain" and c["count"] > 600_000:
+        pytest.skip("plain mode on the large cases is covered by the batched run")
+    print(f"[gpu] seeded {mode} {GC.case_id(c)}", flush=True)
+    eng.set_graph(g)
+    res = eng.enumerate(k=c["k"], theta=c["theta"], partition="seeded", mode=mode)
+    _check_set(g, c["k"], c["theta"], res, expect_case=c)
+
+
+@pytest.mark.parametrize("seed", range(4))
+@pytest.mark.parametrize("k", [0, 1, 2, 3])
+def test_seeded_partition_matches_brute(eng, seed, k):
+    rng = np.random.default_rng(200 + seed)
+    a, b = int(rng.integers(4, 10)), int(rng.integers(4, 10))
+    u, v = gen.er_edges(a, b, float(rng.uniform(0.25, 0.7)), 200 + seed)
+    g = graphio.from_edges(a, b, u, v - a)
+    eng.set_graph(g)
+    for theta in (1, 2, 3, 4):
+        print(f"[gpu] seeded brute seed={seed} k={k} theta={theta} {a}x{b}", flush=True)
+        wb, _ = O.brute(g, k, theta)
+        res = eng.enumerate(k=k, theta=theta, partition="seeded", mode="batched")
+        _check_set(g, k, theta, res, expect_words=wb)
