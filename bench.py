@@ -1,0 +1,204 @@
+"""Benchmark: maximal k-biplexes per second on one MI355X.
+
+Workload (default): Chung-Lu bipartite graph |U| = 50,000, |V| = 30,000,
+500,000 edges, Pareto degree exponent 2.1 capped at 5,000 (seeded, converted
+with the reference converter's relabel), k = 1, theta = 1, result limit
+r = 1,000,000.  One step = one enumeration to the limit through the C ABI,
+graph already resident on the device.  `value` = results per second over the
+timed steps (plain per-partition mode); the batched mode is timed on the same
+graph and reported beside it.
+
+Also reported: the roofline of the dominant kernel (algorithmic bytes counted
+in-kernel / device time from HIP events on the kernel's stream) and the CPU
+baseline: the reference BPPivot (compiled from its sources into oracle/_ref/)
+timed on this host, one core, on a bounded sample of the same enumeration.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import resource
+import subprocess
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "maximal k-biplexes/sec, 1 GPU, exact set; batched+learned vs plain speedup"
+HBM_PEAK_GBS = 8000.0
+
+WORKLOADS = {
+    "cl50kx30k": dict(k=1, theta=1, limit=1_000_000, cpu_sample=4000,
+                      desc="Chung-Lu bipartite |U|=50000 |V|=30000 m=500000 alpha=2.1 cap=5000 (seeded), k=1, theta=1, r=1000000"),
+    "er200x300": dict(k=1, theta=1, limit=0, cpu_sample=0,
+                      desc="Erdos-Renyi bipartite 200x300 p=0.05 (seeded), k=1, theta=1, full enumeration"),
+    "cl2Mx1M": dict(k=2, theta=1, limit=10_000_000, cpu_sample=200,
+                    desc="Chung-Lu bipartite |U|=2000000 |V|=1000000 m=20000000 alpha=2.0 (seeded), k=2, theta=1, r=10000000"),
+    "planted500k": dict(k=3, theta=1, limit=10_000_000, cpu_sample=200,
+                        desc="Chung-Lu 500000x500000 m=5000000 alpha=2.2 + 200 planted near-bicliques (seeded), k=3, theta=1, r=10000000"),
+}
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def traffic_from_profiles(kernel: str, workload: str):
+    """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE,
+    scaled by the copy-kernel calibration), committed under profiles/."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{kernel}_{workload}.txt")
+    if not os.path.exists(path):
+        return None
+    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", open(path).read())
+    return float(m.group(1)) if m else None
+
+
+def cpu_baseline(g, k, theta, sample):
+    """Reference BPPivot (oracle/_ref/bppivot), one core, first `sample` results."""
+    if sample <= 0:
+        return None
+    exe = os.path.join(ROOT, "oracle", "_ref", "bppivot")
+    from gmkbe_amd import graphio
+    with tempfile.TemporaryDirectory() as td:
+        path = os.path.join(td, "g.graph.bin")
+        graphio.write_graph_bin(path, g)
+        if os.path.exists(exe):
+            def unlimit():
+                resource.setrlimit(resource.RLIMIT_STACK, (resource.RLIM_INFINITY, resource.RLIM_INFINITY))
+                if hasattr(os, "sched_setaffinity"):
+                    os.sched_setaffinity(0, {sorted(os.sched_getaffinity(0))[0]})
+            t0 = time.time()
+            p = subprocess.run([exe, "-d", path, "-q", str(theta), "-k", str(k), "-n", str(sample)],
+                               capture_output=True, text=True, preexec_fn=unlimit, timeout=600)
+            wall = time.time() - t0
+            m = re.search(r"Number of biplexes:\s*(\d+)", p.stdout)
+            t = re.search(r"Branch time:\s*(\d+)", p.stdout)
+            if m and t:
+                n, ms = int(m.group(1)), max(1, int(t.group(1)))
+                return {"value": n / (ms / 1000.0), "unit": "biplexes/s", "cores": 1, "kind": "reference",
+                        "sample": f"first {n} results of the reference BPPivot (-q {theta} -k {k} -n {sample}) on the "
+                                  f"same graph; enumeration window {ms} ms (wall {wall:.1f} s incl. load)"}
+        # restatement of BPPivot (oracle/), same sample
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_lib
+        t0 = time.time()
+        _, dg, _ = oracle_lib.bppivot(g, k, theta, limit=sample, mode=2, keep=False)
+        dt = time.time() - t0
+        return {"value": dg.count / dt, "unit": "biplexes/s", "cores": 1, "kind": "port",
+                "sample": f"first {dg.count} results of the BPPivot restatement on the same graph ({dt:.1f} s)"}
+
+
+def run_steps(eng, cfg, mode, partition, steps, warmup, sync):
+    for _ in range(warmup):
+        eng.enumerate(k=cfg["k"], theta=cfg["theta"], limit=cfg["limit"], mode=mode, partition=partition, keep=True)
+    sync()
+    t0 = time.perf_counter()
+    last = None
+    for _ in range(steps):
+        last = eng.enumerate(k=cfg["k"], theta=cfg["theta"], limit=cfg["limit"], mode=mode, partition=partition,
+                             keep=True)
+    sync()
+    dt = time.perf_counter() - t0
+    return dt, last
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="cl50kx30k", choices=sorted(WORKLOADS))
+    ap.add_argument("--partition", default="seeded", choices=["auto", "global", "seeded"])
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched-mode run")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the enumerator is single-device and "
+              "its multi-GPU seed sharding is a design (DESIGN.md), not a measured path.", file=sys.stderr)
+        return 2
+
+    from gmkbe_amd import gen
+    from gmkbe_amd.engine import Engine
+
+    cfg = WORKLOADS[args.workload]
+    log(f"[bench] generating {args.workload}")
+    t0 = time.time()
+    g = gen.workload(args.workload)
+    log(f"[bench] graph |U|={g.n_left} |V|={g.n_right} |E|={g.n_edges} ({time.time() - t0:.1f} s)")
+    try:
+        import torch
+        sync = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
+    except Exception:
+        sync = lambda: None  # noqa: E731
+
+    eng = Engine(0)
+    eng.set_graph(g)
+    log("[bench] plain mode")
+    dt, last = run_steps(eng, cfg, "plain", args.partition, args.steps, args.warmup, sync)
+    st = last.stats
+    count = st["count"]
+    value = args.steps * count / dt
+    log(f"[bench] plain: {count} results/step, {dt / args.steps * 1e3:.2f} ms/step, {value:.3e} biplexes/s")
+
+    batched = None
+    if not args.no_batched:
+        log("[bench] batched mode")
+        dtb, lastb = run_steps(eng, cfg, "batched", args.partition, args.steps, args.warmup, sync)
+        vb = args.steps * lastb.stats["count"] / dtb
+        batched = {"value": vb, "ms_per_step": dtb / args.steps * 1e3, "count": lastb.stats["count"],
+                   "speedup_vs_plain": vb / value}
+        log(f"[bench] batched: {vb:.3e} biplexes/s ({vb / value:.2f}x plain)")
+    eng.close()
+
+    # dominant kernel of the measured (plain) run
+    names = ("bb_expand", "thin_build_families", "thin_check", "partition_build")
+    ms_k = st["ms_kernel"]
+    i = max(range(4), key=lambda q: ms_k[q])
+    launches = max(1, st["launches_kernel"][i])
+    bytes_per_launch = st["bytes_kernel"][i] / launches
+    ms_per_launch = ms_k[i] / launches
+    achieved = bytes_per_launch / (ms_per_launch * 1e-3) / 1e9 if ms_per_launch > 0 else 0.0
+    roofline = {"bound": "hbm", "kernel": names[i], "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profiles(names[i], args.workload),
+                "launches": launches, "algo_bytes_per_launch": bytes_per_launch, "ms_per_launch": ms_per_launch}
+
+    cpu = None
+    if not args.no_cpu:
+        log("[bench] CPU baseline")
+        cpu = cpu_baseline(g, cfg["k"], cfg["theta"], cfg["cpu_sample"])
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "biplexes/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic",
+        "config": {"workload": cfg["desc"], "name": args.workload, "k": cfg["k"], "theta": cfg["theta"],
+                   "limit": cfg["limit"], "mode": "plain", "partition": args.partition,
+                   "graph": {"n_left": g.n_left, "n_right": g.n_right, "edges": g.n_edges}},
+        "count_per_step": count,
+        "hit_limit": bool(st["hit_limit"]),
+        "digest": {"fp_sum": str(st["fp_sum"]), "fp_xor": str(st["fp_xor"])},
+        "kernels": {names[q]: {"ms": ms_k[q], "launches": st["launches_kernel"][q], "algo_bytes": st["bytes_kernel"][q]}
+                    for q in range(4)},
+        "batched": batched,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -498,10 +498,9 @@ finish:
   const u64 tsum = wave_sum_u64(st.intersections);
   const u64 bsum = wave_sum_u64(st.bytes);
   if (lane == 0) {
-    u64* sp = a.stat + (sidx % NSTRIPE) * 4;
+    u64* sp = a.stat + (sidx % NSTRIPE) * NSTAT;
     atomicAdd((unsigned long long*)&sp[0], (unsigned long long)st.states);
     atomicAdd((unsigned long long*)&sp[1], (unsigned long long)tsum);
-    atomicAdd((unsigned long long*)&sp[2], (unsigned long long)st.leaves);
     atomicAdd((unsigned long long*)&sp[3], (unsigned long long)bsum);
   }
 }

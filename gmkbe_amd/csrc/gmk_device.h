@@ -14,6 +14,7 @@ constexpr u32 FLAG_LIMIT = 1u;
 constexpr u32 FLAG_RES_OVERFLOW = 2u;
 constexpr u32 FLAG_TABLE_FULL = 4u;
 constexpr int NSTRIPE = 64;  // striped counters (stat and fingerprint atomics)
+constexpr int NSTAT = 8;     // counters per stripe
 
 // Fingerprint constants: fp(L, R) = fmix64(sum mix64(id) + |L| A + |R| B).
 constexpr u64 FP_A = 0x2545F4914F6CDD1Dull;
@@ -63,7 +64,7 @@ struct ExpandArgs {
   u64* res_count;       // results emitted (claims; may exceed limit)
   u64 limit;
   u64* fp;              // [NSTRIPE][2] fingerprint sum / xor
-  u64* stat;            // [NSTRIPE][4] states, intersections, leaves, bytes
+  u64* stat;            // [NSTRIPE][8] states, intersections, leaves, bytes per kernel (3..6)
   u32* flags;
 };
 

@@ -227,6 +227,8 @@ int gmk_run_bb(gmk_ctx* c, const PartitionSet& ps, const gmk_params* prm, gmk_st
     HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
     rs.ms_expand += ms;
     ++rs.launches;
+    rs.ms_kernel[0] += ms;
+    ++rs.launches_kernel[0];
     const RunCounters& h = *c->h_cnt;
     if (h.flags & FLAG_RES_OVERFLOW) {
       c->err = "result buffer overflow (raise result_capacity_words)";
@@ -298,9 +300,14 @@ extern "C" int gmk_enumerate(gmk_ctx* c, const gmk_params* prm, gmk_stats* st) {
   for (int i = 0; i < NSTRIPE; ++i) {
     st->fp_sum += h.fp[2 * i];
     st->fp_xor ^= h.fp[2 * i + 1];
-    st->states += h.stat[4 * i];
-    st->intersections += h.stat[4 * i + 1];
-    st->algo_bytes += h.stat[4 * i + 3];
+    st->states += h.stat[NSTAT * i];
+    st->intersections += h.stat[NSTAT * i + 1];
+    for (int q = 0; q < 4; ++q) st->bytes_kernel[q] += h.stat[NSTAT * i + 3 + q];
+  }
+  for (int q = 0; q < 4; ++q) {
+    st->algo_bytes += st->bytes_kernel[q];
+    st->ms_kernel[q] = rs.ms_kernel[q];
+    st->launches_kernel[q] = rs.launches_kernel[q];
   }
   st->fp_sum += rs.fp_sum_extra;
   st->fp_xor ^= rs.fp_xor_extra;

@@ -25,6 +25,8 @@ struct RunState {
   u64 launches = 0;
   bool hit_limit = false;
   u64 fp_sum_extra = 0, fp_xor_extra = 0;
+  double ms_kernel[4] = {0, 0, 0, 0};
+  u64 launches_kernel[4] = {0, 0, 0, 0};
 };
 
 // Whole graph as one partition (local IDs = global IDs, right side shifted).
@@ -53,7 +55,7 @@ struct RunCounters {
   u32 out_n, retry_n, flags, pad;
   u64 res_words, res_count;
   u64 fp[NSTRIPE * 2];
-  u64 stat[NSTRIPE * 4];
+  u64 stat[NSTRIPE * 8];  // 0 states, 1 intersections, 2 emitted leaves, 3-6 bytes per kernel
 };
 
 struct DeviceVec {

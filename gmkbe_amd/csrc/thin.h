@@ -96,22 +96,33 @@ GMK_HD void unrank(u64 r, u32 c, u64 n, u32* idx) {
   }
 }
 
+// Algorithmic byte count of the memory the functions below touch: every array
+// element read or written counts its size (binary-search probes included).
+struct Cost {
+  u64 bytes = 0;
+};
+
 // Sorted-list membership.
-GMK_HD bool contains(const u32* a, u32 n, u32 x) {
+GMK_HD bool contains(const u32* a, u32 n, u32 x, Cost& cost) {
   u32 lo = 0, hi = n;
+  u32 probes = 0;
   while (lo < hi) {
     const u32 mid = (lo + hi) >> 1;
+    ++probes;
     if (a[mid] < x) lo = mid + 1; else hi = mid;
   }
-  return lo < n && a[lo] == x;
+  const bool hit = lo < n && a[lo] == x;
+  cost.bytes += 4ull * (probes + (lo < n ? 1 : 0));
+  return hit;
 }
 
 // i-th vertex of [b_lo, b_hi) that is not in the sorted list nl (size d).
-GMK_HD u32 far_select(const u32* nl, u32 d, u32 b_lo, u64 i) {
+GMK_HD u32 far_select(const u32* nl, u32 d, u32 b_lo, u64 i, Cost& cost) {
   // p = number of members m_q with (m_q - b_lo - q) <= i
   u32 lo = 0, hi = d;
   while (lo < hi) {
     const u32 mid = (lo + hi) >> 1;
+    cost.bytes += 4;
     if ((u64)(nl[mid] - b_lo) - mid <= i) lo = mid + 1; else hi = mid;
   }
   return (u32)(b_lo + i + lo);
@@ -143,13 +154,6 @@ struct Graph {
 
 // Threat test: can some w in side A outside L be added to (L, R)?
 //   R = free (sorted, nfree) + ch (nch chosen, each with its type size tsz).
-GMK_HD bool member_of_R(const u32* freel, u32 nfree, const u32* ch, u32 nch, u32 y) {
-  if (contains(freel, nfree, y)) return true;
-  for (u32 i = 0; i < nch; ++i)
-    if (ch[i] == y) return true;
-  return false;
-}
-
 GMK_HD bool in_tuple(const u32* L, u32 j, u32 w) {
   for (u32 i = 0; i < j; ++i)
     if (L[i] == w) return true;
@@ -158,14 +162,15 @@ GMK_HD bool in_tuple(const u32* L, u32 j, u32 w) {
 
 // Does w miss at most k vertices of R, and is w adjacent to every saturated one?
 GMK_HD bool addable(const Graph& g, u32 w, const u32* freel, u32 nfree, const u32* ch, const u32* ctsz, u32 nch,
-                    u32 k) {
+                    u32 k, Cost& cost) {
   const u32* nw = g.adj + g.off[w];
   const u32 dw = g.off[w + 1] - g.off[w];
+  cost.bytes += 8;
   const u32 rs = nfree + nch;
   if (dw + k < rs) return false;
   u32 miss = 0;
   for (u32 i = 0; i < nch; ++i) {
-    if (!contains(nw, dw, ch[i])) {
+    if (!contains(nw, dw, ch[i], cost)) {
       if (ctsz[i] >= k) return false;  // saturated member missed
       if (++miss > k) return false;
     }
@@ -173,11 +178,12 @@ GMK_HD bool addable(const Graph& g, u32 w, const u32* freel, u32 nfree, const u3
   // free vertices have type 0 (< k for k >= 1): only their number matters
   if (nfree <= dw) {
     for (u32 i = 0; i < nfree; ++i)
-      if (!contains(nw, dw, freel[i]) && ++miss > k) return false;
+      if (!contains(nw, dw, freel[i], cost) && ++miss > k) return false;
   } else {
     // count free vertices adjacent to w by scanning N(w)
     u32 hit = 0;
-    for (u32 i = 0; i < dw; ++i) hit += contains(freel, nfree, nw[i]) ? 1 : 0;
+    cost.bytes += 4ull * dw;
+    for (u32 i = 0; i < dw; ++i) hit += contains(freel, nfree, nw[i], cost) ? 1 : 0;
     miss += nfree - hit;
     if (miss > k) return false;
   }
@@ -186,20 +192,22 @@ GMK_HD bool addable(const Graph& g, u32 w, const u32* freel, u32 nfree, const u3
 
 // Returns true iff (L, R) is maximal on side A.
 GMK_HD bool side_a_maximal(const Graph& g, const u32* L, u32 j, const u32* freel, u32 nfree, const u32* ch,
-                           const u32* ctsz, u32 nch, u32 k) {
+                           const u32* ctsz, u32 nch, u32 k, Cost& cost) {
   const u32 rs = nfree + nch;
   // saturated chosen vertex with the smallest degree
   u32 best = 0xFFFFFFFFu, bestdeg = 0xFFFFFFFFu;
   for (u32 i = 0; i < nch; ++i)
     if (ctsz[i] >= k) {
       const u32 d = g.off[ch[i] + 1] - g.off[ch[i]];
+      cost.bytes += 8;
       if (d < bestdeg) { bestdeg = d; best = ch[i]; }
     }
   if (best != 0xFFFFFFFFu) {
     for (u32 e = g.off[best]; e < g.off[best + 1]; ++e) {
       const u32 w = g.adj[e];
+      cost.bytes += 4;
       if (in_tuple(L, j, w)) continue;
-      if (addable(g, w, freel, nfree, ch, ctsz, nch, k)) return false;
+      if (addable(g, w, freel, nfree, ch, ctsz, nch, k, cost)) return false;
     }
     return true;
   }
@@ -213,6 +221,7 @@ GMK_HD bool side_a_maximal(const Graph& g, const u32* L, u32 j, const u32* freel
   u32 np = 0;
   auto consider = [&](u32 y) {
     const u32 d = g.off[y + 1] - g.off[y];
+    cost.bytes += 8;
     if (np < k + 1) { pick[np] = y; pdeg[np] = d; ++np; }
     else {
       u32 worst = 0;
@@ -221,12 +230,14 @@ GMK_HD bool side_a_maximal(const Graph& g, const u32* L, u32 j, const u32* freel
     }
   };
   for (u32 i = 0; i < nch; ++i) consider(ch[i]);
+  cost.bytes += 4ull * nfree;
   for (u32 i = 0; i < nfree; ++i) consider(freel[i]);
   for (u32 q = 0; q < np; ++q)
     for (u32 e = g.off[pick[q]]; e < g.off[pick[q] + 1]; ++e) {
       const u32 w = g.adj[e];
+      cost.bytes += 4;
       if (in_tuple(L, j, w)) continue;
-      if (addable(g, w, freel, nfree, ch, ctsz, nch, k)) return false;
+      if (addable(g, w, freel, nfree, ch, ctsz, nch, k, cost)) return false;
     }
   return true;
 }

@@ -12,10 +12,10 @@ namespace thin {
 // arena allocation succeeded, so a failed tuple emits nothing.
 template <class Arena, class Fams>
 GMK_HD bool build_families(const Graph& g, const u32* L, u32 j, u32 k, u32 min_b, u32 tuple_idx, Arena& ar,
-                           Fams& fo) {
+                           Fams& fo, Cost& cost) {
   const u32 full = (1u << j) - 1u;
   u32 pos[MAXJ], end[MAXJ];
-  for (u32 i = 0; i < j; ++i) { pos[i] = g.off[L[i]]; end[i] = g.off[L[i] + 1]; }
+  for (u32 i = 0; i < j; ++i) { pos[i] = g.off[L[i]]; end[i] = g.off[L[i] + 1]; cost.bytes += 12 + 4ull * (end[i] - pos[i]); }
   u32 mcnt[1 << MAXJ];
   for (u32 m = 0; m <= full; ++m) mcnt[m] = 0;
   u32 d = 0;
@@ -58,6 +58,9 @@ GMK_HD bool build_families(const Graph& g, const u32* L, u32 j, u32 k, u32 min_b
   }
   u32* nl = ar.ptr(nl_off);
   u32* pools = ar.ptr(pool_off);
+  // second merge pass reads the lists again and writes N(L) and the pools
+  for (u32 i = 0; i < j; ++i) cost.bytes += 4ull * (end[i] - g.off[L[i]]);
+  cost.bytes += 8ull * d;
   // pass 2: write N(L) (ascending) and the pools grouped by type (each ascending)
   for (u32 i = 0; i < j; ++i) pos[i] = g.off[L[i]];
   {
@@ -132,6 +135,7 @@ GMK_HD bool build_families(const Graph& g, const u32* L, u32 j, u32 k, u32 min_b
         }
         if (rs >= min_b && rs > 0 && used <= (u32)MAXT) {
           Family* f = fo.next();
+          cost.bytes += sizeof(Family);
           if (f) {
             for (u32 i = 0; i < j; ++i) f->L[i] = L[i];
             f->j = j; f->nfree = nfree; f->rsize = (u32)rs;

@@ -41,10 +41,19 @@ int run_chunk(ThinRun& T, const u32* tuples, u32 count, u32 j) {
   ThinArgs& a = T.a;
   a.ntuples = count;
   a.j = j;
+  HIPCHK(c, hipEventRecord(c->ev0, s));
   HIPCHK(c, launch_thin_build(a, s));
+  HIPCHK(c, hipEventRecord(c->ev1, s));
   ThinCounters tc;
   HIPCHK(c, hipMemcpyAsync(&tc, c->tcnt.p, sizeof(tc), hipMemcpyDeviceToHost, s));
   HIPCHK(c, hipStreamSynchronize(s));
+  {
+    float ms = 0;
+    HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    T.rs->ms_expand += ms;
+    T.rs->ms_kernel[1] += ms;
+    T.rs->launches_kernel[1]++;
+  }
   T.rs->launches++;
   if (tc.tflags) {
     if (count == 1) { c->err = "one tuple's neighbour lists exceed the arena"; return GMK_EOVERFLOW; }
@@ -79,6 +88,8 @@ int run_chunk(ThinRun& T, const u32* tuples, u32 count, u32 j) {
     float ms = 0;
     HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
     T.rs->ms_expand += ms;
+    T.rs->ms_kernel[2] += ms;
+    T.rs->launches_kernel[2]++;
     T.rs->launches++;
     if (c->h_cnt->flags & FLAG_RES_OVERFLOW) { c->err = "result buffer overflow (raise result_capacity_words)"; return GMK_EOVERFLOW; }
     if (c->h_cnt->flags & FLAG_LIMIT) { T.stop = true; T.rs->hit_limit = true; return GMK_OK; }

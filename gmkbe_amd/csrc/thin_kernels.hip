@@ -37,8 +37,10 @@ __global__ __launch_bounds__(256) void thin_build_families(ThinArgs a) {
   if (t >= a.ntuples) return;
   DevArena ar{a.arena, a.arena_cap, a.arena_used};
   DevFamilies fo{a.fams, a.fam_cap, a.fam_n, a.tflags};
-  if (!thin::build_families(a.g, a.tuples + (u64)t * a.j, a.j, a.k, a.min_b, t, ar, fo))
+  thin::Cost cost;
+  if (!thin::build_families(a.g, a.tuples + (u64)t * a.j, a.j, a.k, a.min_b, t, ar, fo, cost))
     atomicOr(a.tflags, FLAG_RES_OVERFLOW);
+  atomicAdd((unsigned long long*)&a.stat[(t / 64 % NSTRIPE) * NSTAT + 4], (unsigned long long)cost.bytes);
 }
 
 __global__ __launch_bounds__(256) void thin_check(ThinArgs a) {
@@ -48,6 +50,7 @@ __global__ __launch_bounds__(256) void thin_check(ThinArgs a) {
   u32 ch[thin::MAXJ * 3], ctsz[thin::MAXJ * 3];
   u32 nch = 0;
   const thin::Family* f = nullptr;
+  thin::Cost cost;
   if (r < a.r1 && !(__hip_atomic_load(a.flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & FLAG_LIMIT)) {
     // family containing rank r
     u32 lo = 0, hi = a.nfam;
@@ -66,12 +69,14 @@ __global__ __launch_bounds__(256) void thin_check(ThinArgs a) {
       const u32 tsz = thin::popc(f->mask[t]);
       for (u32 q = 0; q < f->cnt[t]; ++q) {
         const u32 y = f->poff[t] == thin::FAR
-                          ? thin::far_select(a.arena + f->nl_off, f->nl_size, a.g.b_lo, idx[q])
+                          ? thin::far_select(a.arena + f->nl_off, f->nl_size, a.g.b_lo, idx[q], cost)
                           : a.arena[f->poff[t] + idx[q]];
+        cost.bytes += 4;
         ch[nch] = y; ctsz[nch] = tsz; ++nch;
       }
     }
-    emit = thin::side_a_maximal(a.g, f->L, f->j, a.arena + f->free_off, f->nfree, ch, ctsz, nch, a.k);
+    cost.bytes += 8ull * 8 + 16ull * f->ntypes;  // prefix probes, family header and type fields
+    emit = thin::side_a_maximal(a.g, f->L, f->j, a.arena + f->free_off, f->nfree, ch, ctsz, nch, a.k, cost);
   }
   // per-lane emission (records are independent)
   u64 h = 0;
@@ -95,8 +100,10 @@ __global__ __launch_bounds__(256) void thin_check(ThinArgs a) {
       u64* fs = a.fp + ((blockIdx.x * 4 + (threadIdx.x >> 6)) % NSTRIPE) * 2;
       atomicAdd((unsigned long long*)&fs[0], (unsigned long long)fpv);
       atomicXor((unsigned long long*)&fs[1], (unsigned long long)fpv);
+      cost.bytes += 4ull * f->nfree;  // free list read for the fingerprint / record
       if (a.keep) {
         const u64 len = 2ull + nl + nr;
+        cost.bytes += 4ull * len;
         const u64 pos = atomicAdd((unsigned long long*)a.res_words, (unsigned long long)len);
         if (pos + len > a.res_cap) {
           atomicOr(a.flags, FLAG_RES_OVERFLOW);
@@ -121,11 +128,12 @@ __global__ __launch_bounds__(256) void thin_check(ThinArgs a) {
   (void)lane;
   const u64 e = emit ? 1 : 0;
   const u64 tested = (r < a.r1 && f) ? 1 : 0;
-  const u64 se = wave_sum_u64(e), stt = wave_sum_u64(tested);
+  const u64 se = wave_sum_u64(e), stt = wave_sum_u64(tested), sb = wave_sum_u64(cost.bytes);
   if (lane == 0 && stt) {
-    u64* sp = a.stat + ((blockIdx.x * 4 + (threadIdx.x >> 6)) % NSTRIPE) * 4;
-    atomicAdd((unsigned long long*)&sp[0], (unsigned long long)stt);
+    u64* sp = a.stat + ((blockIdx.x * 4 + (threadIdx.x >> 6)) % NSTRIPE) * NSTAT;
+    atomicAdd((unsigned long long*)&sp[1], (unsigned long long)stt);
     atomicAdd((unsigned long long*)&sp[2], (unsigned long long)se);
+    atomicAdd((unsigned long long*)&sp[5], (unsigned long long)sb);
   }
 }
 

@@ -69,9 +69,14 @@ typedef struct {
   uint64_t launches;       /* kernel launches */
   uint64_t dedup_inserts, dedup_probes, dedup_dups;
   double ms;               /* enumeration window (after graph upload) */
-  double ms_expand;        /* device time inside the expansion kernel */
+  double ms_expand;        /* device time of all enumeration kernels */
   int32_t hit_limit;
   int32_t pad;
+  /* per kernel: 0 state expansion, 1 thin family construction,
+   * 2 thin combination check, 3 partition construction */
+  double ms_kernel[4];       /* device time (HIP events around each launch) */
+  uint64_t launches_kernel[4];
+  uint64_t bytes_kernel[4];  /* algorithmic bytes read + written */
 } gmk_stats;
 
 /* Create a context on HIP device `device` (-d, src/main.cu:1765-1766). */

@@ -46,7 +46,8 @@ extern "C" long long thin_model(unsigned n, unsigned n_left, const unsigned* off
         for (u32 i = 1; i < j; ++i) L[i] = idx[i];
         HostArena ar;
         HostFams fo;
-        build_families(g, L, j, k, min_b, 0, ar, fo);
+        Cost cost;
+        build_families(g, L, j, k, min_b, 0, ar, fo, cost);
         for (const Family& f : fo.f) {
           for (u64 r = 0; r < f.total; ++r) {
             u64 rr = r;
@@ -58,12 +59,12 @@ extern "C" long long thin_model(unsigned n, unsigned n_left, const unsigned* off
               u32 id[3];
               unrank(rt, f.cnt[t], f.psize[t], id);
               for (u32 q = 0; q < f.cnt[t]; ++q) {
-                ch[nch] = f.poff[t] == FAR ? far_select(ar.ptr(f.nl_off), f.nl_size, g.b_lo, id[q]) : ar.v[f.poff[t] + id[q]];
+                ch[nch] = f.poff[t] == FAR ? far_select(ar.ptr(f.nl_off), f.nl_size, g.b_lo, id[q], cost) : ar.v[f.poff[t] + id[q]];
                 ct[nch] = popc(f.mask[t]);
                 ++nch;
               }
             }
-            if (!side_a_maximal(g, f.L, f.j, ar.ptr(f.free_off), f.nfree, ch, ct, nch, k)) continue;
+            if (!side_a_maximal(g, f.L, f.j, ar.ptr(f.free_off), f.nfree, ch, ct, nch, k, cost)) continue;
             std::vector<u32> R(ar.ptr(f.free_off), ar.ptr(f.free_off) + f.nfree);
             R.insert(R.end(), ch, ch + nch);
             std::sort(R.begin(), R.end());
