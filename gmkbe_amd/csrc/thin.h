@@ -47,7 +47,7 @@ typedef uint64_t u64;
 constexpr int MAXJ = 6;       // tuple size limit (2k with k <= 3)
 constexpr int MAXT = 18;      // chosen types per family (<= j * k)
 constexpr u64 SAT = 1ull << 62;  // saturation value of combination counts
-constexpr u32 FAR = 0xFFFFFFFFu; // pool offset marking the far pool
+constexpr u64 FARP = ~0ull;       // pool address marking the far pool
 
 GMK_HD int popc(u32 x) {
   int c = 0;
@@ -128,22 +128,31 @@ GMK_HD u32 far_select(const u32* nl, u32 d, u32 b_lo, u64 i, Cost& cost) {
   return (u32)(b_lo + i + lo);
 }
 
+// A family: one tuple with one maximal count vector.  Lists are addressed by
+// pointer (arena or, for one-vertex tuples, the CSR itself).
 struct Family {
   u32 L[MAXJ];
   u32 j;
   u32 nfree;          // free vertices (type 0)
   u32 ntypes;         // types with c > 0
   u32 rsize;          // |R| = nfree + sum c
-  u64 free_off;       // arena offset of the free list (sorted)
-  u64 nl_off;         // arena offset of N(L) sorted (far-pool select)
+  const u32* freep;   // free list, ascending
+  const u32* nlp;     // N(L) ascending (far-pool select); valid when a far pool is used
   u32 nl_size;
   u32 tuple;          // index of the tuple in its batch
   u32 mask[MAXT];
   u32 cnt[MAXT];
   u32 psize[MAXT];
-  u64 poff[MAXT];     // arena offset of the pool (sorted), FAR for the far pool
+  u64 pool[MAXT];     // address of the pool, FARP for the far pool
   u64 total;          // number of concrete R (product of binomials), saturating
 };
+
+// The y of index i in pool t of family f.
+GMK_HD u32 pool_member(const Family& f, u32 t, u64 i, u32 b_lo, Cost& cost) {
+  cost.bytes += 4;
+  if (f.pool[t] == FARP) return far_select(f.nlp, f.nl_size, b_lo, i, cost);
+  return ((const u32*)f.pool[t])[i];
+}
 
 struct Graph {
   const u32* off;

@@ -37,5 +37,6 @@ struct ThinArgs {
 
 hipError_t launch_thin_build(const ThinArgs& a, hipStream_t s);
 hipError_t launch_thin_check(const ThinArgs& a, hipStream_t s);
+hipError_t launch_thin_scan(const thin::Family* fams, u32 nfam, u64* prefix, hipStream_t s);
 
 }  // namespace gmk

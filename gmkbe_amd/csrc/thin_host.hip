@@ -64,18 +64,11 @@ int run_chunk(ThinRun& T, const u32* tuples, u32 count, u32 j) {
   }
   const u32 nfam = tc.fam_n;
   if (nfam == 0) return GMK_OK;
-  T.totals.resize(nfam);
-  HIPCHK(c, hipMemcpy2DAsync(T.totals.data(), 8, (char*)c->fams.p + offsetof(Family, total), sizeof(Family), 8, nfam,
-                             hipMemcpyDeviceToHost, s));
+  HIPCHK(c, launch_thin_scan((const Family*)c->fams.p, nfam, (u64*)c->prefix.p, s));
+  u64 total = 0;
+  HIPCHK(c, hipMemcpyAsync(&total, (u64*)c->prefix.p + nfam, 8, hipMemcpyDeviceToHost, s));
   HIPCHK(c, hipStreamSynchronize(s));
-  T.prefix.assign(nfam + 1, 0);
-  for (u32 i = 0; i < nfam; ++i) {
-    const u64 t = T.prefix[i] + T.totals[i];
-    T.prefix[i + 1] = (t < T.prefix[i] || t > thin::SAT) ? thin::SAT : t;
-  }
-  HIPCHK(c, hipMemcpyAsync(c->prefix.p, T.prefix.data(), (nfam + 1) * 8, hipMemcpyHostToDevice, s));
   a.nfam = nfam;
-  const u64 total = T.prefix[nfam];
   RunCounters* dc = (RunCounters*)c->counters.p;
   for (u64 r0 = 0; r0 < total; r0 += RCH) {
     a.r0 = r0;

@@ -32,15 +32,142 @@ struct DevFamilies {
   }
 };
 
+// One lane per tuple: one-vertex tuples (O(1): the lists are the CSR row) and
+// tuples whose far pool needs N(L) in ascending order (sequential merge).
 __global__ __launch_bounds__(256) void thin_build_families(ThinArgs a) {
   const u32 t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= a.ntuples) return;
-  DevArena ar{a.arena, a.arena_cap, a.arena_used};
-  DevFamilies fo{a.fams, a.fam_cap, a.fam_n, a.tflags};
   thin::Cost cost;
-  if (!thin::build_families(a.g, a.tuples + (u64)t * a.j, a.j, a.k, a.min_b, t, ar, fo, cost))
-    atomicOr(a.tflags, FLAG_RES_OVERFLOW);
-  atomicAdd((unsigned long long*)&a.stat[(t / 64 % NSTRIPE) * NSTAT + 4], (unsigned long long)cost.bytes);
+  if (t < a.ntuples) {
+    DevArena ar{a.arena, a.arena_cap, a.arena_used};
+    DevFamilies fo{a.fams, a.fam_cap, a.fam_n, a.tflags};
+    if (!thin::build_families(a.g, a.tuples + (u64)t * a.j, a.j, a.k, a.min_b, t, ar, fo, cost))
+      atomicOr(a.tflags, FLAG_RES_OVERFLOW);
+  }
+  const u64 b = wave_sum_u64(cost.bytes);
+  if ((threadIdx.x & 63) == 0 && b)
+    atomicAdd((unsigned long long*)&a.stat[((t >> 6) % NSTRIPE) * NSTAT + 4], (unsigned long long)b);
+}
+
+// One wave per tuple of j >= 2 vertices without a far pool (j > k): lanes sweep
+// the members' neighbour lists in coalesced chunks; an element is counted once,
+// by the first list that holds it, with its miss type found by binary search in
+// the other lists.  Pass 1 counts the types, pass 2 writes the pools (the free
+// list in ascending order, other pools in any order); lane 0 then emits the
+// families.
+__global__ __launch_bounds__(256) void thin_build_wave(ThinArgs a) {
+  __shared__ u32 s_off[4][1 << thin::MAXJ];  // pass 1: counts; pass 2: pool offsets
+  __shared__ u32 s_cur[4][1 << thin::MAXJ];  // pass 2: fill cursors
+  __shared__ u64 s_base[4];
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const u32 t = blockIdx.x * 4 + wv;
+  if (t >= a.ntuples) return;  // uniform per wave
+  const u32 j = a.j, full = (1u << j) - 1u;
+  const u32* L = a.tuples + (u64)t * j;
+  thin::Cost cost;
+  u32 lo[thin::MAXJ], dg[thin::MAXJ];
+  for (u32 i = 0; i < j; ++i) { lo[i] = a.g.off[L[i]]; dg[i] = a.g.off[L[i] + 1] - lo[i]; }
+  s_off[wv][lane] = 0;
+  s_cur[wv][lane] = 0;
+  wave_barrier();
+  u32 cnt_lane = 0;  // lane m keeps the count of type m
+  for (int pass = 0; pass < 2; ++pass) {
+    u32* pools = pass ? a.arena + s_base[wv] : nullptr;
+    u32 free_fill = 0;
+    for (u32 i = 0; i < j; ++i) {
+      const u32* li = a.g.adj + lo[i];
+      for (u32 base = 0; base < dg[i]; base += 64) {
+        const u32 e = base + lane;
+        const bool valid = e < dg[i];
+        u32 hit = 1u << i;
+        bool first = valid;
+        u32 y = 0;
+        if (valid) {
+          y = li[e];
+          cost.bytes += 4;
+          for (u32 q = 0; q < j; ++q) {
+            if (q == i) continue;
+            if (thin::contains(a.g.adj + lo[q], dg[q], y, cost)) {
+              hit |= 1u << q;
+              if (q < i) first = false;
+            }
+          }
+        }
+        const u32 m = full & ~hit;
+        if (pass == 0) {
+          if (first) atomicAdd(&s_off[wv][m], 1u);
+        } else {
+          // free vertices come only from list 0: ordered compaction keeps them ascending
+          const u64 fb = __ballot(first && m == 0);
+          if (first) {
+            const u32 at = m == 0 ? free_fill + (u32)__popcll(fb & ((1ull << lane) - 1ull))
+                                  : s_off[wv][m] + atomicAdd(&s_cur[wv][m], 1u);
+            pools[at] = y;
+            cost.bytes += 4;
+          }
+          free_fill += (u32)__popcll(fb);
+        }
+      }
+    }
+    wave_barrier();
+    if (pass == 1) break;
+    cnt_lane = s_off[wv][lane];
+    if (lane == 0) {
+      u32 d = 0;
+      u64 maxr = s_off[wv][0];
+      for (u32 mm = 0; mm <= full; ++mm) {
+        const u32 cm = s_off[wv][mm];
+        d += cm;
+        if (mm > 0 && mm < full && (u32)thin::popc(mm) <= a.k) maxr += cm < a.k ? cm : a.k;
+      }
+      u64 at = thin::SAT;
+      if (maxr >= a.min_b && maxr > 0) {
+        DevArena ar{a.arena, a.arena_cap, a.arena_used};
+        at = ar.alloc(d);
+        if (at == thin::SAT) atomicOr(a.tflags, FLAG_RES_OVERFLOW);
+      }
+      s_base[wv] = at;
+      u32 acc = 0;
+      for (u32 mm = 0; mm <= full; ++mm) { const u32 cm = s_off[wv][mm]; s_off[wv][mm] = acc; acc += cm; }
+    }
+    wave_barrier();
+    if (s_base[wv] == thin::SAT) break;
+  }
+  if (s_base[wv] != thin::SAT) {
+    u32 mcnt[1 << thin::MAXJ];
+    u32 d = 0;
+    for (u32 mm = 0; mm <= full; ++mm) { mcnt[mm] = (u32)__shfl((int)cnt_lane, (int)mm, 64); d += mcnt[mm]; }
+    if (lane == 0) {
+      u64 poolp[1 << thin::MAXJ];
+      for (u32 mm = 0; mm <= full; ++mm) poolp[mm] = (u64)(a.arena + s_base[wv] + s_off[wv][mm]);
+      DevFamilies fo{a.fams, a.fam_cap, a.fam_n, a.tflags};
+      thin::emit_families(L, j, a.k, a.min_b, t, mcnt, (u64)(a.g.b_hi - a.g.b_lo) - d, poolp, nullptr, d, fo, cost);
+    }
+  }
+  const u64 b = wave_sum_u64(cost.bytes);
+  if (lane == 0 && b) atomicAdd((unsigned long long*)&a.stat[(t % NSTRIPE) * NSTAT + 4], (unsigned long long)b);
+}
+
+// Exclusive prefix sums of the family totals (saturating), one block.
+__global__ __launch_bounds__(1024) void thin_scan_totals(const thin::Family* fams, u32 nfam, u64* prefix) {
+  __shared__ u64 part[1024];
+  const u32 t = threadIdx.x;
+  const u32 seg = (nfam + 1023) / 1024;
+  const u32 b = t * seg < nfam ? t * seg : nfam;
+  const u32 e = b + seg < nfam ? b + seg : nfam;
+  auto add = [](u64 x, u64 y) -> u64 { const u64 z = x + y; return (z < x || z > thin::SAT) ? thin::SAT : z; };
+  u64 s = 0;
+  for (u32 i = b; i < e; ++i) s = add(s, fams[i].total);
+  part[t] = s;
+  __syncthreads();
+  if (t == 0) {
+    u64 run = 0;
+    for (u32 i = 0; i < 1024; ++i) { const u64 v = part[i]; part[i] = run; run = add(run, v); }
+  }
+  __syncthreads();
+  u64 run = part[t];
+  for (u32 i = b; i < e; ++i) { prefix[i] = run; run = add(run, fams[i].total); }
+  if (b < e && e == nfam) prefix[nfam] = run;
+  if (nfam == 0 && t == 0) prefix[0] = 0;
 }
 
 __global__ __launch_bounds__(256) void thin_check(ThinArgs a) {
@@ -68,15 +195,13 @@ __global__ __launch_bounds__(256) void thin_check(ThinArgs a) {
       thin::unrank(rt, f->cnt[t], f->psize[t], idx);
       const u32 tsz = thin::popc(f->mask[t]);
       for (u32 q = 0; q < f->cnt[t]; ++q) {
-        const u32 y = f->poff[t] == thin::FAR
-                          ? thin::far_select(a.arena + f->nl_off, f->nl_size, a.g.b_lo, idx[q], cost)
-                          : a.arena[f->poff[t] + idx[q]];
-        cost.bytes += 4;
-        ch[nch] = y; ctsz[nch] = tsz; ++nch;
+        ch[nch] = thin::pool_member(*f, t, idx[q], a.g.b_lo, cost);
+        ctsz[nch] = tsz;
+        ++nch;
       }
     }
     cost.bytes += 8ull * 8 + 16ull * f->ntypes;  // prefix probes, family header and type fields
-    emit = thin::side_a_maximal(a.g, f->L, f->j, a.arena + f->free_off, f->nfree, ch, ctsz, nch, a.k, cost);
+    emit = thin::side_a_maximal(a.g, f->L, f->j, f->freep, f->nfree, ch, ctsz, nch, a.k, cost);
   }
   // per-lane emission (records are independent)
   u64 h = 0;
@@ -91,7 +216,7 @@ __global__ __launch_bounds__(256) void thin_check(ThinArgs a) {
       for (u32 i = 1; i < nch; ++i)
         for (u32 q = i; q > 0 && ch[q - 1] > ch[q]; --q) { const u32 tmp = ch[q]; ch[q] = ch[q - 1]; ch[q - 1] = tmp; }
       const u32 nr = f->nfree + nch;
-      const u32* fr = a.arena + f->free_off;
+      const u32* fr = f->freep;
       for (u32 i = 0; i < f->j; ++i) h += mix64(f->L[i]);
       for (u32 i = 0; i < f->nfree; ++i) h += mix64(fr[i]);
       for (u32 i = 0; i < nch; ++i) h += mix64(ch[i]);
@@ -139,7 +264,15 @@ __global__ __launch_bounds__(256) void thin_check(ThinArgs a) {
 
 hipError_t launch_thin_build(const ThinArgs& a, hipStream_t s) {
   if (a.ntuples == 0) return hipSuccess;
-  hipLaunchKernelGGL(thin_build_families, dim3((a.ntuples + 255) / 256), dim3(256), 0, s, a);
+  if (a.j >= 2 && a.j > a.k)
+    hipLaunchKernelGGL(thin_build_wave, dim3((a.ntuples + 3) / 4), dim3(256), 0, s, a);
+  else
+    hipLaunchKernelGGL(thin_build_families, dim3((a.ntuples + 255) / 256), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_thin_scan(const thin::Family* fams, u32 nfam, u64* prefix, hipStream_t s) {
+  hipLaunchKernelGGL(thin_scan_totals, dim3(1), dim3(1024), 0, s, fams, nfam, prefix);
   return hipGetLastError();
 }
 

@@ -13,10 +13,11 @@
 using namespace gmk::thin;
 
 namespace {
+// Arena with stable addresses (families keep pointers into it).
 struct HostArena {
-  std::vector<u32> v;
-  u64 alloc(u64 n) { const u64 at = v.size(); v.resize(at + n); return at; }
-  u32* ptr(u64 off) { return v.data() + off; }
+  std::vector<std::vector<u32>> blocks;
+  u64 alloc(u64 n) { blocks.emplace_back(n ? n : 1); return blocks.size() - 1; }
+  u32* ptr(u64 id) { return blocks[id].data(); }
 };
 struct HostFams {
   std::vector<Family> f;
@@ -59,13 +60,13 @@ extern "C" long long thin_model(unsigned n, unsigned n_left, const unsigned* off
               u32 id[3];
               unrank(rt, f.cnt[t], f.psize[t], id);
               for (u32 q = 0; q < f.cnt[t]; ++q) {
-                ch[nch] = f.poff[t] == FAR ? far_select(ar.ptr(f.nl_off), f.nl_size, g.b_lo, id[q], cost) : ar.v[f.poff[t] + id[q]];
+                ch[nch] = pool_member(f, t, id[q], g.b_lo, cost);
                 ct[nch] = popc(f.mask[t]);
                 ++nch;
               }
             }
-            if (!side_a_maximal(g, f.L, f.j, ar.ptr(f.free_off), f.nfree, ch, ct, nch, k, cost)) continue;
-            std::vector<u32> R(ar.ptr(f.free_off), ar.ptr(f.free_off) + f.nfree);
+            if (!side_a_maximal(g, f.L, f.j, f.freep, f.nfree, ch, ct, nch, k, cost)) continue;
+            std::vector<u32> R(f.freep, f.freep + f.nfree);
             R.insert(R.end(), ch, ch + nch);
             std::sort(R.begin(), R.end());
             std::vector<u32> T(f.L, f.L + f.j);
