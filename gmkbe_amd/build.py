@@ -32,6 +32,11 @@ def build(verbose: bool = False) -> str:
     tmp = LIB + ".tmp"
     subprocess.run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp], check=True)
     os.replace(tmp, LIB)
+    # command-line driver over the C ABI
+    cli = os.path.join(OUT_DIR, "gmkbpe")
+    subprocess.run(["g++", "-std=c++20", "-O2", "-Wall", os.path.join(CSRC, "gmkbpe.cpp"), "-o", cli + ".tmp",
+                    "-L" + OUT_DIR, "-lgmk", "-Wl,-rpath,$ORIGIN"], check=True)
+    os.replace(cli + ".tmp", cli)
     return LIB
 
 

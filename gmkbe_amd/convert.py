@@ -16,6 +16,8 @@ Restates preprocess/graph_conver.cpp:500-817 (``Convert``):
 """
 from __future__ import annotations
 
+import os
+
 import numpy as np
 
 from .graphio import Graph
@@ -109,3 +111,25 @@ def write_edge_text(path: str, u: np.ndarray, v: np.ndarray) -> None:
     with open(path, "w") as f:
         f.write("\n".join(f"{a} {b}" for a, b in zip(np.asarray(u).tolist(), np.asarray(v).tolist())))
         f.write("\n")
+
+
+def main(argv=None) -> int:
+    """graph_convert -i <edge list> -o <prefix|file.graph.bin> (graph_conver.cpp:453-498, 813-816)."""
+    import argparse
+    from .graphio import write_graph_bin
+    ap = argparse.ArgumentParser(prog="graph_convert")
+    ap.add_argument("-i", required=True)
+    ap.add_argument("-o", required=True)
+    a = ap.parse_args(argv)
+    out = a.o if a.o.endswith(".graph.bin") else a.o + ".graph.bin"
+    d = os.path.dirname(out)
+    if d:
+        os.makedirs(d, exist_ok=True)
+    g = convert_file(a.i)
+    write_graph_bin(out, g)
+    print(f"Nodes:   {g.n}\n2|E|  :  {int(g.off[-1])}\nnLeft :  {g.n_left}\nnRight:  {g.n_right}")
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())

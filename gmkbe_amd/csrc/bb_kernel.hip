@@ -125,9 +125,22 @@ __device__ void emit_state(Smem<W>& sm, const PartDesc& P, const u32* maps, cons
   const u64* SL = setp(sm.set, 0, 0, W);
   const u64* SR = setp(sm.set, 1, 0, W);
   const u32 nl = popc_set<W>(SL, P.wl), nr = popc_set<W>(SR, P.wr);
+  // fingerprint first: it decides whether the record is a duplicate
+  u64 h = 0;
+  for (int side = 0; side < 2; ++side) {
+    const u64* Sx = side ? SR : SL;
+    const u32 wx = side ? P.wr : P.wl;
+    const u32* map = maps + (side ? P.map_r : P.map_l);
+    for (u32 i = 0; i < wx; ++i)
+      if ((Sx[i] >> lane) & 1ull) h += mix64(map[i * 64 + lane]);
+  }
+  h = wave_sum_u64(h);
+  const u64 fp = fmix64(h + (u64)nl * FP_A + (u64)nr * FP_B);
   u64 slot = 0;
-  if (lane == 0) slot = atomicAdd((unsigned long long*)a.res_count, 1ull);
+  if (lane == 0) slot = dedup_insert(a.dedup, fp, a.flags, blockIdx.x) ? atomicAdd((unsigned long long*)a.res_count, 1ull)
+                                                                    : ~0ull;
   slot = wave_bcast_u64(slot, 0);
+  if (slot == ~0ull) return;  // duplicate
   if (slot >= a.limit) {
     if (lane == 0) atomicOr(a.flags, FLAG_LIMIT);
     return;
@@ -139,37 +152,33 @@ __device__ void emit_state(Smem<W>& sm, const PartDesc& P, const u32* maps, cons
   pos = wave_bcast_u64(pos, 0);
   const bool write = a.keep && pos + len <= a.res_cap;
   if (a.keep && !write && lane == 0) atomicOr(a.flags, FLAG_RES_OVERFLOW);
-  u64 h = 0;
-  for (int side = 0; side < 2; ++side) {
-    const u64* Sx = side ? SR : SL;
-    const u32 wx = side ? P.wr : P.wl;
-    const u32* map = maps + (side ? P.map_r : P.map_l);
-    const u64 off = pos + 1 + (side ? 1 + nl : 0);
-    u32 before = 0;
-    for (u32 i = 0; i < wx; ++i) {
-      const u64 word = Sx[i];
-      if ((word >> lane) & 1ull) {
-        const u32 rank = before + __popcll(word & ((1ull << lane) - 1ull));
-        const u32 g = map[i * 64 + lane];
-        h += mix64(g);
-        if (write) a.res[off + rank] = g;
+  if (write) {
+    for (int side = 0; side < 2; ++side) {
+      const u64* Sx = side ? SR : SL;
+      const u32 wx = side ? P.wr : P.wl;
+      const u32* map = maps + (side ? P.map_r : P.map_l);
+      const u64 off = pos + 1 + (side ? 1 + nl : 0);
+      u32 before = 0;
+      for (u32 i = 0; i < wx; ++i) {
+        const u64 word = Sx[i];
+        if ((word >> lane) & 1ull) {
+          const u32 rank = before + __popcll(word & ((1ull << lane) - 1ull));
+          a.res[off + rank] = map[i * 64 + lane];
+        }
+        before += __popcll(word);
       }
-      before += __popcll(word);
     }
   }
-  h = wave_sum_u64(h);
   if (lane == 0) {
     if (write) {
       a.res[pos] = nl;
       a.res[pos + 1 + nl] = nr;
     }
-    const u64 fp = fmix64(h + (u64)nl * FP_A + (u64)nr * FP_B);
     u64* fs = a.fp + (blockIdx.x % NSTRIPE) * 2;
     atomicAdd((unsigned long long*)&fs[0], (unsigned long long)fp);
     atomicXor((unsigned long long*)&fs[1], (unsigned long long)fp);
   }
 }
-
 
 // Write one child.  The child is the parent with `moved` (on side ms_side)
 // moved from C to X, then (when inc != NONE) vertex inc of side us included.

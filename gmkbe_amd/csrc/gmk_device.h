@@ -45,6 +45,38 @@ struct PartDesc {
   u32 seed;     // seed vertex (global ID) or NONE for a global partition
 };
 
+// Result dedup table: open addressing over 64-bit fingerprints (0 = empty),
+// linear probing from a slot taken from a second mix of the key, insertion by
+// compare-and-swap.  A fingerprint found already present is a duplicate.
+struct DedupTable {
+  unsigned long long* slots;  // capacity = mask + 1 (power of two)
+  u64 mask;
+  u64* stats;                 // [NSTRIPE][4]: inserts, probes, duplicates
+  int on;
+};
+
+// Returns false when fp is already in the table (the result is a duplicate).
+__device__ __forceinline__ bool dedup_insert(const DedupTable& t, u64 fp, u32* flags, u32 stripe) {
+  if (!t.on) return true;
+  const unsigned long long key = fp ? fp : 1ull;
+  u64 h = mix64(key ^ 0xD6E8FEB86659FD93ull) & t.mask;
+  u64 probes = 0;
+  bool fresh = true;
+  for (;;) {
+    ++probes;
+    const unsigned long long old = atomicCAS(&t.slots[h], 0ull, key);
+    if (old == 0ull) break;
+    if (old == key) { fresh = false; break; }
+    h = (h + 1) & t.mask;
+    if (probes > t.mask) { atomicOr(flags, 4u /* FLAG_TABLE_FULL */); break; }
+  }
+  u64* st = t.stats + (stripe % NSTRIPE) * 4;
+  atomicAdd((unsigned long long*)&st[0], 1ull);
+  atomicAdd((unsigned long long*)&st[1], (unsigned long long)probes);
+  if (!fresh) atomicAdd((unsigned long long*)&st[2], 1ull);
+  return fresh;
+}
+
 struct ExpandArgs {
   const u64* in;        // input states (stride 1 + 6W words)
   u32 n_in;
@@ -66,6 +98,7 @@ struct ExpandArgs {
   u64* fp;              // [NSTRIPE][2] fingerprint sum / xor
   u64* stat;            // [NSTRIPE][8] states, intersections, leaves, bytes per kernel (3..6)
   u32* flags;
+  DedupTable dedup;
 };
 
 __device__ __forceinline__ void wave_barrier() {

@@ -42,7 +42,7 @@ extern "C" void gmk_destroy(gmk_ctx* c) {
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
   for (DeviceVec* v : {&c->stack, &c->next, &c->retry, &c->rows, &c->maps, &c->parts, &c->res, &c->counters,
-                        &c->tuples, &c->arena, &c->fams, &c->prefix, &c->tcnt})
+                        &c->tuples, &c->arena, &c->fams, &c->prefix, &c->tcnt, &c->dedup})
     if (v->p) (void)hipFree(v->p);
   if (c->d_off) (void)hipFree(c->d_off);
   if (c->d_adj) (void)hipFree(c->d_adj);
@@ -144,6 +144,24 @@ extern "C" int gmk_load_graph_bin(gmk_ctx* c, const char* path, uint32_t* n_out,
   return GMK_OK;
 }
 
+void gmk::RunState::progress(u64 count) {
+  const double now =
+      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+  prog_count.push_back(count);
+  prog_ms.push_back(now - t0_ms);
+}
+
+extern "C" int gmk_fetch_progress(gmk_ctx* c, uint64_t* counts, double* ms, uint32_t cap, uint32_t* n) {
+  if (!c) return GMK_EINVAL;
+  const uint32_t len = (uint32_t)c->prog_count.size();
+  if (n) *n = len;
+  for (uint32_t i = 0; i < len && i < cap; ++i) {
+    if (counts) counts[i] = c->prog_count[i];
+    if (ms) ms[i] = c->prog_ms[i];
+  }
+  return GMK_OK;
+}
+
 extern "C" int gmk_fetch_results(gmk_ctx* c, uint32_t* words, uint64_t cap, uint64_t* nwords) {
   if (!c) return GMK_EINVAL;
   if (nwords) *nwords = c->res_words;
@@ -203,6 +221,7 @@ int gmk_run_bb(gmk_ctx* c, const PartitionSet& ps, const gmk_params* prm, gmk_st
   a.fp = dc->fp;
   a.stat = dc->stat;
   a.flags = &dc->flags;
+  a.dedup = rs.dedup;
   a.out_n = &dc->out_n;
   a.retry_n = &dc->retry_n;
   a.retry = (u64*)c->retry.p;
@@ -230,6 +249,7 @@ int gmk_run_bb(gmk_ctx* c, const PartitionSet& ps, const gmk_params* prm, gmk_st
     rs.ms_kernel[0] += ms;
     ++rs.launches_kernel[0];
     const RunCounters& h = *c->h_cnt;
+    rs.progress(std::min<u64>(h.res_count, rs.limit));
     if (h.flags & FLAG_RES_OVERFLOW) {
       c->err = "result buffer overflow (raise result_capacity_words)";
       return GMK_EOVERFLOW;
@@ -261,6 +281,7 @@ extern "C" int gmk_enumerate(gmk_ctx* c, const gmk_params* prm, gmk_stats* st) {
   memset(st, 0, sizeof(*st));
   const auto t0 = std::chrono::steady_clock::now();
   RunState rs{};
+  rs.t0_ms = std::chrono::duration<double, std::milli>(t0.time_since_epoch()).count();
   rs.keep = prm->keep_results != 0;
   rs.limit = prm->limit ? prm->limit : ~0ull;
   int r;
@@ -276,6 +297,18 @@ extern "C" int gmk_enumerate(gmk_ctx* c, const gmk_params* prm, gmk_stats* st) {
     rs.res_cap = 0;
   }
   c->res_words = 0;
+  if (prm->dedup) {
+    const u64 expect = prm->limit ? prm->limit : (1ull << 25);
+    const double lf = (prm->dedup_load_permille ? prm->dedup_load_permille : 500) / 1000.0;
+    u64 cap = 1024;
+    while ((double)cap * lf < (double)expect) cap <<= 1;
+    if ((r = ensure(c, c->dedup, cap * 8))) return r;
+    HIPCHK(c, hipMemsetAsync(c->dedup.p, 0, cap * 8, c->stream));
+    rs.dedup.slots = (unsigned long long*)c->dedup.p;
+    rs.dedup.mask = cap - 1;
+    rs.dedup.stats = ((RunCounters*)c->counters.p)->dstat;
+    rs.dedup.on = 1;
+  }
 
   const u32 nr = c->n - c->n_left;
   const u32 wl = (c->n_left + 63) / 64, wr = (nr + 63) / 64;
@@ -309,6 +342,12 @@ extern "C" int gmk_enumerate(gmk_ctx* c, const gmk_params* prm, gmk_stats* st) {
     st->ms_kernel[q] = rs.ms_kernel[q];
     st->launches_kernel[q] = rs.launches_kernel[q];
   }
+  for (int i = 0; i < NSTRIPE; ++i) {
+    st->dedup_inserts += h.dstat[4 * i];
+    st->dedup_probes += h.dstat[4 * i + 1];
+    st->dedup_dups += h.dstat[4 * i + 2];
+  }
+  if (h.flags & FLAG_TABLE_FULL) { c->err = "dedup table full (raise dedup_load_permille headroom)"; return GMK_EOVERFLOW; }
   st->fp_sum += rs.fp_sum_extra;
   st->fp_xor ^= rs.fp_xor_extra;
   c->res_words = rs.keep ? std::min<u64>(h.res_words, rs.res_cap) : 0;
@@ -316,5 +355,8 @@ extern "C" int gmk_enumerate(gmk_ctx* c, const gmk_params* prm, gmk_stats* st) {
   st->ms_expand = rs.ms_expand;
   st->hit_limit = rs.hit_limit || h.res_count >= rs.limit;
   st->ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  rs.progress(st->count);
+  c->prog_count = std::move(rs.prog_count);
+  c->prog_ms = std::move(rs.prog_ms);
   return GMK_OK;
 }

@@ -18,6 +18,7 @@ struct PartitionSet {
 };
 
 struct RunState {
+  DedupTable dedup{};
   bool keep = false;
   u64 limit = ~0ull;
   u64 res_cap = 0;
@@ -27,6 +28,11 @@ struct RunState {
   u64 fp_sum_extra = 0, fp_xor_extra = 0;
   double ms_kernel[4] = {0, 0, 0, 0};
   u64 launches_kernel[4] = {0, 0, 0, 0};
+  // progress log: results so far and elapsed ms after each launch
+  std::vector<u64> prog_count;
+  std::vector<double> prog_ms;
+  double t0_ms = 0;
+  void progress(u64 count);
 };
 
 // Whole graph as one partition (local IDs = global IDs, right side shifted).
@@ -56,6 +62,7 @@ struct RunCounters {
   u64 res_words, res_count;
   u64 fp[NSTRIPE * 2];
   u64 stat[NSTRIPE * 8];  // 0 states, 1 intersections, 2 emitted leaves, 3-6 bytes per kernel
+  u64 dstat[NSTRIPE * 4]; // dedup: inserts, probes, duplicates
 };
 
 struct DeviceVec {
@@ -75,11 +82,14 @@ struct gmk_ctx {
   // per-run buffers, grown on demand and reused
   DeviceVec stack, next, retry, rows, maps, parts, res, counters;
   DeviceVec tuples, arena, fams, prefix, tcnt;  // thin-result kernels
+  DeviceVec dedup;                              // result fingerprint table
   std::vector<u32> order;                        // degeneracy order (seeded runs)
   bool order_valid = false;
   RunCounters* h_cnt = nullptr;  // pinned
   u64 res_words = 0;             // record stream length of the last run
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::vector<u64> prog_count;   // progress log of the last run
+  std::vector<double> prog_ms;
 };
 
 inline int ensure(gmk_ctx* c, DeviceVec& v, size_t bytes) {

@@ -33,6 +33,7 @@ struct ThinArgs {
   u64 limit;
   u64* fp;
   u64* stat;
+  DedupTable dedup;
 };
 
 hipError_t launch_thin_build(const ThinArgs& a, hipStream_t s);

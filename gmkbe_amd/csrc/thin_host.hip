@@ -84,6 +84,7 @@ int run_chunk(ThinRun& T, const u32* tuples, u32 count, u32 j) {
     T.rs->ms_kernel[2] += ms;
     T.rs->launches_kernel[2]++;
     T.rs->launches++;
+    T.rs->progress(std::min<u64>(c->h_cnt->res_count, T.rs->limit));
     if (c->h_cnt->flags & FLAG_RES_OVERFLOW) { c->err = "result buffer overflow (raise result_capacity_words)"; return GMK_EOVERFLOW; }
     if (c->h_cnt->flags & FLAG_LIMIT) { T.stop = true; T.rs->hit_limit = true; return GMK_OK; }
   }
@@ -134,6 +135,7 @@ int gmk_run_thin(gmk_ctx* c, const gmk_params* prm, gmk_stats* st, RunState& rs,
   a.limit = rs.limit;
   a.fp = dc->fp;
   a.stat = dc->stat;
+  a.dedup = rs.dedup;
 
   const u32 a_lo = a.g.a_lo, a_hi = a.g.a_hi;
   const u32* off = c->off.data();

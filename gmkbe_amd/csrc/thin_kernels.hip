@@ -205,6 +205,17 @@ __global__ __launch_bounds__(256) void thin_check(ThinArgs a) {
   }
   // per-lane emission (records are independent)
   u64 h = 0;
+  u64 fpv = 0;
+  if (emit) {
+    const u32 nr0 = f->nfree + nch;
+    const u32* fr0 = f->freep;
+    for (u32 i = 0; i < f->j; ++i) h += mix64(f->L[i]);
+    for (u32 i = 0; i < f->nfree; ++i) h += mix64(fr0[i]);
+    for (u32 i = 0; i < nch; ++i) h += mix64(ch[i]);
+    cost.bytes += 4ull * f->nfree;
+    fpv = fmix64(h + (u64)(a.swap_sides ? nr0 : f->j) * FP_A + (u64)(a.swap_sides ? f->j : nr0) * FP_B);
+    if (!dedup_insert(a.dedup, fpv, a.flags, blockIdx.x)) emit = false;
+  }
   if (emit) {
     const u64 slot = atomicAdd((unsigned long long*)a.res_count, 1ull);
     if (slot >= a.limit) {
@@ -217,15 +228,10 @@ __global__ __launch_bounds__(256) void thin_check(ThinArgs a) {
         for (u32 q = i; q > 0 && ch[q - 1] > ch[q]; --q) { const u32 tmp = ch[q]; ch[q] = ch[q - 1]; ch[q - 1] = tmp; }
       const u32 nr = f->nfree + nch;
       const u32* fr = f->freep;
-      for (u32 i = 0; i < f->j; ++i) h += mix64(f->L[i]);
-      for (u32 i = 0; i < f->nfree; ++i) h += mix64(fr[i]);
-      for (u32 i = 0; i < nch; ++i) h += mix64(ch[i]);
       const u32 nl = f->j;
-      const u64 fpv = fmix64(h + (u64)(a.swap_sides ? nr : nl) * FP_A + (u64)(a.swap_sides ? nl : nr) * FP_B);
       u64* fs = a.fp + ((blockIdx.x * 4 + (threadIdx.x >> 6)) % NSTRIPE) * 2;
       atomicAdd((unsigned long long*)&fs[0], (unsigned long long)fpv);
       atomicXor((unsigned long long*)&fs[1], (unsigned long long)fpv);
-      cost.bytes += 4ull * f->nfree;  // free list read for the fingerprint / record
       if (a.keep) {
         const u64 len = 2ull + nl + nr;
         cost.bytes += 4ull * len;

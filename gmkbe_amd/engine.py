@@ -28,7 +28,8 @@ STATUS = {0: "ok", -1: "invalid argument", -2: "io error", -3: "device error", -
 class Params(ctypes.Structure):
     _fields_ = [("k", ctypes.c_uint32), ("theta", ctypes.c_uint32), ("limit", ctypes.c_uint64),
                 ("mode", ctypes.c_int32), ("partition", ctypes.c_int32), ("keep_results", ctypes.c_int32),
-                ("verbose", ctypes.c_int32), ("result_capacity_words", ctypes.c_uint64)]
+                ("verbose", ctypes.c_int32), ("result_capacity_words", ctypes.c_uint64),
+                ("dedup", ctypes.c_int32), ("dedup_load_permille", ctypes.c_uint32)]
 
 
 class Stats(ctypes.Structure):
@@ -83,6 +84,9 @@ def load_library():
     L.gmk_enumerate.restype = ctypes.c_int
     L.gmk_fetch_results.argtypes = [ctypes.c_void_p, u32p, ctypes.c_uint64, P(ctypes.c_uint64)]
     L.gmk_fetch_results.restype = ctypes.c_int
+    L.gmk_fetch_progress.argtypes = [ctypes.c_void_p, P(ctypes.c_uint64), P(ctypes.c_double), ctypes.c_uint32,
+                                     P(ctypes.c_uint32)]
+    L.gmk_fetch_progress.restype = ctypes.c_int
     _lib = L
     return L
 
@@ -145,8 +149,10 @@ class Engine:
         return n.value, nl.value, nnz.value
 
     def enumerate(self, k: int = 1, theta: int = 1, limit: int = 0, mode: str = "plain",
-                  partition: str = "auto", keep: bool = True, capacity_words: int = 0) -> Result:
-        prm = Params(k, theta, limit, MODES[mode], PARTS[partition], int(keep), 0, capacity_words)
+                  partition: str = "auto", keep: bool = True, capacity_words: int = 0, dedup: bool = False,
+                  dedup_load: float = 0.5) -> Result:
+        prm = Params(k, theta, limit, MODES[mode], PARTS[partition], int(keep), 0, capacity_words, int(dedup),
+                     int(round(dedup_load * 1000)))
         st = Stats()
         self._check(self._L.gmk_enumerate(self._h, ctypes.byref(prm), ctypes.byref(st)), "gmk_enumerate")
         words = None
@@ -158,3 +164,12 @@ class Engine:
                 self._check(self._L.gmk_fetch_results(self._h, words.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
                                                       nw.value, ctypes.byref(nw)), "gmk_fetch_results")
         return Result(st.as_dict(), words)
+
+    def progress(self) -> list[tuple[int, float]]:
+        """(results so far, elapsed ms) after each launch of the last run."""
+        n = ctypes.c_uint32()
+        self._check(self._L.gmk_fetch_progress(self._h, None, None, 0, ctypes.byref(n)), "gmk_fetch_progress")
+        cnt = (ctypes.c_uint64 * max(1, n.value))()
+        ms = (ctypes.c_double * max(1, n.value))()
+        self._check(self._L.gmk_fetch_progress(self._h, cnt, ms, n.value, ctypes.byref(n)), "gmk_fetch_progress")
+        return [(int(cnt[i]), float(ms[i])) for i in range(n.value)]

@@ -57,6 +57,10 @@ typedef struct {
   int32_t keep_results;  /* 1: keep the record stream for gmk_fetch_results */
   int32_t verbose;       /* -m */
   uint64_t result_capacity_words; /* 0 = automatic */
+  int32_t dedup;         /* 1: insert every result's 64-bit fingerprint into a device
+                            hash table and drop repeats (a safety net: the
+                            decomposition emits no repeats) */
+  uint32_t dedup_load_permille;   /* table load factor at the expected result count, 0 = 500 */
 } gmk_params;
 
 typedef struct {
@@ -102,6 +106,14 @@ int gmk_enumerate(gmk_ctx* ctx, const gmk_params* params, gmk_stats* stats);
  * records in emission order.  *nwords receives the stream length; when
  * cap_words is smaller nothing is copied and GMK_EOVERFLOW is returned. */
 int gmk_fetch_results(gmk_ctx* ctx, uint32_t* words, uint64_t cap_words, uint64_t* nwords);
+
+/* Progress log of the last gmk_enumerate run: after each device launch, the
+ * number of results emitted so far and the elapsed time in ms since the run
+ * started.  Replaces the checkpoint records (count, ms) the reference takes at
+ * result milestones (src/main.cu:1753-1761, include/for_generate.cuh:451-459):
+ * a milestone n is reached at the first entry with count >= n.  *n receives
+ * the log length; at most cap entries are copied. */
+int gmk_fetch_progress(gmk_ctx* ctx, uint64_t* counts, double* ms, uint32_t cap, uint32_t* n);
 
 #ifdef __cplusplus
 }

@@ -29,3 +29,10 @@ def test_engine_fails_loudly_without_gpu_or_library(monkeypatch):
     monkeypatch.setattr(engine, "_lib", None)
     with pytest.raises(engine.GmkError):
         engine.Engine(0)
+
+
+def test_params_and_stats_layout_match_header():
+    # the ctypes mirrors must have the C structs' sizes (x86-64 SysV layout)
+    import ctypes
+    assert ctypes.sizeof(engine.Params) == 48
+    assert ctypes.sizeof(engine.Stats) == 8 * 13 + 8 + 32 * 3

@@ -1,0 +1,58 @@
+"""The gmkbpe command-line driver: result lines, milestone records and the JSON
+time record, in the reference's formats (src/main.cu:1709-1715,
+include/output.cuh:93-176, include/timer.cuh:98-194)."""
+import json
+import os
+import re
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+import golden_cases as GC
+import oracle_lib as O
+from gmkbe_amd import engine
+
+pytestmark = pytest.mark.gpu
+CLI = os.path.join(os.path.dirname(engine.LIB_PATH), "gmkbpe")
+
+
+def _records_from_lines(out: str):
+    words = []
+    for ln in out.splitlines():
+        m = re.match(r"Result (\d+): X: ([0-9 ]*) \| Y: ([0-9 ]*)$", ln)
+        if m:
+            L = [int(x) for x in m.group(2).split()]
+            R = [int(x) for x in m.group(3).split()]
+            words += [len(L), *L, len(R), *R]
+    return np.asarray(words, dtype=np.uint32)
+
+
+def test_cli_full_enumeration_matches_reference(tmp_path):
+    c = [c for c in GC.cases(min_k=1) if c["graph"] == "er40x50" and c["k"] == 1 and c["theta"] == 1][0]
+    src = os.path.join(GC.HERE, GC._G["graphs"]["er40x50"])
+    g = str(tmp_path / "er40x50.graph.bin")
+    shutil.copy(src, g)
+    p = subprocess.run([CLI, "-f", g, "-k", "1", "-r", "0", "-q", "1"], capture_output=True, text=True, timeout=120)
+    assert p.returncode == 0, p.stderr
+    words = _records_from_lines(p.stdout)
+    assert GC.digest_matches(c, O.digest(words))
+    assert "Running Time:" in p.stdout
+
+
+def test_cli_limit_milestones_and_time_record(tmp_path):
+    src = os.path.join(GC.HERE, GC._G["graphs"]["er200x300"])
+    g = str(tmp_path / "er200x300.graph.bin")
+    shutil.copy(src, g)
+    p = subprocess.run([CLI, "-f", g, "-k", "1", "-r", "100000"], capture_output=True, text=True, timeout=120)
+    assert p.returncode == 0, p.stderr
+    marks = [int(m) for m in re.findall(r"Output (\d+), cost Time: [0-9.e+]+ ms", p.stdout)]
+    assert marks == [1, 10, 100, 1000, 10000, 100000]
+    rec = json.load(open(tmp_path / "er200x300.gpu1.json"))
+    assert set(rec) == {"1"} and set(rec["1"]) == {"100000"} and rec["1"]["100000"] < 5 * 3600 * 1000
+    # a run that cannot reach its limit keeps the 5-hour sentinel
+    p = subprocess.run([CLI, "-f", g, "-k", "1", "-t", "9", "-r", "100"], capture_output=True, text=True, timeout=120)
+    assert p.returncode == 0, p.stderr
+    rec = json.load(open(tmp_path / "er200x300.gpu1.json"))
+    assert rec["1"]["100"] == 5 * 3600 * 1000 and "100000" in rec["1"]

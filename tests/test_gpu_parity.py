@@ -89,3 +89,14 @@ def test_seeded_partition_matches_brute(eng, seed, k):
         wb, _ = O.brute(g, k, theta)
         res = eng.enumerate(k=k, theta=theta, partition="seeded", mode="batched")
         _check_set(g, k, theta, res, expect_words=wb)
+
+
+@pytest.mark.parametrize("partition", ["global", "seeded"])
+def test_dedup_table_finds_no_repeats(eng, partition):
+    c = [c for c in GC.cases(min_k=1) if c["graph"] == "er40x50" and c["k"] == 1 and c["theta"] == 1][0]
+    g = GC.graph(c["graph"])
+    eng.set_graph(g)
+    res = eng.enumerate(k=1, theta=1, partition=partition, mode="batched", dedup=True, dedup_load=0.7)
+    _check_set(g, 1, 1, res, expect_case=c)
+    assert res.stats["dedup_inserts"] == c["count"] and res.stats["dedup_dups"] == 0
+    assert res.stats["dedup_probes"] >= res.stats["dedup_inserts"]
