@@ -49,10 +49,10 @@ def test_cli_limit_milestones_and_time_record(tmp_path):
     assert p.returncode == 0, p.stderr
     marks = [int(m) for m in re.findall(r"Output (\d+), cost Time: [0-9.e+]+ ms", p.stdout)]
     assert marks == [1, 10, 100, 1000, 10000, 100000]
-    rec = json.load(open(tmp_path / "er200x300.gpu1.json"))
+    rec = json.load(open(tmp_path / "er200x300.graph.gpu1.json"))
     assert set(rec) == {"1"} and set(rec["1"]) == {"100000"} and rec["1"]["100000"] < 5 * 3600 * 1000
     # a run that cannot reach its limit keeps the 5-hour sentinel
     p = subprocess.run([CLI, "-f", g, "-k", "1", "-t", "9", "-r", "100"], capture_output=True, text=True, timeout=120)
     assert p.returncode == 0, p.stderr
-    rec = json.load(open(tmp_path / "er200x300.gpu1.json"))
+    rec = json.load(open(tmp_path / "er200x300.graph.gpu1.json"))
     assert rec["1"]["100"] == 5 * 3600 * 1000 and "100000" in rec["1"]
