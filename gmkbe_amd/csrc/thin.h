@@ -41,6 +41,7 @@
 namespace gmk {
 namespace thin {
 
+typedef uint8_t u8;
 typedef uint32_t u32;
 typedef uint64_t u64;
 
@@ -170,16 +171,20 @@ GMK_HD bool in_tuple(const u32* L, u32 j, u32 w) {
 }
 
 // Does w miss at most k vertices of R, and is w adjacent to every saturated one?
+// `known` is a member of R already known to be adjacent to w (the vertex whose
+// list produced w as a candidate), or NONE.  Members are searched in N(w).
 GMK_HD bool addable(const Graph& g, u32 w, const u32* freel, u32 nfree, const u32* ch, const u32* ctsz, u32 nch,
-                    u32 k, Cost& cost) {
-  const u32* nw = g.adj + g.off[w];
-  const u32 dw = g.off[w + 1] - g.off[w];
+                    u32 k, u32 known, Cost& cost) {
+  const u32 ow = g.off[w], dw = g.off[w + 1] - ow;
+  const u32* nw = g.adj + ow;
   cost.bytes += 8;
   const u32 rs = nfree + nch;
   if (dw + k < rs) return false;
   u32 miss = 0;
   for (u32 i = 0; i < nch; ++i) {
-    if (!contains(nw, dw, ch[i], cost)) {
+    const u32 y = ch[i];
+    if (y == known) continue;
+    if (!contains(nw, dw, y, cost)) {
       if (ctsz[i] >= k) return false;  // saturated member missed
       if (++miss > k) return false;
     }
@@ -188,6 +193,7 @@ GMK_HD bool addable(const Graph& g, u32 w, const u32* freel, u32 nfree, const u3
   if (nfree <= dw) {
     for (u32 i = 0; i < nfree; ++i)
       if (!contains(nw, dw, freel[i], cost) && ++miss > k) return false;
+    cost.bytes += 4ull * nfree;
   } else {
     // count free vertices adjacent to w by scanning N(w)
     u32 hit = 0;
@@ -216,7 +222,7 @@ GMK_HD bool side_a_maximal(const Graph& g, const u32* L, u32 j, const u32* freel
       const u32 w = g.adj[e];
       cost.bytes += 4;
       if (in_tuple(L, j, w)) continue;
-      if (addable(g, w, freel, nfree, ch, ctsz, nch, k, cost)) return false;
+      if (addable(g, w, freel, nfree, ch, ctsz, nch, k, best, cost)) return false;
     }
     return true;
   }
@@ -246,7 +252,7 @@ GMK_HD bool side_a_maximal(const Graph& g, const u32* L, u32 j, const u32* freel
       const u32 w = g.adj[e];
       cost.bytes += 4;
       if (in_tuple(L, j, w)) continue;
-      if (addable(g, w, freel, nfree, ch, ctsz, nch, k, cost)) return false;
+      if (addable(g, w, freel, nfree, ch, ctsz, nch, k, 0xFFFFFFFFu, cost)) return false;
     }
   return true;
 }

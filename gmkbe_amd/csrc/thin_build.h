@@ -14,12 +14,13 @@ namespace thin {
 //   far      number of side-B vertices adjacent to no tuple member
 //   poolp[m] address of the list of type-m vertices (m = 0 is the free list)
 //   nlp, d   N(L) ascending (needed only when the far pool is feasible)
-template <class Fams>
+// JM bounds the tuple size (j <= JM) so the per-lane arrays have 2^JM entries.
+template <int JM, class Fams>
 GMK_HD void emit_families(const u32* L, u32 j, u32 k, u32 min_b, u32 tuple_idx, const u32* mcnt, u64 far,
                           const u64* poolp, const u32* nlp, u32 d, Fams& fo, Cost& cost) {
   const u32 full = (1u << j) - 1u;
   const u32 nfree = mcnt[0];
-  u32 tmask[(1 << MAXJ)];
+  u8 tmask[(1 << JM)];
   u32 nt = 0;
   for (u32 m = 1; m < full; ++m)
     if (mcnt[m] && (u32)popc(m) <= k) tmask[nt++] = m;
@@ -47,8 +48,8 @@ GMK_HD void emit_families(const u32* L, u32 j, u32 k, u32 min_b, u32 tuple_idx, 
   // Depth-first over the types, c[t] bounded by the members' remaining budgets;
   // a leaf is kept when every type with spare vertices has a member with no
   // budget left.
-  u32 c[(1 << MAXJ)];
-  u32 budget[MAXJ];
+  u8 c[(1 << JM)];
+  u32 budget[JM];
   for (u32 i = 0; i < j; ++i) budget[i] = k;
   auto cap_of = [&](u32 t) -> u32 {
     u32 cp = k;
@@ -118,12 +119,12 @@ GMK_HD void emit_families(const u32* L, u32 j, u32 k, u32 min_b, u32 tuple_idx, 
 // Sequential census by a j-way merge of the members' lists.  Returns false
 // when the arena could not hold the tuple's lists (the caller retries with a
 // smaller batch); a tuple that fails emits no family.
-template <class Arena, class Fams>
-GMK_HD bool build_families(const Graph& g, const u32* L, u32 j, u32 k, u32 min_b, u32 tuple_idx, Arena& ar,
-                           Fams& fo, Cost& cost) {
+template <int JM, class Arena, class Fams>
+GMK_HD bool build_families_t(const Graph& g, const u32* L, u32 j, u32 k, u32 min_b, u32 tuple_idx, Arena& ar,
+                             Fams& fo, Cost& cost) {
   const u32 full = (1u << j) - 1u;
-  u32 mcnt[1 << MAXJ];
-  u64 poolp[1 << MAXJ];
+  u32 mcnt[1 << JM];
+  u64 poolp[1 << JM];
   if (j == 1) {
     // one vertex: free list = N(s), the far pool is everything else
     const u32 s = L[0];
@@ -132,10 +133,10 @@ GMK_HD bool build_families(const Graph& g, const u32* L, u32 j, u32 k, u32 min_b
     cost.bytes += 8;
     mcnt[0] = d;
     poolp[0] = (u64)ns;
-    emit_families(L, 1, k, min_b, tuple_idx, mcnt, (u64)(g.b_hi - g.b_lo) - d, poolp, ns, d, fo, cost);
+    emit_families<JM>(L, 1, k, min_b, tuple_idx, mcnt, (u64)(g.b_hi - g.b_lo) - d, poolp, ns, d, fo, cost);
     return true;
   }
-  u32 pos[MAXJ], end[MAXJ];
+  u32 pos[JM], end[JM];
   for (u32 i = 0; i < j; ++i) { pos[i] = g.off[L[i]]; end[i] = g.off[L[i] + 1]; cost.bytes += 12 + 4ull * (end[i] - pos[i]); }
   for (u32 m = 0; m <= full; ++m) mcnt[m] = 0;
   u32 d = 0;
@@ -155,7 +156,7 @@ GMK_HD bool build_families(const Graph& g, const u32* L, u32 j, u32 k, u32 min_b
   if (nl_off == SAT) return false;
   const u64 pool_off = ar.alloc(d);
   if (pool_off == SAT) return false;
-  u32 moff[1 << MAXJ];
+  u32 moff[1 << JM];
   {
     u32 acc = 0;
     for (u32 m = 0; m <= full; ++m) { moff[m] = acc; acc += mcnt[m]; }
@@ -167,7 +168,7 @@ GMK_HD bool build_families(const Graph& g, const u32* L, u32 j, u32 k, u32 min_b
   cost.bytes += (far_ok ? 8ull : 4ull) * d;
   for (u32 i = 0; i < j; ++i) pos[i] = g.off[L[i]];
   {
-    u32 fillc[1 << MAXJ];
+    u32 fillc[1 << JM];
     for (u32 m = 0; m <= full; ++m) fillc[m] = 0;
     u32 w = 0;
     for (;;) {
@@ -184,8 +185,22 @@ GMK_HD bool build_families(const Graph& g, const u32* L, u32 j, u32 k, u32 min_b
       pools[moff[m] + fillc[m]++] = y;
     }
   }
-  emit_families(L, j, k, min_b, tuple_idx, mcnt, (u64)(g.b_hi - g.b_lo) - d, poolp, nl, d, fo, cost);
+  emit_families<JM>(L, j, k, min_b, tuple_idx, mcnt, (u64)(g.b_hi - g.b_lo) - d, poolp, nl, d, fo, cost);
   return true;
+}
+
+// Dispatch on the tuple size so the arrays are sized 2^j.
+template <class Arena, class Fams>
+GMK_HD bool build_families(const Graph& g, const u32* L, u32 j, u32 k, u32 min_b, u32 tuple_idx, Arena& ar,
+                           Fams& fo, Cost& cost) {
+  switch (j) {
+    case 1: return build_families_t<1>(g, L, j, k, min_b, tuple_idx, ar, fo, cost);
+    case 2: return build_families_t<2>(g, L, j, k, min_b, tuple_idx, ar, fo, cost);
+    case 3: return build_families_t<3>(g, L, j, k, min_b, tuple_idx, ar, fo, cost);
+    case 4: return build_families_t<4>(g, L, j, k, min_b, tuple_idx, ar, fo, cost);
+    case 5: return build_families_t<5>(g, L, j, k, min_b, tuple_idx, ar, fo, cost);
+    default: return build_families_t<MAXJ>(g, L, j, k, min_b, tuple_idx, ar, fo, cost);
+  }
 }
 
 }  // namespace thin

@@ -54,6 +54,7 @@ __global__ __launch_bounds__(256) void thin_build_families(ThinArgs a) {
 // the other lists.  Pass 1 counts the types, pass 2 writes the pools (the free
 // list in ascending order, other pools in any order); lane 0 then emits the
 // families.
+template <int J>
 __global__ __launch_bounds__(256) void thin_build_wave(ThinArgs a) {
   __shared__ u32 s_off[4][1 << thin::MAXJ];  // pass 1: counts; pass 2: pool offsets
   __shared__ u32 s_cur[4][1 << thin::MAXJ];  // pass 2: fill cursors
@@ -61,10 +62,10 @@ __global__ __launch_bounds__(256) void thin_build_wave(ThinArgs a) {
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const u32 t = blockIdx.x * 4 + wv;
   if (t >= a.ntuples) return;  // uniform per wave
-  const u32 j = a.j, full = (1u << j) - 1u;
+  constexpr u32 j = J, full = (1u << J) - 1u;
   const u32* L = a.tuples + (u64)t * j;
   thin::Cost cost;
-  u32 lo[thin::MAXJ], dg[thin::MAXJ];
+  u32 lo[J], dg[J];
   for (u32 i = 0; i < j; ++i) { lo[i] = a.g.off[L[i]]; dg[i] = a.g.off[L[i] + 1] - lo[i]; }
   s_off[wv][lane] = 0;
   s_cur[wv][lane] = 0;
@@ -133,38 +134,48 @@ __global__ __launch_bounds__(256) void thin_build_wave(ThinArgs a) {
     if (s_base[wv] == thin::SAT) break;
   }
   if (s_base[wv] != thin::SAT) {
-    u32 mcnt[1 << thin::MAXJ];
+    u32 mcnt[1 << J];
     u32 d = 0;
     for (u32 mm = 0; mm <= full; ++mm) { mcnt[mm] = (u32)__shfl((int)cnt_lane, (int)mm, 64); d += mcnt[mm]; }
     if (lane == 0) {
-      u64 poolp[1 << thin::MAXJ];
+      u64 poolp[1 << J];
       for (u32 mm = 0; mm <= full; ++mm) poolp[mm] = (u64)(a.arena + s_base[wv] + s_off[wv][mm]);
       DevFamilies fo{a.fams, a.fam_cap, a.fam_n, a.tflags};
-      thin::emit_families(L, j, a.k, a.min_b, t, mcnt, (u64)(a.g.b_hi - a.g.b_lo) - d, poolp, nullptr, d, fo, cost);
+      thin::emit_families<J>(L, j, a.k, a.min_b, t, mcnt, (u64)(a.g.b_hi - a.g.b_lo) - d, poolp, nullptr, d, fo,
+                             cost);
     }
   }
   const u64 b = wave_sum_u64(cost.bytes);
   if (lane == 0 && b) atomicAdd((unsigned long long*)&a.stat[(t % NSTRIPE) * NSTAT + 4], (unsigned long long)b);
 }
 
-// Exclusive prefix sums of the family totals (saturating), one block.
+// Exclusive prefix sums of the family totals (saturating), one block of 1024
+// threads: per-thread segment sums, a wave-level scan of the segment sums, then
+// a scan over the 16 wave totals.
 __global__ __launch_bounds__(1024) void thin_scan_totals(const thin::Family* fams, u32 nfam, u64* prefix) {
-  __shared__ u64 part[1024];
-  const u32 t = threadIdx.x;
+  __shared__ u64 wsum[16];
+  const u32 t = threadIdx.x, lane = t & 63, wv = t >> 6;
   const u32 seg = (nfam + 1023) / 1024;
   const u32 b = t * seg < nfam ? t * seg : nfam;
   const u32 e = b + seg < nfam ? b + seg : nfam;
   auto add = [](u64 x, u64 y) -> u64 { const u64 z = x + y; return (z < x || z > thin::SAT) ? thin::SAT : z; };
   u64 s = 0;
   for (u32 i = b; i < e; ++i) s = add(s, fams[i].total);
-  part[t] = s;
+  // inclusive scan within the wave
+  u64 inc = s;
+  for (int d = 1; d < 64; d <<= 1) {
+    const u64 o = wave_bcast_u64(inc, (int)lane >= d ? (int)lane - d : (int)lane);
+    if ((int)lane >= d) inc = add(inc, o);
+  }
+  if (lane == 63) wsum[wv] = inc;
   __syncthreads();
   if (t == 0) {
     u64 run = 0;
-    for (u32 i = 0; i < 1024; ++i) { const u64 v = part[i]; part[i] = run; run = add(run, v); }
+    for (int w = 0; w < 16; ++w) { const u64 v = wsum[w]; wsum[w] = run; run = add(run, v); }
   }
   __syncthreads();
-  u64 run = part[t];
+  u64 run = add(wsum[wv], inc - s > inc ? 0 : inc - s);  // exclusive within the wave
+  if (inc >= thin::SAT) run = thin::SAT;
   for (u32 i = b; i < e; ++i) { prefix[i] = run; run = add(run, fams[i].total); }
   if (b < e && e == nfam) prefix[nfam] = run;
   if (nfam == 0 && t == 0) prefix[0] = 0;
@@ -270,9 +281,16 @@ __global__ __launch_bounds__(256) void thin_check(ThinArgs a) {
 
 hipError_t launch_thin_build(const ThinArgs& a, hipStream_t s) {
   if (a.ntuples == 0) return hipSuccess;
-  if (a.j >= 2 && a.j > a.k)
-    hipLaunchKernelGGL(thin_build_wave, dim3((a.ntuples + 3) / 4), dim3(256), 0, s, a);
-  else
+  if (a.j >= 2 && a.j > a.k) {
+    const dim3 g((a.ntuples + 3) / 4), b(256);
+    switch (a.j) {
+      case 2: hipLaunchKernelGGL(thin_build_wave<2>, g, b, 0, s, a); break;
+      case 3: hipLaunchKernelGGL(thin_build_wave<3>, g, b, 0, s, a); break;
+      case 4: hipLaunchKernelGGL(thin_build_wave<4>, g, b, 0, s, a); break;
+      case 5: hipLaunchKernelGGL(thin_build_wave<5>, g, b, 0, s, a); break;
+      default: hipLaunchKernelGGL(thin_build_wave<6>, g, b, 0, s, a); break;
+    }
+  } else
     hipLaunchKernelGGL(thin_build_families, dim3((a.ntuples + 255) / 256), dim3(256), 0, s, a);
   return hipGetLastError();
 }
