@@ -39,6 +39,7 @@ struct __align__(16) Smem {
   u64 cand[W];            // pivot candidates (on side cand_side)
   u64 sat[W];             // S members saturated by the include being built
   u64 andsat[W];          // AND of the rows of sat
+  u64 prefix[W];           // pivot candidates already branched on
   uint16_t deg[2][64 * W];
   uint8_t ms[2][64 * W];
 };
@@ -137,10 +138,21 @@ __device__ void emit_state(Smem<W>& sm, const PartDesc& P, const u32* maps, cons
   h = wave_sum_u64(h);
   const u64 fp = fmix64(h + (u64)nl * FP_A + (u64)nr * FP_B);
   u64 slot = 0;
-  if (lane == 0) slot = dedup_insert(a.dedup, fp, a.flags, blockIdx.x) ? atomicAdd((unsigned long long*)a.res_count, 1ull)
-                                                                    : ~0ull;
+  if (lane == 0) {
+    bool take = dedup_insert(a.dedup, fp, a.flags, blockIdx.x);
+    if (take && a.gcount) {
+      // per-graph cap and digest
+      const unsigned long long gs = atomicAdd(&a.gcount[P.graph], 1ull);
+      if (gs >= a.gcap) take = false;
+      else {
+        atomicAdd((unsigned long long*)&a.gfp[2 * P.graph], (unsigned long long)fp);
+        atomicXor((unsigned long long*)&a.gfp[2 * P.graph + 1], (unsigned long long)fp);
+      }
+    }
+    slot = take ? atomicAdd((unsigned long long*)a.res_count, 1ull) : ~0ull;
+  }
   slot = wave_bcast_u64(slot, 0);
-  if (slot == ~0ull) return;  // duplicate
+  if (slot == ~0ull) return;  // duplicate, or the graph's cap is reached
   if (slot >= a.limit) {
     if (lane == 0) atomicOr(a.flags, FLAG_LIMIT);
     return;
@@ -248,24 +260,23 @@ __device__ void write_child(Smem<W>& sm, const PartDesc& P, const u64* rows, u32
   }
 }
 
+// Expands state sidx of the batch; counters accumulate into st.
 template <int W>
-__global__ __launch_bounds__(64) void bb_expand(ExpandArgs a) {
-  __shared__ Smem<W> sm;
-  const int lane = threadIdx.x;
-  const u32 sidx = blockIdx.x;
-  if (sidx >= a.n_in) return;
+__device__ void expand_one(Smem<W>& sm, const ExpandArgs& a, u32 sidx, Stats& st) {
+  const int lane = threadIdx.x & 63;
   constexpr int STRIDE = 1 + 6 * W;
   const u64* in = a.in + (u64)sidx * STRIDE;
   const u32 pid = (u32)in[0];
   if (pid == INVALID_PID) return;  // hole left by a child block that did not fit
   const PartDesc P = a.parts[pid];
-  const u32 k = a.k;
+  const u32 k = P.k;
   const u32 lb = P.lb;
-  Stats st = {0, 0, 0, 0};
+  if (a.gcount && __hip_atomic_load(&a.gcount[P.graph], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= a.gcap)
+    return;  // this graph already has all the results it keeps
+  wave_barrier();  // the previous state of this wave is done with the LDS
   for (int j = lane; j < 6 * W; j += 64) sm.set[j] = in[1 + j];
   wave_barrier();
-  if (__hip_atomic_load(a.flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & FLAG_LIMIT) return;
-  st.states = 1;
+  st.states += 1;
   st.bytes += STRIDE * 8;
 
   side_stats<W>(sm, P, a.rows, 0, st);
@@ -480,7 +491,7 @@ finish:
       const int os = pside ^ 1;
       const u32 w_os = os ? P.wr : P.wl;
       // running prefix of candidates already branched on (moved to X)
-      __shared__ u64 prefix[W];
+      u64* prefix = sm.prefix;
       if (lane < W) prefix[lane] = 0;
       wave_barrier();
       u32 c = 0;
@@ -504,24 +515,74 @@ finish:
       st.bytes += (u64)c * STRIDE * 8;
     }
   }
+}
+
+template <int W>
+__device__ void flush_stats(const ExpandArgs& a, const Stats& st, u32 stripe) {
   const u64 tsum = wave_sum_u64(st.intersections);
   const u64 bsum = wave_sum_u64(st.bytes);
-  if (lane == 0) {
-    u64* sp = a.stat + (sidx % NSTRIPE) * NSTAT;
+  if ((threadIdx.x & 63) == 0) {
+    u64* sp = a.stat + (stripe % NSTRIPE) * NSTAT;
     atomicAdd((unsigned long long*)&sp[0], (unsigned long long)st.states);
     atomicAdd((unsigned long long*)&sp[1], (unsigned long long)tsum);
     atomicAdd((unsigned long long*)&sp[3], (unsigned long long)bsum);
   }
 }
 
+// One block (one wave) per state.
+template <int W>
+__global__ __launch_bounds__(64) void bb_expand(ExpandArgs a) {
+  __shared__ Smem<W> sm;
+  if (blockIdx.x >= a.n_in) return;
+  if (__hip_atomic_load(a.flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & FLAG_LIMIT) return;
+  Stats st = {0, 0, 0, 0};
+  expand_one<W>(sm, a, blockIdx.x, st);
+  flush_stats<W>(a, st, blockIdx.x);
+}
+
+// Persistent form: a fixed grid of waves takes states from the batch through
+// a device counter until the batch is drained (or the result limit is hit),
+// so uneven states balance across the chip without one block per state.
+template <int W>
+__global__ __launch_bounds__(64) void bb_expand_persistent(ExpandArgs a) {
+  __shared__ Smem<W> sm;
+  const int lane = threadIdx.x & 63;
+  Stats st = {0, 0, 0, 0};
+  for (;;) {
+    u32 idx = 0;
+    if (lane == 0) idx = atomicAdd(a.work_next, 1u);
+    idx = (u32)wave_bcast_u64(idx, 0);
+    if (idx >= a.n_in) break;
+    if (__hip_atomic_load(a.flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & FLAG_LIMIT) break;
+    expand_one<W>(sm, a, idx, st);
+  }
+  flush_stats<W>(a, st, blockIdx.x);
+}
+
 
 }  // namespace gmk
 
 namespace gmk {
-// Host-side launcher for the expansion kernel (one wave per state).
-hipError_t launch_bb_expand(int W, u32 n_states, hipStream_t s, const ExpandArgs& a) {
+// Host-side launcher for the expansion kernel: one wave per state, or, with
+// persistent_waves > 0, that many persistent waves draining the batch.
+hipError_t launch_bb_expand(int W, u32 n_states, hipStream_t s, const ExpandArgs& a, u32 persistent_waves) {
   if (n_states == 0) return hipSuccess;
-  dim3 grid(n_states), block(64);
+  dim3 block(64);
+  if (persistent_waves) {
+    dim3 grid(n_states < persistent_waves ? n_states : persistent_waves);
+    switch (W) {
+      case 1: hipLaunchKernelGGL(bb_expand_persistent<1>, grid, block, 0, s, a); break;
+      case 2: hipLaunchKernelGGL(bb_expand_persistent<2>, grid, block, 0, s, a); break;
+      case 4: hipLaunchKernelGGL(bb_expand_persistent<4>, grid, block, 0, s, a); break;
+      case 8: hipLaunchKernelGGL(bb_expand_persistent<8>, grid, block, 0, s, a); break;
+      case 16: hipLaunchKernelGGL(bb_expand_persistent<16>, grid, block, 0, s, a); break;
+      case 32: hipLaunchKernelGGL(bb_expand_persistent<32>, grid, block, 0, s, a); break;
+      case 64: hipLaunchKernelGGL(bb_expand_persistent<64>, grid, block, 0, s, a); break;
+      default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+  }
+  dim3 grid(n_states);
   switch (W) {
     case 1: hipLaunchKernelGGL(bb_expand<1>, grid, block, 0, s, a); break;
     case 2: hipLaunchKernelGGL(bb_expand<2>, grid, block, 0, s, a); break;

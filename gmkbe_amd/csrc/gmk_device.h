@@ -43,6 +43,8 @@ struct PartDesc {
   u64 map_r;    // same, right
   u32 lb;       // both result sides must reach lb
   u32 seed;     // seed vertex (global ID) or NONE for a global partition
+  u32 k;        // misses allowed per vertex
+  u32 graph;    // graph of a multi-graph batch (0 otherwise)
 };
 
 // Result dedup table: open addressing over 64-bit fingerprints (0 = empty),
@@ -80,6 +82,7 @@ __device__ __forceinline__ bool dedup_insert(const DedupTable& t, u64 fp, u32* f
 struct ExpandArgs {
   const u64* in;        // input states (stride 1 + 6W words)
   u32 n_in;
+  u32* work_next;       // persistent form: next state of the batch to take
   u64* out;             // children
   u32 out_cap;
   u32* out_n;
@@ -99,6 +102,10 @@ struct ExpandArgs {
   u64* stat;            // [NSTRIPE][8] states, intersections, leaves, bytes per kernel (3..6)
   u32* flags;
   DedupTable dedup;
+  // multi-graph batches: per-graph result caps and digests (null otherwise)
+  unsigned long long* gcount;  // results per graph
+  u64* gfp;                    // [graph][2] fingerprint sum / xor
+  u64 gcap;                    // results kept per graph
 };
 
 __device__ __forceinline__ void wave_barrier() {

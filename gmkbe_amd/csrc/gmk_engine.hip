@@ -16,7 +16,7 @@
 using namespace gmk;
 
 namespace gmk {
-hipError_t launch_bb_expand(int W, u32 n_states, hipStream_t s, const ExpandArgs& a);
+hipError_t launch_bb_expand(int W, u32 n_states, hipStream_t s, const ExpandArgs& a, u32 persistent_waves);
 }
 
 extern "C" int gmk_create(int device, gmk_ctx** out) {
@@ -27,6 +27,11 @@ extern "C" int gmk_create(int device, gmk_ctx** out) {
   if (device < 0 || device >= ndev) return GMK_EINVAL;
   gmk_ctx* c = new gmk_ctx();
   c->device = device;
+  {
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
+      c->persistent_waves = (u32)prop.multiProcessorCount * 16;
+  }
   if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
       hipHostMalloc((void**)&c->h_cnt, sizeof(RunCounters), hipHostMallocDefault) != hipSuccess ||
       hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
@@ -42,7 +47,7 @@ extern "C" void gmk_destroy(gmk_ctx* c) {
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
   for (DeviceVec* v : {&c->stack, &c->next, &c->retry, &c->rows, &c->maps, &c->parts, &c->res, &c->counters,
-                        &c->tuples, &c->arena, &c->fams, &c->prefix, &c->tcnt, &c->dedup})
+                        &c->tuples, &c->arena, &c->fams, &c->prefix, &c->tcnt, &c->dedup, &c->gcnt})
     if (v->p) (void)hipFree(v->p);
   if (c->d_off) (void)hipFree(c->d_off);
   if (c->d_adj) (void)hipFree(c->d_adj);
@@ -222,8 +227,12 @@ int gmk_run_bb(gmk_ctx* c, const PartitionSet& ps, const gmk_params* prm, gmk_st
   a.stat = dc->stat;
   a.flags = &dc->flags;
   a.dedup = rs.dedup;
+  a.gcount = rs.gcount;
+  a.gfp = rs.gfp;
+  a.gcap = rs.gcap;
   a.out_n = &dc->out_n;
   a.retry_n = &dc->retry_n;
+  a.work_next = &dc->pad;
   a.retry = (u64*)c->retry.p;
   a.out = (u64*)c->next.p;
   u64* stack = (u64*)c->stack.p;
@@ -237,8 +246,9 @@ int gmk_run_bb(gmk_ctx* c, const PartitionSet& ps, const gmk_params* prm, gmk_st
     a.in = stack + (top - B) * stride;
     a.n_in = B;
     HIPCHK(c, hipMemsetAsync(dc, 0, 2 * sizeof(u32), s));
+    HIPCHK(c, hipMemsetAsync(&dc->pad, 0, sizeof(u32), s));
     HIPCHK(c, hipEventRecord(c->ev0, s));
-    HIPCHK(c, launch_bb_expand(W, B, s, a));
+    HIPCHK(c, launch_bb_expand(W, B, s, a, prm->mode == GMK_MODE_PLAIN ? 0u : c->persistent_waves));
     HIPCHK(c, hipEventRecord(c->ev1, s));
     HIPCHK(c, hipMemcpyAsync(c->h_cnt, dc, offsetof(RunCounters, fp), hipMemcpyDeviceToHost, s));
     HIPCHK(c, hipStreamSynchronize(s));
@@ -318,7 +328,7 @@ extern "C" int gmk_enumerate(gmk_ctx* c, const gmk_params* prm, gmk_stats* st) {
   if (part == GMK_PART_GLOBAL) {
     if (!Wg) { c->err = "graph too large for a global partition"; return GMK_EUNSUPPORTED; }
     PartitionSet ps;
-    build_global_partition(c->n, c->n_left, c->off.data(), c->adj.data(), prm->theta, Wg, ps);
+    build_global_partition(c->n, c->n_left, c->off.data(), c->adj.data(), prm->k, prm->theta, Wg, ps);
     st->partitions = 1;
     if ((r = gmk_run_bb(c, ps, prm, st, rs))) return r;
   } else {
@@ -356,6 +366,112 @@ extern "C" int gmk_enumerate(gmk_ctx* c, const gmk_params* prm, gmk_stats* st) {
   st->hit_limit = rs.hit_limit || h.res_count >= rs.limit;
   st->ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   rs.progress(st->count);
+  c->prog_count = std::move(rs.prog_count);
+  c->prog_ms = std::move(rs.prog_ms);
+  return GMK_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Many graphs in one run.
+// ---------------------------------------------------------------------------
+extern "C" int gmk_enumerate_batch(gmk_ctx* c, const gmk_graph_desc* graphs, uint32_t n_graphs,
+                                   const gmk_params* prm, const uint32_t* order, gmk_graph_stats* per_graph,
+                                   gmk_stats* st) {
+  if (!c || !graphs || !prm || !per_graph) return GMK_EINVAL;
+  HIPCHK(c, hipSetDevice(c->device));
+  gmk_stats local{};
+  if (!st) st = &local;
+  memset(st, 0, sizeof(*st));
+  memset(per_graph, 0, sizeof(gmk_graph_stats) * n_graphs);
+  const auto t0 = std::chrono::steady_clock::now();
+  int r;
+  if ((r = ensure(c, c->counters, sizeof(RunCounters)))) return r;
+  HIPCHK(c, hipMemsetAsync(c->counters.p, 0, sizeof(RunCounters), c->stream));
+  if ((r = ensure(c, c->gcnt, (size_t)n_graphs * 24 + 24))) return r;
+  HIPCHK(c, hipMemsetAsync(c->gcnt.p, 0, (size_t)n_graphs * 24 + 24, c->stream));
+  if ((r = ensure(c, c->res, 4))) return r;
+  RunState rs{};
+  rs.t0_ms = std::chrono::duration<double, std::milli>(t0.time_since_epoch()).count();
+  rs.keep = false;
+  rs.limit = ~0ull;
+  rs.gcount = (unsigned long long*)c->gcnt.p;
+  rs.gfp = (u64*)((char*)c->gcnt.p + (size_t)n_graphs * 8);
+  rs.gcap = prm->limit ? prm->limit : ~0ull;
+  c->res_words = 0;
+  // widths and validation
+  std::vector<int> W(n_graphs);
+  for (uint32_t i = 0; i < n_graphs; ++i) {
+    const gmk_graph_desc& g = graphs[i];
+    if (!g.off || (!g.adj && g.off[g.n] > 0) || g.n_left > g.n || g.theta == 0) { c->err = "bad graph " + std::to_string(i); return GMK_EINVAL; }
+    const u32 nr = g.n - g.n_left;
+    W[i] = gmk_pick_w(std::max((g.n_left + 63) / 64, (nr + 63) / 64));
+    if (!W[i]) { c->err = "graph " + std::to_string(i) + " has a side above 4096 vertices"; return GMK_EUNSUPPORTED; }
+  }
+  // schedule
+  std::vector<uint32_t> sched(n_graphs);
+  for (uint32_t i = 0; i < n_graphs; ++i) sched[i] = order ? order[i] : i;
+  if (prm->mode == GMK_MODE_LEARNED) {
+    std::vector<double> cost(n_graphs);
+    if ((r = gmk_predict_costs(c, graphs, n_graphs, cost.data()))) return r;
+    std::stable_sort(sched.begin(), sched.end(), [&](uint32_t a, uint32_t b) { return cost[a] > cost[b]; });
+  }
+  if (prm->mode == GMK_MODE_PLAIN) {
+    for (uint32_t i : sched) {
+      const gmk_graph_desc& g = graphs[i];
+      PartitionSet ps;
+      build_global_partition(g.n, g.n_left, g.off, g.adj, g.k, g.theta, W[i], ps, i);
+      const double before = rs.ms_kernel[0];
+      if ((r = gmk_run_bb(c, ps, prm, st, rs))) return r;
+      per_graph[i].ms = rs.ms_kernel[0] - before;
+      ++st->partitions;
+    }
+  } else {
+    // one run per width; graphs pushed so that the first in the schedule ends
+    // on top of the stack (taken first)
+    for (int w : {1, 2, 4, 8, 16, 32, 64}) {
+      PartitionSet ps;
+      u32 cnt = 0;
+      for (auto it = sched.rbegin(); it != sched.rend(); ++it) {
+        const uint32_t i = *it;
+        if (W[i] != w) continue;
+        const gmk_graph_desc& g = graphs[i];
+        build_global_partition(g.n, g.n_left, g.off, g.adj, g.k, g.theta, w, ps, i);
+        ++cnt;
+      }
+      if (!cnt) continue;
+      gmk_params p2 = *prm;
+      p2.mode = GMK_MODE_BATCHED;
+      if ((r = gmk_run_bb(c, ps, &p2, st, rs))) return r;
+      st->partitions += cnt;
+    }
+  }
+  HIPCHK(c, hipMemcpyAsync(c->h_cnt, c->counters.p, sizeof(RunCounters), hipMemcpyDeviceToHost, c->stream));
+  std::vector<u64> gc((size_t)n_graphs * 3);
+  HIPCHK(c, hipMemcpyAsync(gc.data(), c->gcnt.p, (size_t)n_graphs * 24, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  const RunCounters& h = *c->h_cnt;
+  for (uint32_t i = 0; i < n_graphs; ++i) {
+    per_graph[i].count = std::min<u64>(gc[i], rs.gcap);
+    per_graph[i].hit_cap = gc[i] >= rs.gcap;
+    per_graph[i].fp_sum = gc[n_graphs + 2 * i];
+    per_graph[i].fp_xor = gc[n_graphs + 2 * i + 1];
+    st->count += per_graph[i].count;
+    st->fp_sum += per_graph[i].fp_sum;
+    st->fp_xor ^= per_graph[i].fp_xor;
+  }
+  for (int i = 0; i < NSTRIPE; ++i) {
+    st->states += h.stat[NSTAT * i];
+    st->intersections += h.stat[NSTAT * i + 1];
+    for (int q = 0; q < 4; ++q) st->bytes_kernel[q] += h.stat[NSTAT * i + 3 + q];
+  }
+  for (int q = 0; q < 4; ++q) {
+    st->algo_bytes += st->bytes_kernel[q];
+    st->ms_kernel[q] = rs.ms_kernel[q];
+    st->launches_kernel[q] = rs.launches_kernel[q];
+  }
+  st->launches = rs.launches;
+  st->ms_expand = rs.ms_expand;
+  st->ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   c->prog_count = std::move(rs.prog_count);
   c->prog_ms = std::move(rs.prog_ms);
   return GMK_OK;

@@ -19,6 +19,9 @@ struct PartitionSet {
 
 struct RunState {
   DedupTable dedup{};
+  unsigned long long* gcount = nullptr;  // per-graph caps (multi-graph batches)
+  u64* gfp = nullptr;
+  u64 gcap = ~0ull;
   bool keep = false;
   u64 limit = ~0ull;
   u64 res_cap = 0;
@@ -36,7 +39,8 @@ struct RunState {
 };
 
 // Whole graph as one partition (local IDs = global IDs, right side shifted).
-void build_global_partition(u32 n, u32 n_left, const u32* off, const u32* adj, u32 theta, int W, PartitionSet& ps);
+void build_global_partition(u32 n, u32 n_left, const u32* off, const u32* adj, u32 k, u32 theta, int W, PartitionSet& ps,
+                            u32 graph = 0);
 
 // Appends a root state (header + 6W words) for partition pid.
 u64* push_root(PartitionSet& ps, u32 pid);
@@ -58,7 +62,7 @@ using gmk::NSTRIPE;
 
 // Device counters of one run, read back after every batch.
 struct RunCounters {
-  u32 out_n, retry_n, flags, pad;
+  u32 out_n, retry_n, flags, pad;  // pad: work counter of the persistent expansion kernel
   u64 res_words, res_count;
   u64 fp[NSTRIPE * 2];
   u64 stat[NSTRIPE * 8];  // 0 states, 1 intersections, 2 emitted leaves, 3-6 bytes per kernel
@@ -72,6 +76,7 @@ struct DeviceVec {
 
 struct gmk_ctx {
   int device = 0;
+  u32 persistent_waves = 256 * 16;  // waves of the persistent expansion kernel (16 per CU)
   hipStream_t stream = nullptr;
   std::string err;
   // graph (host copy kept for partition construction)
@@ -83,6 +88,7 @@ struct gmk_ctx {
   DeviceVec stack, next, retry, rows, maps, parts, res, counters;
   DeviceVec tuples, arena, fams, prefix, tcnt;  // thin-result kernels
   DeviceVec dedup;                              // result fingerprint table
+  DeviceVec gcnt;                               // per-graph counters of a batch
   std::vector<u32> order;                        // degeneracy order (seeded runs)
   bool order_valid = false;
   RunCounters* h_cnt = nullptr;  // pinned
@@ -90,6 +96,8 @@ struct gmk_ctx {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   std::vector<u64> prog_count;   // progress log of the last run
   std::vector<double> prog_ms;
+  bool has_model = false;        // cost model for GMK_MODE_LEARNED
+  gmk_mlp model{};
 };
 
 inline int ensure(gmk_ctx* c, DeviceVec& v, size_t bytes) {
@@ -108,6 +116,7 @@ inline int ensure(gmk_ctx* c, DeviceVec& v, size_t bytes) {
 
 
 int gmk_pick_w(uint32_t words);
+int gmk_predict_costs(gmk_ctx* c, const gmk_graph_desc* graphs, uint32_t n, double* cost);
 int gmk_run_bb(gmk_ctx* c, const gmk::PartitionSet& ps, const gmk_params* prm, gmk_stats* st, gmk::RunState& rs);
 int gmk_run_seeded(gmk_ctx* c, const gmk_params* prm, gmk_stats* st, gmk::RunState& rs);
 // Thin results with the tuple on side_a (0 left, 1 right); generated side >= min_b.

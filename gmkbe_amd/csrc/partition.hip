@@ -19,7 +19,8 @@ u64* push_root(PartitionSet& ps, u32 pid) {
   return ps.roots.data() + at + 1;
 }
 
-void build_global_partition(u32 n, u32 n_left, const u32* off, const u32* adj, u32 theta, int W, PartitionSet& ps) {
+void build_global_partition(u32 n, u32 n_left, const u32* off, const u32* adj, u32 k, u32 theta, int W, PartitionSet& ps,
+                            u32 graph) {
   const u32 nr = n - n_left;
   const u32 wl = (n_left + 63) / 64, wr = (nr + 63) / 64;
   ps.W = W;
@@ -45,6 +46,8 @@ void build_global_partition(u32 n, u32 n_left, const u32* off, const u32* adj, u
   for (u32 v = 0; v < nr; ++v) ps.maps.push_back(n_left + v);
   P.lb = theta;
   P.seed = NONE;
+  P.k = k;
+  P.graph = graph;
   const u32 pid = (u32)ps.parts.size();
   ps.parts.push_back(P);
   u64* st = push_root(ps, pid);
@@ -116,6 +119,8 @@ int gmk_run_seeded(gmk_ctx* c, const gmk_params* prm, gmk_stats* st, RunState& r
     P.nl = nl; P.nr = nr; P.wl = wl; P.wr = wr;
     P.lb = lb;
     P.seed = s;
+    P.k = k;
+    P.graph = 0;
     P.map_l = ps.maps.size();
     ps.maps.insert(ps.maps.end(), sp.left.begin(), sp.left.end());
     P.map_r = ps.maps.size();

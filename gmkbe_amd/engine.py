@@ -53,6 +53,41 @@ class Stats(ctypes.Structure):
         return d
 
 
+MLP_IN, MLP_HID = 8, 64
+
+
+class GraphDesc(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_uint32), ("n_left", ctypes.c_uint32), ("off", ctypes.POINTER(ctypes.c_uint32)),
+                ("adj", ctypes.POINTER(ctypes.c_uint32)), ("k", ctypes.c_uint32), ("theta", ctypes.c_uint32)]
+
+
+class GraphStats(ctypes.Structure):
+    _fields_ = [("count", ctypes.c_uint64), ("fp_sum", ctypes.c_uint64), ("fp_xor", ctypes.c_uint64),
+                ("ms", ctypes.c_double), ("hit_cap", ctypes.c_int32), ("pad", ctypes.c_int32)]
+
+
+class Mlp(ctypes.Structure):
+    _fields_ = [("w1", ctypes.c_float * (MLP_IN * MLP_HID)), ("b1", ctypes.c_float * MLP_HID),
+                ("w2", ctypes.c_float * (MLP_HID * MLP_HID)), ("b2", ctypes.c_float * MLP_HID),
+                ("w3", ctypes.c_float * MLP_HID), ("b3", ctypes.c_float),
+                ("mu", ctypes.c_float * MLP_IN), ("sd", ctypes.c_float * MLP_IN)]
+
+    def arrays(self) -> dict:
+        """Parameters as numpy arrays (w1: in x hid, w2: hid x hid)."""
+        a = lambda f, shape=None: np.ctypeslib.as_array(f).copy().reshape(shape) if shape else np.ctypeslib.as_array(f).copy()
+        return {"w1": a(self.w1, (MLP_IN, MLP_HID)), "b1": a(self.b1), "w2": a(self.w2, (MLP_HID, MLP_HID)),
+                "b2": a(self.b2), "w3": a(self.w3), "b3": np.float32(self.b3), "mu": a(self.mu), "sd": a(self.sd)}
+
+    @staticmethod
+    def from_arrays(d: dict) -> "Mlp":
+        m = Mlp()
+        for name in ("w1", "b1", "w2", "b2", "w3", "mu", "sd"):
+            src = np.ascontiguousarray(np.asarray(d[name], dtype=np.float32).ravel())
+            ctypes.memmove(getattr(m, name), src.ctypes.data, src.nbytes)
+        m.b3 = float(d["b3"])
+        return m
+
+
 class GmkError(RuntimeError):
     pass
 
@@ -87,6 +122,21 @@ def load_library():
     L.gmk_fetch_progress.argtypes = [ctypes.c_void_p, P(ctypes.c_uint64), P(ctypes.c_double), ctypes.c_uint32,
                                      P(ctypes.c_uint32)]
     L.gmk_fetch_progress.restype = ctypes.c_int
+    L.gmk_enumerate_batch.argtypes = [ctypes.c_void_p, P(GraphDesc), ctypes.c_uint32, P(Params), u32p,
+                                      P(GraphStats), P(Stats)]
+    L.gmk_enumerate_batch.restype = ctypes.c_int
+    fp = P(ctypes.c_float)
+    L.gmk_graph_features.argtypes = [P(GraphDesc), fp]
+    L.gmk_graph_features.restype = ctypes.c_int
+    L.gmk_mlp_train.argtypes = [ctypes.c_void_p, fp, fp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_float,
+                                ctypes.c_uint32, P(Mlp), fp]
+    L.gmk_mlp_train.restype = ctypes.c_int
+    L.gmk_mlp_predict.argtypes = [ctypes.c_void_p, P(Mlp), fp, ctypes.c_uint32, fp]
+    L.gmk_mlp_predict.restype = ctypes.c_int
+    L.gmk_mlp_grad.argtypes = [ctypes.c_void_p, P(Mlp), fp, fp, ctypes.c_uint32, P(Mlp), fp]
+    L.gmk_mlp_grad.restype = ctypes.c_int
+    L.gmk_set_cost_model.argtypes = [ctypes.c_void_p, P(Mlp)]
+    L.gmk_set_cost_model.restype = ctypes.c_int
     _lib = L
     return L
 
@@ -173,3 +223,84 @@ class Engine:
         ms = (ctypes.c_double * max(1, n.value))()
         self._check(self._L.gmk_fetch_progress(self._h, cnt, ms, n.value, ctypes.byref(n)), "gmk_fetch_progress")
         return [(int(cnt[i]), float(ms[i])) for i in range(n.value)]
+
+
+def _graph_desc(g: Graph, k: int, theta: int, keep: list) -> GraphDesc:
+    off = np.ascontiguousarray(g.off, dtype=np.uint32)
+    adj = np.ascontiguousarray(g.adj, dtype=np.uint32)
+    keep += [off, adj]
+    p = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))
+    return GraphDesc(g.n, g.n_left, p(off), p(adj), k, theta)
+
+
+def graph_features(g: Graph, k: int, theta: int = 1) -> np.ndarray:
+    """The cost model's input features of one graph (gmk_graph_features)."""
+    L = load_library()
+    keep: list = []
+    d = _graph_desc(g, k, theta, keep)
+    out = np.zeros(MLP_IN, dtype=np.float32)
+    rc = L.gmk_graph_features(ctypes.byref(d), out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)))
+    if rc:
+        raise GmkError(f"gmk_graph_features: {STATUS.get(rc, rc)}")
+    return out
+
+
+def _fp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+def _engine_batch_methods():
+    def enumerate_batch(self, graphs, limit: int = 0, mode: str = "batched", order=None):
+        """graphs: list of (Graph, k, theta).  Returns (per-graph stats dicts, total stats dict)."""
+        keep: list = []
+        arr = (GraphDesc * len(graphs))(*[_graph_desc(g, k, t, keep) for g, k, t in graphs])
+        prm = Params(0, 1, limit, MODES[mode], PARTS["global"], 0, 0, 0, 0, 0)
+        per = (GraphStats * len(graphs))()
+        tot = Stats()
+        op = None
+        if order is not None:
+            o = np.ascontiguousarray(order, dtype=np.uint32)
+            keep.append(o)
+            op = o.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))
+        self._check(self._L.gmk_enumerate_batch(self._h, arr, len(graphs), ctypes.byref(prm), op, per,
+                                                ctypes.byref(tot)), "gmk_enumerate_batch")
+        out = [{"count": p.count, "fp_sum": p.fp_sum, "fp_xor": p.fp_xor, "ms": p.ms, "hit_cap": bool(p.hit_cap)}
+               for p in per]
+        return out, tot.as_dict()
+
+    def mlp_train(self, X, y, epochs: int = 500, lr: float = 1e-2, seed: int = 1):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        y = np.ascontiguousarray(y, dtype=np.float32)
+        m = Mlp()
+        loss = ctypes.c_float()
+        self._check(self._L.gmk_mlp_train(self._h, _fp(X), _fp(y), len(y), epochs, lr, seed, ctypes.byref(m),
+                                          ctypes.byref(loss)), "gmk_mlp_train")
+        return m, float(loss.value)
+
+    def mlp_predict(self, model: Mlp, X):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        y = np.zeros(len(X), dtype=np.float32)
+        self._check(self._L.gmk_mlp_predict(self._h, ctypes.byref(model), _fp(X), len(X), _fp(y)), "gmk_mlp_predict")
+        return y
+
+    def mlp_grad(self, model: Mlp, X, y):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        y = np.ascontiguousarray(y, dtype=np.float32)
+        g = Mlp()
+        loss = ctypes.c_float()
+        self._check(self._L.gmk_mlp_grad(self._h, ctypes.byref(model), _fp(X), _fp(y), len(y), ctypes.byref(g),
+                                         ctypes.byref(loss)), "gmk_mlp_grad")
+        return g, float(loss.value)
+
+    def set_cost_model(self, model):
+        self._check(self._L.gmk_set_cost_model(self._h, ctypes.byref(model) if model is not None else None),
+                    "gmk_set_cost_model")
+
+    Engine.enumerate_batch = enumerate_batch
+    Engine.mlp_train = mlp_train
+    Engine.mlp_predict = mlp_predict
+    Engine.mlp_grad = mlp_grad
+    Engine.set_cost_model = set_cost_model
+
+
+_engine_batch_methods()

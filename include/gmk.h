@@ -115,6 +115,72 @@ int gmk_fetch_results(gmk_ctx* ctx, uint32_t* words, uint64_t cap_words, uint64_
  * the log length; at most cap entries are copied. */
 int gmk_fetch_progress(gmk_ctx* ctx, uint64_t* counts, double* ms, uint32_t cap, uint32_t* n);
 
+/* ---- Many graphs in one run -------------------------------------------- */
+
+typedef struct {
+  uint32_t n, n_left;      /* as gmk_set_graph_csr */
+  const uint32_t* off;     /* n + 1 offsets (host memory) */
+  const uint32_t* adj;     /* neighbour lists, ascending (host memory) */
+  uint32_t k, theta;       /* per-graph k and size lower bound */
+} gmk_graph_desc;
+
+typedef struct {
+  uint64_t count;          /* results kept for this graph (<= params->limit when set) */
+  uint64_t fp_sum, fp_xor; /* order-free digest of the kept results */
+  double ms;               /* GMK_MODE_PLAIN: device time of this graph's run */
+  int32_t hit_cap;         /* 1 when the per-graph cap was reached */
+  int32_t pad;
+} gmk_graph_stats;
+
+/* Enumerate many graphs in one call (the reference runs one graph per process).
+ * Each graph is one global partition (both sides <= 4096).  params->limit is a
+ * per-graph cap (0 = none); params->k / theta are ignored (per-graph values).
+ *   GMK_MODE_PLAIN    graphs one after another, one block per state
+ *   GMK_MODE_BATCHED  all graphs' states in one device stack, drained by the
+ *                     persistent expansion kernel, graphs taken in `order`
+ *                     (null = input order)
+ *   GMK_MODE_LEARNED  as BATCHED, with the order given by the cost model set
+ *                     with gmk_set_cost_model (most expensive first), or by
+ *                     the size heuristic (edges x vertices) when none is set.
+ * per_graph receives n_graphs entries. */
+int gmk_enumerate_batch(gmk_ctx* ctx, const gmk_graph_desc* graphs, uint32_t n_graphs, const gmk_params* params,
+                        const uint32_t* order, gmk_graph_stats* per_graph, gmk_stats* total);
+
+/* ---- Learned cost model (no counterpart in the reference) ---------------
+ * A small FP32 MLP, GMK_MLP_IN features -> 64 -> 64 -> 1 with ReLU, predicts
+ * log(device ms) of a work item.  Training, inference and gradients run in
+ * the library's own HIP kernels.  The model only orders work: results do not
+ * depend on it. */
+#define GMK_MLP_IN 8
+#define GMK_MLP_HID 64
+typedef struct {
+  float w1[GMK_MLP_IN * GMK_MLP_HID];   /* [input][hidden] */
+  float b1[GMK_MLP_HID];
+  float w2[GMK_MLP_HID * GMK_MLP_HID];  /* [hidden][hidden] */
+  float b2[GMK_MLP_HID];
+  float w3[GMK_MLP_HID];
+  float b3;
+  float mu[GMK_MLP_IN], sd[GMK_MLP_IN]; /* input standardisation */
+} gmk_mlp;
+
+/* Features of a graph work item: log(1+nL), log(1+nR), log(1+|E|),
+ * log(1+mean left degree), log(1+mean right degree), log(1+max degree),
+ * density |E|/(nL nR), k. */
+int gmk_graph_features(const gmk_graph_desc* g, float out[GMK_MLP_IN]);
+
+/* Full-batch Adam on mean squared error; X is n x GMK_MLP_IN row-major (raw
+ * features), y the targets.  Standardisation is fitted on X.  *loss receives
+ * the final training loss. */
+int gmk_mlp_train(gmk_ctx* ctx, const float* X, const float* y, uint32_t n, uint32_t epochs, float lr, uint32_t seed,
+                  gmk_mlp* model, float* loss);
+int gmk_mlp_predict(gmk_ctx* ctx, const gmk_mlp* model, const float* X, uint32_t n, float* y);
+/* Loss and gradient of the loss with respect to every parameter (mu/sd of the
+ * gradient are left zero), for gradient checks. */
+int gmk_mlp_grad(gmk_ctx* ctx, const gmk_mlp* model, const float* X, const float* y, uint32_t n, gmk_mlp* grad,
+                 float* loss);
+/* Cost model used by GMK_MODE_LEARNED (null: the size heuristic). */
+int gmk_set_cost_model(gmk_ctx* ctx, const gmk_mlp* model);
+
 #ifdef __cplusplus
 }
 #endif
