@@ -40,7 +40,98 @@ WORKLOADS = {
                     desc="Chung-Lu bipartite |U|=2000000 |V|=1000000 m=20000000 alpha=2.0 (seeded), k=2, theta=1, r=10000000"),
     "planted500k": dict(k=3, theta=1, limit=10_000_000, cpu_sample=200,
                         desc="Chung-Lu 500000x500000 m=5000000 alpha=2.2 + 200 planted near-bicliques (seeded), k=3, theta=1, r=10000000"),
+    "er_batch": dict(k=0, theta=0, limit=1000, cpu_sample=0,
+                     desc="2000 ER bipartite graphs, sides log-uniform in [20,2000], p log-uniform in [0.01,0.2], "
+                          "k uniform in {1,2}, theta=2k+1, at most 1000 results per graph, one call"),
 }
+
+
+def bench_batch(args, cfg) -> int:
+    """Many graphs in one call: plain (graph by graph) against batched (one
+    persistent drain), size-heuristic order and learned order.  The cost model
+    is trained on per-graph device times of a plain run over a separate batch."""
+    import numpy as np
+    from gmkbe_amd import engine, gen
+    from gmkbe_amd.engine import Engine
+
+    def make(count, seed):
+        out = []
+        for a, b, p, k, sd in gen.er_batch_specs(count, seed):
+            out.append((gen.er(a, b, p, sd), k, 2 * k + 1))
+        return out
+
+    log("[bench] generating 2000 graphs")
+    t0 = time.time()
+    graphs = make(2000, gen.SEED_C3)
+    train = make(400, gen.SEED_C3 + 1000)
+    log(f"[bench] generated ({time.time() - t0:.1f} s), {sum(g.n_edges for g, _, _ in graphs)} edges")
+    eng = Engine(0)
+    cap = cfg["limit"]
+
+    def run(gs, mode, chunk=None):
+        """enumerate_batch over gs; plain runs (graph by graph anyway) go in
+        chunks with a line per chunk."""
+        if not chunk:
+            return eng.enumerate_batch(gs, limit=cap, mode=mode)
+        per, tot = [], None
+        for i in range(0, len(gs), chunk):
+            t1 = time.perf_counter()
+            p, t = eng.enumerate_batch(gs[i:i + chunk], limit=cap, mode=mode)
+            per += p
+            if tot is None:
+                tot = dict(t)
+            else:
+                for key in ("count", "states", "launches"):
+                    tot[key] += t[key]
+            log(f"[bench]   graphs {i}..{i + len(p) - 1}: {t['count']} results, {time.perf_counter() - t1:.2f} s")
+        return per, tot
+
+    # cost model: per-graph device time of plain runs on the training batch
+    log("[bench] training data")
+    per_train, _ = run(train, "plain", chunk=100)
+    X = np.stack([engine.graph_features(g, k, t) for g, k, t in train])
+    y = np.log(np.array([max(p["ms"], 1e-3) for p in per_train], dtype=np.float32))
+    model, loss = eng.mlp_train(X, y, epochs=800, lr=5e-3, seed=1)
+    Xb = np.stack([engine.graph_features(g, k, t) for g, k, t in graphs])
+    runs = {}
+    per_ref = None
+    for name, mode, mdl in (("plain", "plain", None), ("batched", "batched", None), ("heuristic", "learned", None),
+                            ("learned", "learned", model)):
+        eng.set_cost_model(mdl)
+        log(f"[bench] {name}")
+        eng.enumerate_batch(graphs[:50], limit=cap, mode=mode)  # warm-up
+        times = []
+        for _ in range(max(1, args.steps)):
+            t1 = time.perf_counter()
+            per, tot = run(graphs, mode, chunk=250 if mode == "plain" else None)
+            times.append(time.perf_counter() - t1)
+        dt = min(times)
+        if per_ref is None:
+            per_ref = per
+        consistent = all(a["count"] == b["count"] and (a["hit_cap"] or (a["fp_sum"], a["fp_xor"]) == (b["fp_sum"], b["fp_xor"]))
+                         for a, b in zip(per_ref, per))
+        runs[name] = {"s": dt, "results": tot["count"], "rate": tot["count"] / dt, "consistent_with_plain": consistent,
+                      "launches": tot["launches"], "states": tot["states"]}
+        log(f"[bench] {name}: {dt:.3f} s, {tot['count']} results, {tot['count'] / dt:.3e}/s")
+    eng.set_cost_model(None)
+    pred = eng.mlp_predict(model, Xb)
+    per_plain_ms = np.array([max(p["ms"], 1e-3) for p in per_ref])
+    corr = float(np.corrcoef(pred, np.log(per_plain_ms))[0, 1])
+    eng.close()
+    out = {
+        "metric": METRIC, "value": runs["learned"]["rate"], "unit": "biplexes/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": 1, "ms_per_step": runs["learned"]["s"] * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "u32", "data": "synthetic",
+        "config": {"workload": cfg["desc"], "name": "er_batch", "graphs": len(graphs), "cap_per_graph": cap},
+        "runs": runs,
+        "speedup_batched_vs_plain": runs["plain"]["s"] / runs["batched"]["s"],
+        "speedup_learned_vs_plain": runs["plain"]["s"] / runs["learned"]["s"],
+        "speedup_learned_vs_heuristic": runs["heuristic"]["s"] / runs["learned"]["s"],
+        "cost_model": {"train_graphs": len(train), "train_loss": loss,
+                       "corr_pred_vs_plain_log_ms": corr},
+    }
+    print(json.dumps(out), flush=True)
+    return 0
 
 
 def log(msg: str) -> None:
@@ -137,6 +228,8 @@ def main() -> int:
     from gmkbe_amd.engine import Engine
 
     cfg = WORKLOADS[args.workload]
+    if args.workload == "er_batch":
+        return bench_batch(args, cfg)
     log(f"[bench] generating {args.workload}")
     t0 = time.time()
     g = gen.workload(args.workload)

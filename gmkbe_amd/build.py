@@ -9,7 +9,7 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(PKG, "csrc")
 OUT_DIR = os.path.join(PKG, "_lib")
 LIB = os.path.join(OUT_DIR, "libgmk.so")
-SOURCES = ["bb_kernel.hip", "gmk_engine.hip", "partition.hip", "thin_kernels.hip", "thin_host.hip", "seeds.cpp", "mlp.hip"]
+SOURCES = ["bb_kernel.hip", "bb_run.hip", "gmk_engine.hip", "partition.hip", "thin_kernels.hip", "thin_host.hip", "seeds.cpp", "mlp.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # -packed-fp32-ops: keep the compiler from emitting packed FP32 instructions.
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++20", "-fPIC", "-Wall", "-Wno-unused-function",
@@ -30,7 +30,7 @@ def build(verbose: bool = False) -> str:
         subprocess.run(cmd, check=True)
         objs.append(obj)
     tmp = LIB + ".tmp"
-    subprocess.run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp], check=True)
+    subprocess.run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-pthread", *objs, "-o", tmp], check=True)
     os.replace(tmp, LIB)
     # command-line driver over the C ABI
     cli = os.path.join(OUT_DIR, "gmkbpe")

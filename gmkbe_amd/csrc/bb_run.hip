@@ -1,0 +1,511 @@
+// Host driver of the branch-and-bound kernels: one device state stack per
+// bitset width, drained in batches, with partitions fed while the stacks run.
+//
+// Seed partitions (BPPivot constructSets, biplex.cpp:315-393) are built on host
+// threads only when the stacks run low, graph by graph in schedule order, with
+// a per-graph quota that doubles on every visit.  A graph that reaches its
+// result cap, or a run that reaches its limit, stops receiving partitions, so a
+// capped search builds and expands only the partitions it needs.
+#include "gmk_device.h"
+#include "gmk_host.h"
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cstring>
+#include <deque>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/gmk.h"
+
+using namespace gmk;
+
+namespace gmk {
+hipError_t launch_bb_expand(int W, u32 n_states, hipStream_t s, const ExpandArgs& a, u32 persistent_waves);
+
+namespace {
+constexpr int NLANE = 7;  // widths 1, 2, 4, 8, 16, 32, 64 words
+int lane_of(int W) {
+  int l = 0;
+  while ((1 << l) < W) ++l;
+  return l;
+}
+}  // namespace
+
+BBEngine::BBEngine(gmk_ctx* c, const gmk_params* prm, RunState& rs, bool persistent)
+    : c_(c), prm_(prm), rs_(rs), persistent_(persistent) {}
+
+int BBEngine::reserve(size_t rows_words, size_t maps_words, size_t nparts) {
+  int r;
+  if ((r = ensure(c_, c_->rows, std::max<size_t>(8, rows_words * 8)))) return r;
+  if ((r = ensure(c_, c_->maps, std::max<size_t>(4, maps_words * 4)))) return r;
+  if ((r = ensure(c_, c_->parts, std::max<size_t>(sizeof(PartDesc), nparts * sizeof(PartDesc))))) return r;
+  rows_cap_ = c_->rows.bytes / 8;
+  maps_cap_ = c_->maps.bytes / 4;
+  parts_cap_ = c_->parts.bytes / sizeof(PartDesc);
+  return GMK_OK;
+}
+
+bool BBEngine::fits(const BuiltPart& b) const {
+  return rows_used_ + b.rows.size() <= rows_cap_ && maps_used_ + b.maps.size() <= maps_cap_ &&
+         parts_used_ + 1 <= parts_cap_;
+}
+
+void BBEngine::add(BuiltPart& b) {
+  const u32 pid = (u32)parts_used_;
+  PartDesc P = b.P;
+  P.rows_l += rows_used_;
+  P.rows_r += rows_used_;
+  P.map_l += maps_used_;
+  P.map_r += maps_used_;
+  rows_stage_.insert(rows_stage_.end(), b.rows.begin(), b.rows.end());
+  maps_stage_.insert(maps_stage_.end(), b.maps.begin(), b.maps.end());
+  parts_stage_.push_back(P);
+  rows_used_ += b.rows.size();
+  maps_used_ += b.maps.size();
+  ++parts_used_;
+  std::vector<u64>& rt = roots_stage_[lane_of(b.W)];
+  rt.push_back(pid);
+  rt.insert(rt.end(), b.root.begin(), b.root.end());
+  ++staged_roots_;
+}
+
+int BBEngine::add_set(const PartitionSet& ps) {
+  int r;
+  if (rows_used_ + ps.rows.size() > rows_cap_ || maps_used_ + ps.maps.size() > maps_cap_ ||
+      parts_used_ + ps.parts.size() > parts_cap_) {
+    if (live() != 0 || rows_used_ != 0) { c_->err = "partition tables full"; return GMK_EOVERFLOW; }
+    if ((r = reserve(ps.rows.size(), ps.maps.size(), ps.parts.size()))) return r;
+  }
+  const u64 base_pid = parts_used_;
+  for (PartDesc P : ps.parts) {
+    P.rows_l += rows_used_;
+    P.rows_r += rows_used_;
+    P.map_l += maps_used_;
+    P.map_r += maps_used_;
+    parts_stage_.push_back(P);
+  }
+  rows_stage_.insert(rows_stage_.end(), ps.rows.begin(), ps.rows.end());
+  maps_stage_.insert(maps_stage_.end(), ps.maps.begin(), ps.maps.end());
+  rows_used_ += ps.rows.size();
+  maps_used_ += ps.maps.size();
+  parts_used_ += ps.parts.size();
+  const size_t stride = 1 + 6 * (size_t)ps.W;
+  std::vector<u64>& rt = roots_stage_[lane_of(ps.W)];
+  for (size_t i = 0; i + stride <= ps.roots.size(); i += stride) {
+    rt.push_back(ps.roots[i] + base_pid);
+    rt.insert(rt.end(), ps.roots.begin() + i + 1, ps.roots.begin() + i + stride);
+    ++staged_roots_;
+  }
+  return GMK_OK;
+}
+
+void BBEngine::reset_tables() {
+  rows_used_ = maps_used_ = parts_used_ = 0;
+  rows_dev_ = maps_dev_ = parts_dev_ = 0;
+}
+
+u64 BBEngine::live() const {
+  u64 n = staged_roots_;
+  for (int l = 0; l < NLANE; ++l) n += top_[l];
+  return n;
+}
+
+int BBEngine::lane_alloc(int l) {
+  if (cap_[l]) return GMK_OK;
+  const int W = 1 << l;
+  const u64 state_bytes = (1 + 6ull * W) * 8;
+  // 4 GiB of states for widths up to 4 words, 8 GiB up to 16, 16 GiB above
+  // (HBM3E holds 288 GB: wide searches branch on up to 4096 candidates); the
+  // children buffer is half of it
+  const u64 bytes = W <= 4 ? (4ull << 30) : W <= 16 ? (8ull << 30) : (16ull << 30);
+  const u64 cap = std::max<u64>(1 << 17, bytes / state_bytes);
+  int r;
+  if ((r = ensure(c_, c_->lane_stack[l], cap * state_bytes))) return r;
+  if ((r = ensure(c_, c_->lane_next[l], cap / 2 * state_bytes))) return r;
+  if ((r = ensure(c_, c_->lane_retry[l], (u64)kBatchMax * state_bytes))) return r;
+  cap_[l] = cap;
+  return GMK_OK;
+}
+
+// Uploads staged table entries and pushes staged roots on their stacks (the
+// first-fed root on top, expanded first).
+int BBEngine::upload() {
+  hipStream_t s = c_->stream;
+  if (!rows_stage_.empty())
+    HIPCHK(c_, hipMemcpyAsync((u64*)c_->rows.p + rows_dev_, rows_stage_.data(), rows_stage_.size() * 8,
+                              hipMemcpyHostToDevice, s));
+  if (!maps_stage_.empty())
+    HIPCHK(c_, hipMemcpyAsync((u32*)c_->maps.p + maps_dev_, maps_stage_.data(), maps_stage_.size() * 4,
+                              hipMemcpyHostToDevice, s));
+  if (!parts_stage_.empty())
+    HIPCHK(c_, hipMemcpyAsync((PartDesc*)c_->parts.p + parts_dev_, parts_stage_.data(),
+                              parts_stage_.size() * sizeof(PartDesc), hipMemcpyHostToDevice, s));
+  rows_dev_ += rows_stage_.size();
+  maps_dev_ += maps_stage_.size();
+  parts_dev_ += parts_stage_.size();
+  for (int l = 0; l < NLANE; ++l) {
+    std::vector<u64>& rt = roots_stage_[l];
+    if (rt.empty()) continue;
+    int r;
+    if ((r = lane_alloc(l))) return r;
+    const size_t stride = 1 + 6 * ((size_t)1 << l);
+    const size_t n = rt.size() / stride;
+    if (top_[l] + n > cap_[l]) { c_->err = "state stack full"; return GMK_EOVERFLOW; }
+    // reverse, so that the first staged root ends on top
+    std::vector<u64>& rev = rev_stage_[l];
+    rev.resize(rt.size());
+    for (size_t i = 0; i < n; ++i)
+      std::memcpy(rev.data() + (n - 1 - i) * stride, rt.data() + i * stride, stride * 8);
+    HIPCHK(c_, hipMemcpyAsync((u64*)c_->lane_stack[l].p + top_[l] * stride, rev.data(), rev.size() * 8,
+                              hipMemcpyHostToDevice, s));
+    top_[l] += n;
+  }
+  staged_roots_ = 0;
+  return GMK_OK;
+}
+
+// Host staging buffers are released after the stream has synchronised.
+void BBEngine::clear_stage() {
+  rows_stage_.clear();
+  maps_stage_.clear();
+  parts_stage_.clear();
+  for (int l = 0; l < NLANE; ++l) { roots_stage_[l].clear(); rev_stage_[l].clear(); }
+}
+
+int BBEngine::step() {
+  int r;
+  if ((r = upload())) return r;
+  hipStream_t s = c_->stream;
+  RunCounters* dc = (RunCounters*)c_->counters.p;
+  ExpandArgs a{};
+  a.parts = (const PartDesc*)c_->parts.p;
+  a.rows = (const u64*)c_->rows.p;
+  a.maps = (const u32*)c_->maps.p;
+  a.k = prm_->k;
+  a.keep = rs_.keep;
+  a.res = (u32*)c_->res.p;
+  a.res_cap = rs_.res_cap;
+  a.res_words = &dc->res_words;
+  a.res_count = &dc->res_count;
+  a.limit = rs_.limit;
+  a.fp = dc->fp;
+  a.stat = dc->stat;
+  a.flags = &dc->flags;
+  a.dedup = rs_.dedup;
+  a.gcount = rs_.gcount;
+  a.gfp = rs_.gfp;
+  a.gcap = rs_.gcap;
+  u32 B[NLANE] = {0};
+  u32 out_cap[NLANE] = {0};
+  HIPCHK(c_, hipMemsetAsync(dc->lane, 0, sizeof(dc->lane), s));
+  HIPCHK(c_, hipEventRecord(c_->ev0, s));
+  int launched = 0;
+  for (int l = 0; l < NLANE; ++l) {
+    if (!top_[l]) continue;
+    const int W = 1 << l;
+    const u64 stride = 1 + 6ull * W;
+    const u64 room = cap_[l] - top_[l];
+    // Fan-out of the last batch (children per parent): the next batch takes no
+    // more parents than the free stack can hold at that fan-out, so a search
+    // that branches widely drifts toward depth-first instead of overflowing.
+    const u64 by_room = (u64)((double)room / (2.0 * std::max(fanout_[l], 1.0)));
+    B[l] = (u32)std::max<u64>(1, std::min<u64>({top_[l], (u64)kBatchMax, by_room}));
+    out_cap[l] = (u32)std::min<u64>(cap_[l] / 2, room);
+    // A parent whose children do not fit is requeued; with the stack nearly full
+    // only the top (deepest) states run, and the step fails below when none of
+    // them fits.
+    if (out_cap[l] == 0) { c_->err = "state stack full"; return GMK_EOVERFLOW; }
+    u64* stack = (u64*)c_->lane_stack[l].p;
+    a.in = stack + (top_[l] - B[l]) * stride;
+    a.n_in = B[l];
+    a.out = (u64*)c_->lane_next[l].p;
+    a.out_cap = out_cap[l];
+    a.retry = (u64*)c_->lane_retry[l].p;
+    a.out_n = &dc->lane[l][0];
+    a.retry_n = &dc->lane[l][1];
+    a.work_next = &dc->lane[l][2];
+    HIPCHK(c_, launch_bb_expand(W, B[l], s, a, persistent_ ? c_->persistent_waves : 0u));
+    ++launched;
+  }
+  HIPCHK(c_, hipEventRecord(c_->ev1, s));
+  HIPCHK(c_, hipMemcpyAsync(c_->h_cnt, dc, offsetof(RunCounters, fp), hipMemcpyDeviceToHost, s));
+  HIPCHK(c_, hipStreamSynchronize(s));
+  clear_stage();
+  float ms = 0;
+  HIPCHK(c_, hipEventElapsedTime(&ms, c_->ev0, c_->ev1));
+  rs_.ms_expand += ms;
+  rs_.ms_kernel[0] += ms;
+  rs_.launches += launched;
+  rs_.launches_kernel[0] += launched;
+  const RunCounters& h = *c_->h_cnt;
+  rs_.progress(std::min<u64>(h.res_count, rs_.limit));
+  if (h.flags & FLAG_RES_OVERFLOW) {
+    c_->err = "result buffer overflow (raise result_capacity_words)";
+    return GMK_EOVERFLOW;
+  }
+  for (int l = 0; l < NLANE; ++l) {
+    if (!B[l]) continue;
+    const u64 stride = 1 + 6ull * (1u << l);
+    const u64 state_bytes = stride * 8;
+    const u64 nout = std::min<u64>(h.lane[l][0], out_cap[l]);
+    const u64 nret = h.lane[l][1];
+    if (B[l] > nret) fanout_[l] = 0.5 * fanout_[l] + 0.5 * (double)h.lane[l][0] / (double)(B[l] - nret);
+    u64* stack = (u64*)c_->lane_stack[l].p;
+    u64 dst = top_[l] - B[l];
+    if (nret)
+      HIPCHK(c_, hipMemcpyAsync(stack + dst * stride, c_->lane_retry[l].p, nret * state_bytes, hipMemcpyDeviceToDevice,
+                                s));
+    dst += nret;
+    if (nout)
+      HIPCHK(c_, hipMemcpyAsync(stack + dst * stride, c_->lane_next[l].p, nout * state_bytes, hipMemcpyDeviceToDevice,
+                                s));
+    top_[l] = dst + nout;
+    if (nret == B[l] && h.lane[l][0] >= out_cap[l]) {
+      // every parent was requeued: the free stack is smaller than the children
+      // of each state that ran
+      c_->err = "state stack full (no parent's children fit)";
+      return GMK_EOVERFLOW;
+    }
+  }
+  if (h.flags & FLAG_LIMIT) rs_.hit_limit = true;
+  return GMK_OK;
+}
+
+int BBEngine::drain() {
+  while (live() > 0 && !rs_.hit_limit) {
+    int r = step();
+    if (r) return r;
+  }
+  return GMK_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Partition construction off the main thread.
+// ---------------------------------------------------------------------------
+
+// Rows, maps and root state of a seed partition, with offsets relative to the
+// part's own vectors.  Returns false when a side exceeds 4096 vertices.
+static bool make_built(const SeedJob& job, const SeedPartition& sp, std::vector<u32>& loc, BuiltPart& b) {
+  const HostGraph& g = job.g;
+  const u32 nl = (u32)sp.left.size(), nr = (u32)sp.right.size();
+  const u32 wl = (nl + 63) / 64, wr = (nr + 63) / 64;
+  const int W = gmk_pick_w(std::max(wl, wr));
+  if (!W) return false;
+  b.W = W;
+  PartDesc& P = b.P;
+  P = PartDesc{};
+  P.nl = nl; P.nr = nr; P.wl = wl; P.wr = wr;
+  P.lb = job.lb;
+  P.seed = sp.seed;
+  P.k = job.k;
+  P.graph = job.graph;
+  P.map_l = 0;
+  P.map_r = nl;
+  P.rows_l = 0;
+  P.rows_r = (u64)nl * wr;
+  b.maps.resize((size_t)nl + nr);
+  std::copy(sp.left.begin(), sp.left.end(), b.maps.begin());
+  std::copy(sp.right.begin(), sp.right.end(), b.maps.begin() + nl);
+  for (u32 i = 0; i < nl; ++i) loc[sp.left[i]] = i;
+  for (u32 i = 0; i < nr; ++i) loc[sp.right[i]] = i;
+  b.rows.assign((size_t)nl * wr + (size_t)nr * wl, 0);
+  for (u32 i = 0; i < nl; ++i) {
+    const u32 u = sp.left[i];
+    u64* row = b.rows.data() + (size_t)i * wr;
+    for (u32 e = g.off[u]; e < g.off[u + 1]; ++e) {
+      const u32 q = loc[g.adj[e]];
+      if (q != NONE && g.adj[e] >= g.n_left) row[q / 64] |= 1ull << (q % 64);
+    }
+  }
+  for (u32 i = 0; i < nr; ++i) {
+    const u32 v = sp.right[i];
+    u64* row = b.rows.data() + P.rows_r + (size_t)i * wl;
+    for (u32 e = g.off[v]; e < g.off[v + 1]; ++e) {
+      const u32 q = loc[g.adj[e]];
+      if (q != NONE && g.adj[e] < g.n_left) row[q / 64] |= 1ull << (q % 64);
+    }
+  }
+  for (u32 v : sp.left) loc[v] = NONE;
+  for (u32 v : sp.right) loc[v] = NONE;
+  // root: S = {seed}, C / X from the flags (0 = C, 1 = X, 2 = S)
+  b.root.assign(6 * (size_t)W, 0);
+  for (int side = 0; side < 2; ++side) {
+    const std::vector<uint8_t>& fl = side ? sp.rflag : sp.lflag;
+    for (u32 i = 0; i < fl.size(); ++i) {
+      const int which = fl[i] == 2 ? 0 : (fl[i] == 0 ? 1 : 2);
+      b.root[(side * 3 + which) * W + i / 64] |= 1ull << (i % 64);
+    }
+  }
+  return true;
+}
+
+namespace {
+struct BuildTask {
+  u32 job;
+  u32 seed;
+};
+struct BuildOut {
+  bool ok = false;       // a partition was built
+  bool too_wide = false; // a side exceeds 4096 vertices
+  BuiltPart part;
+};
+struct Worker {
+  std::vector<int32_t> cnt;
+  std::vector<uint8_t> mark;
+  std::vector<u32> loc;
+  SeedPartition sp;
+};
+
+// Builds the partitions of tasks[i] into outs[i] on up to nthreads threads.
+void build_tasks(std::vector<SeedJob>& jobs, const std::vector<BuildTask>& tasks, std::vector<BuildOut>& outs,
+                 std::vector<Worker>& workers) {
+  outs.clear();
+  outs.resize(tasks.size());
+  std::atomic<size_t> next{0};
+  auto run = [&](Worker& w) {
+    for (;;) {
+      const size_t i = next.fetch_add(1);
+      if (i >= tasks.size()) break;
+      SeedJob& job = jobs[tasks[i].job];
+      if (!build_seed_partition(job.g, job.order, tasks[i].seed, job.k, job.lb, w.sp, w.cnt, w.mark)) continue;
+      if (make_built(job, w.sp, w.loc, outs[i].part)) outs[i].ok = true;
+      else outs[i].too_wide = true;
+    }
+  };
+  const size_t nt = std::min(workers.size(), std::max<size_t>(1, tasks.size() / 16));
+  if (nt <= 1) {
+    run(workers[0]);
+    return;
+  }
+  std::vector<std::thread> th;
+  for (size_t t = 1; t < nt; ++t) th.emplace_back(run, std::ref(workers[t]));
+  run(workers[0]);
+  for (auto& t : th) t.join();
+}
+}  // namespace
+
+}  // namespace gmk
+
+// Runs the seed partitions of the jobs (seeds of each job in degeneracy order).
+//   plain, one job:        one partition at a time, each drained before the next
+//                          (the per-partition processing of the reference);
+//   one_job_per_run:       jobs one after another, each drained before the next
+//                          (ms_per_job receives each job's device time);
+//   otherwise (batched):   all jobs share the stacks; partitions are fed while
+//                          the stacks run, jobs in order with a doubling quota.
+int run_seed_jobs(gmk_ctx* c, std::vector<SeedJob>& jobs, const gmk_params* prm, gmk_stats* st, RunState& rs,
+                  bool one_job_per_run, double* ms_per_job) {
+  const bool plain = prm->mode == GMK_MODE_PLAIN;
+  const bool per_part = plain && !one_job_per_run;
+  BBEngine eng(c, prm, rs, !plain);
+  int r;
+  if ((r = eng.reserve((size_t)1 << 27, (size_t)1 << 25, (size_t)1 << 21))) return r;  // 1 GiB of rows
+  u32 nmax = 0, ngraphs = 0;
+  for (auto& j : jobs) { nmax = std::max(nmax, j.g.n); ngraphs = std::max(ngraphs, j.graph + 1); }
+  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+  std::vector<Worker> workers(per_part ? 1 : std::min(16u, hw));
+  for (Worker& w : workers) {
+    w.cnt.assign(nmax, 0);
+    w.mark.assign(nmax, 0);
+    w.loc.assign(nmax, NONE);
+  }
+  const u64 low_water = per_part ? 1 : 8192;   // feed when fewer states than this are live
+  const size_t want = per_part ? 1 : 2048;     // seeds examined per feed
+  std::vector<u32> quota(jobs.size(), 32);
+  std::vector<u64> gcnt_h(ngraphs, 0);
+  std::deque<BuiltPart> carry;                 // built, waiting for table room
+  std::vector<BuildTask> tasks;
+  std::vector<BuildOut> outs;
+  size_t cur = 0;                              // first job not exhausted
+  double job_t0 = rs.ms_kernel[0];
+  auto capped = [&](const SeedJob& j) { return rs.gcount && gcnt_h[j.graph] >= rs.gcap; };
+  auto done = [&](const SeedJob& j) { return j.next >= j.seeds.size() || capped(j); };
+  for (;;) {
+    if (rs.hit_limit) break;
+    // feed
+    while (!carry.empty() && eng.fits(carry.front())) {
+      eng.add(carry.front());
+      carry.pop_front();
+    }
+    if (carry.empty() && eng.live() < low_water) {
+      if (rs.gcount) {
+        HIPCHK(c, hipMemcpyAsync(gcnt_h.data(), rs.gcount, (size_t)ngraphs * 8, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+      }
+      while (cur < jobs.size() && done(jobs[cur])) {
+        if (one_job_per_run) break;
+        ++cur;
+      }
+      tasks.clear();
+      if (per_part) {
+        // one partition: build seeds until one survives the reductions
+        while (cur < jobs.size() && !done(jobs[cur])) {
+          SeedJob& j = jobs[cur];
+          tasks.assign(1, BuildTask{(u32)cur, j.seeds[j.next++]});
+          build_tasks(jobs, tasks, outs, workers);
+          if (outs[0].too_wide) break;
+          if (outs[0].ok) break;
+          tasks.clear();
+        }
+        if (jobs.size() > cur && done(jobs[cur]) && tasks.empty()) ++cur;
+      } else {
+        const size_t last = one_job_per_run ? std::min(cur + 1, jobs.size()) : jobs.size();
+        for (size_t ji = cur; ji < last && tasks.size() < want; ++ji) {
+          SeedJob& j = jobs[ji];
+          if (done(j)) continue;
+          const size_t take = std::min<size_t>({(size_t)quota[ji], j.seeds.size() - j.next, want - tasks.size()});
+          for (size_t q = 0; q < take; ++q) tasks.push_back(BuildTask{(u32)ji, j.seeds[j.next++]});
+          quota[ji] = std::min<u32>(quota[ji] * 2, 1u << 16);
+        }
+        if (!tasks.empty()) build_tasks(jobs, tasks, outs, workers);
+      }
+      for (size_t i = 0; i < tasks.size(); ++i) {
+        BuildOut& o = outs[i];
+        if (o.too_wide) {
+          c->err = "seed partition of vertex " + std::to_string(tasks[i].seed) + " has a side above 4096 vertices";
+          return GMK_EUNSUPPORTED;
+        }
+        if (!o.ok) continue;
+        st->partitions += 1;
+        if (carry.empty() && eng.fits(o.part)) eng.add(o.part);
+        else carry.push_back(std::move(o.part));
+      }
+    }
+    if (eng.live() == 0) {
+      if (!carry.empty()) {  // tables full and every state drained: start over
+        eng.reset_tables();
+        continue;
+      }
+      eng.reset_tables();
+      if (one_job_per_run) {
+        if (cur >= jobs.size()) break;
+        if (done(jobs[cur])) {
+          if (ms_per_job) ms_per_job[cur] = rs.ms_kernel[0] - job_t0;
+          job_t0 = rs.ms_kernel[0];
+          ++cur;
+          if (cur >= jobs.size()) break;
+        }
+        continue;
+      }
+      bool more = false;
+      for (size_t ji = cur; ji < jobs.size() && !more; ++ji) more = !done(jobs[ji]);
+      if (!more) break;
+      continue;
+    }
+    if ((r = eng.step())) return r;
+  }
+  return GMK_OK;
+}
+
+// Whole partition set, all roots at once.
+int gmk_run_bb(gmk_ctx* c, const PartitionSet& ps, const gmk_params* prm, gmk_stats* st, RunState& rs) {
+  (void)st;
+  BBEngine eng(c, prm, rs, prm->mode != GMK_MODE_PLAIN);
+  int r;
+  if ((r = eng.reserve(ps.rows.size(), ps.maps.size(), ps.parts.size()))) return r;
+  if ((r = eng.add_set(ps))) return r;
+  return eng.drain();
+}

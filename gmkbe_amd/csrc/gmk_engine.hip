@@ -46,9 +46,12 @@ extern "C" void gmk_destroy(gmk_ctx* c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
-  for (DeviceVec* v : {&c->stack, &c->next, &c->retry, &c->rows, &c->maps, &c->parts, &c->res, &c->counters,
-                        &c->tuples, &c->arena, &c->fams, &c->prefix, &c->tcnt, &c->dedup, &c->gcnt})
+  for (DeviceVec* v : {&c->rows, &c->maps, &c->parts, &c->res, &c->counters, &c->tuples, &c->arena, &c->fams,
+                        &c->prefix, &c->tcnt, &c->dedup, &c->gcnt})
     if (v->p) (void)hipFree(v->p);
+  for (int l = 0; l < 7; ++l)
+    for (DeviceVec* v : {&c->lane_stack[l], &c->lane_next[l], &c->lane_retry[l]})
+      if (v->p) (void)hipFree(v->p);
   if (c->d_off) (void)hipFree(c->d_off);
   if (c->d_adj) (void)hipFree(c->d_adj);
   if (c->h_cnt) (void)hipHostFree(c->h_cnt);
@@ -188,100 +191,6 @@ int gmk_pick_w(u32 words) {
   return 0;
 }
 
-int gmk_run_bb(gmk_ctx* c, const PartitionSet& ps, const gmk_params* prm, gmk_stats* st, RunState& rs) {
-  const int W = ps.W;
-  const u64 stride = 1 + 6ull * W;
-  const u64 state_bytes = stride * 8;
-  // Device stack: ~1.5 GiB, children buffer half of it.
-  const u64 cap = std::max<u64>(1 << 16, (3ull << 29) / state_bytes);
-  const u64 ncap = cap / 2;
-  const u32 batch_max = 1u << 16;
-  int r;
-  if ((r = ensure(c, c->stack, cap * state_bytes))) return r;
-  if ((r = ensure(c, c->next, ncap * state_bytes))) return r;
-  if ((r = ensure(c, c->retry, (u64)batch_max * state_bytes))) return r;
-  if ((r = ensure(c, c->rows, std::max<size_t>(8, ps.rows.size() * 8)))) return r;
-  if ((r = ensure(c, c->maps, std::max<size_t>(4, ps.maps.size() * 4)))) return r;
-  if ((r = ensure(c, c->parts, std::max<size_t>(sizeof(PartDesc), ps.parts.size() * sizeof(PartDesc))))) return r;
-  if (ps.roots.size() / stride > cap) { c->err = "too many root states"; return GMK_EOVERFLOW; }
-  hipStream_t s = c->stream;
-  HIPCHK(c, hipMemcpyAsync(c->rows.p, ps.rows.data(), ps.rows.size() * 8, hipMemcpyHostToDevice, s));
-  HIPCHK(c, hipMemcpyAsync(c->maps.p, ps.maps.data(), ps.maps.size() * 4, hipMemcpyHostToDevice, s));
-  HIPCHK(c, hipMemcpyAsync(c->parts.p, ps.parts.data(), ps.parts.size() * sizeof(PartDesc), hipMemcpyHostToDevice, s));
-  HIPCHK(c, hipMemcpyAsync(c->stack.p, ps.roots.data(), ps.roots.size() * 8, hipMemcpyHostToDevice, s));
-  u64 top = ps.roots.size() / stride;
-
-  RunCounters* dc = (RunCounters*)c->counters.p;
-  ExpandArgs a{};
-  a.parts = (const PartDesc*)c->parts.p;
-  a.rows = (const u64*)c->rows.p;
-  a.maps = (const u32*)c->maps.p;
-  a.k = prm->k;
-  a.keep = rs.keep;
-  a.res = (u32*)c->res.p;
-  a.res_cap = rs.res_cap;
-  a.res_words = &dc->res_words;
-  a.res_count = &dc->res_count;
-  a.limit = rs.limit;
-  a.fp = dc->fp;
-  a.stat = dc->stat;
-  a.flags = &dc->flags;
-  a.dedup = rs.dedup;
-  a.gcount = rs.gcount;
-  a.gfp = rs.gfp;
-  a.gcap = rs.gcap;
-  a.out_n = &dc->out_n;
-  a.retry_n = &dc->retry_n;
-  a.work_next = &dc->pad;
-  a.retry = (u64*)c->retry.p;
-  a.out = (u64*)c->next.p;
-  u64* stack = (u64*)c->stack.p;
-  const u32 max_children = 2 * 64 * (u32)W + 2;
-
-  while (top > 0) {
-    const u32 B = (u32)std::min<u64>(top, batch_max);
-    const u64 room = cap - top;
-    a.out_cap = (u32)std::min<u64>(ncap, room);
-    if (a.out_cap < max_children) { c->err = "state stack full"; return GMK_EOVERFLOW; }
-    a.in = stack + (top - B) * stride;
-    a.n_in = B;
-    HIPCHK(c, hipMemsetAsync(dc, 0, 2 * sizeof(u32), s));
-    HIPCHK(c, hipMemsetAsync(&dc->pad, 0, sizeof(u32), s));
-    HIPCHK(c, hipEventRecord(c->ev0, s));
-    HIPCHK(c, launch_bb_expand(W, B, s, a, prm->mode == GMK_MODE_PLAIN ? 0u : c->persistent_waves));
-    HIPCHK(c, hipEventRecord(c->ev1, s));
-    HIPCHK(c, hipMemcpyAsync(c->h_cnt, dc, offsetof(RunCounters, fp), hipMemcpyDeviceToHost, s));
-    HIPCHK(c, hipStreamSynchronize(s));
-    float ms = 0;
-    HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
-    rs.ms_expand += ms;
-    ++rs.launches;
-    rs.ms_kernel[0] += ms;
-    ++rs.launches_kernel[0];
-    const RunCounters& h = *c->h_cnt;
-    rs.progress(std::min<u64>(h.res_count, rs.limit));
-    if (h.flags & FLAG_RES_OVERFLOW) {
-      c->err = "result buffer overflow (raise result_capacity_words)";
-      return GMK_EOVERFLOW;
-    }
-    const u64 nout = std::min<u64>(h.out_n, a.out_cap);
-    const u64 nret = h.retry_n;
-    u64 dst = top - B;
-    if (nret) HIPCHK(c, hipMemcpyAsync(stack + dst * stride, a.retry, nret * state_bytes, hipMemcpyDeviceToDevice, s));
-    dst += nret;
-    if (nout) HIPCHK(c, hipMemcpyAsync(stack + dst * stride, a.out, nout * state_bytes, hipMemcpyDeviceToDevice, s));
-    top = dst + nout;
-    if (h.flags & FLAG_LIMIT) { rs.hit_limit = true; break; }
-    if (nout == 0 && nret == B && B > 0 && top > 0 && h.out_n >= a.out_cap) {
-      // every parent was requeued: only possible when the buffer is smaller than
-      // one state's children, which the room check above excludes
-      c->err = "no progress in state expansion";
-      return GMK_EOVERFLOW;
-    }
-  }
-  return GMK_OK;
-}
-
 extern "C" int gmk_enumerate(gmk_ctx* c, const gmk_params* prm, gmk_stats* st) {
   if (!c || !prm) return GMK_EINVAL;
   if (c->n == 0) { c->err = "no graph loaded"; return GMK_ENOGRAPH; }
@@ -405,7 +314,11 @@ extern "C" int gmk_enumerate_batch(gmk_ctx* c, const gmk_graph_desc* graphs, uin
     if (!g.off || (!g.adj && g.off[g.n] > 0) || g.n_left > g.n || g.theta == 0) { c->err = "bad graph " + std::to_string(i); return GMK_EINVAL; }
     const u32 nr = g.n - g.n_left;
     W[i] = gmk_pick_w(std::max((g.n_left + 63) / 64, (nr + 63) / 64));
-    if (!W[i]) { c->err = "graph " + std::to_string(i) + " has a side above 4096 vertices"; return GMK_EUNSUPPORTED; }
+    // graphs with theta <= 2k run as one global partition, bounded by the widest state
+    if (!W[i] && g.theta <= 2 * g.k) {
+      c->err = "graph " + std::to_string(i) + " has a side above 4096 vertices and theta <= 2k";
+      return GMK_EUNSUPPORTED;
+    }
   }
   // schedule
   std::vector<uint32_t> sched(n_graphs);
@@ -415,8 +328,23 @@ extern "C" int gmk_enumerate_batch(gmk_ctx* c, const gmk_graph_desc* graphs, uin
     if ((r = gmk_predict_costs(c, graphs, n_graphs, cost.data()))) return r;
     std::stable_sort(sched.begin(), sched.end(), [&](uint32_t a, uint32_t b) { return cost[a] > cost[b]; });
   }
+  // Graphs with theta > 2k run as seed partitions (every result has both sides
+  // >= 2k+1, the regime where 2-hop seed partitions are exact); the others are
+  // one global partition each.
+  std::vector<SeedJob> jobs;
+  std::vector<uint32_t> job_graph;
+  std::vector<uint32_t> global_graphs;
+  for (uint32_t i : sched) {
+    const gmk_graph_desc& g = graphs[i];
+    if (g.theta > 2 * g.k) {
+      jobs.push_back(make_seed_job(HostGraph{g.n, g.n_left, g.off, g.adj}, g.k, g.theta, i));
+      job_graph.push_back(i);
+    } else {
+      global_graphs.push_back(i);
+    }
+  }
   if (prm->mode == GMK_MODE_PLAIN) {
-    for (uint32_t i : sched) {
+    for (uint32_t i : global_graphs) {
       const gmk_graph_desc& g = graphs[i];
       PartitionSet ps;
       build_global_partition(g.n, g.n_left, g.off, g.adj, g.k, g.theta, W[i], ps, i);
@@ -425,25 +353,32 @@ extern "C" int gmk_enumerate_batch(gmk_ctx* c, const gmk_graph_desc* graphs, uin
       per_graph[i].ms = rs.ms_kernel[0] - before;
       ++st->partitions;
     }
+    std::vector<double> ms(jobs.size(), 0.0);
+    if ((r = run_seed_jobs(c, jobs, prm, st, rs, true, ms.data()))) return r;
+    for (size_t q = 0; q < jobs.size(); ++q) per_graph[job_graph[q]].ms = ms[q];
   } else {
-    // one run per width; graphs pushed so that the first in the schedule ends
-    // on top of the stack (taken first)
-    for (int w : {1, 2, 4, 8, 16, 32, 64}) {
-      PartitionSet ps;
-      u32 cnt = 0;
-      for (auto it = sched.rbegin(); it != sched.rend(); ++it) {
-        const uint32_t i = *it;
-        if (W[i] != w) continue;
+    gmk_params p2 = *prm;
+    p2.mode = GMK_MODE_BATCHED;
+    // global partitions of every width in one run; the first graph of the
+    // schedule is expanded first
+    if (!global_graphs.empty()) {
+      std::vector<PartitionSet> sets(7);
+      size_t rows = 0, maps = 0, parts = 0;
+      for (uint32_t i : global_graphs) {
         const gmk_graph_desc& g = graphs[i];
-        build_global_partition(g.n, g.n_left, g.off, g.adj, g.k, g.theta, w, ps, i);
-        ++cnt;
+        int l = 0;
+        while ((1 << l) < W[i]) ++l;
+        build_global_partition(g.n, g.n_left, g.off, g.adj, g.k, g.theta, W[i], sets[l], i);
       }
-      if (!cnt) continue;
-      gmk_params p2 = *prm;
-      p2.mode = GMK_MODE_BATCHED;
-      if ((r = gmk_run_bb(c, ps, &p2, st, rs))) return r;
-      st->partitions += cnt;
+      for (auto& ps : sets) { rows += ps.rows.size(); maps += ps.maps.size(); parts += ps.parts.size(); }
+      BBEngine eng(c, &p2, rs, true);
+      if ((r = eng.reserve(rows, maps, parts))) return r;
+      for (auto& ps : sets)
+        if (!ps.parts.empty() && (r = eng.add_set(ps))) return r;
+      if ((r = eng.drain())) return r;
+      st->partitions += global_graphs.size();
     }
+    if ((r = run_seed_jobs(c, jobs, &p2, st, rs, false, nullptr))) return r;
   }
   HIPCHK(c, hipMemcpyAsync(c->h_cnt, c->counters.p, sizeof(RunCounters), hipMemcpyDeviceToHost, c->stream));
   std::vector<u64> gc((size_t)n_graphs * 3);

@@ -4,6 +4,7 @@
 #include <vector>
 
 #include "gmk_device.h"
+#include "seeds.h"
 #include "../../include/gmk.h"
 
 namespace gmk {
@@ -62,8 +63,9 @@ using gmk::NSTRIPE;
 
 // Device counters of one run, read back after every batch.
 struct RunCounters {
-  u32 out_n, retry_n, flags, pad;  // pad: work counter of the persistent expansion kernel
+  u32 flags, pad;
   u64 res_words, res_count;
+  u32 lane[8][4];         // per bitset width 2^l: children, requeued parents, persistent work counter, unused
   u64 fp[NSTRIPE * 2];
   u64 stat[NSTRIPE * 8];  // 0 states, 1 intersections, 2 emitted leaves, 3-6 bytes per kernel
   u64 dstat[NSTRIPE * 4]; // dedup: inserts, probes, duplicates
@@ -85,7 +87,8 @@ struct gmk_ctx {
   u32* d_off = nullptr;
   u32* d_adj = nullptr;
   // per-run buffers, grown on demand and reused
-  DeviceVec stack, next, retry, rows, maps, parts, res, counters;
+  DeviceVec lane_stack[7], lane_next[7], lane_retry[7];  // state stacks per bitset width
+  DeviceVec rows, maps, parts, res, counters;
   DeviceVec tuples, arena, fams, prefix, tcnt;  // thin-result kernels
   DeviceVec dedup;                              // result fingerprint table
   DeviceVec gcnt;                               // per-graph counters of a batch
@@ -117,6 +120,67 @@ inline int ensure(gmk_ctx* c, DeviceVec& v, size_t bytes) {
 
 int gmk_pick_w(uint32_t words);
 int gmk_predict_costs(gmk_ctx* c, const gmk_graph_desc* graphs, uint32_t n, double* cost);
+namespace gmk {
+// Seed partitions of one graph, consumed in degeneracy order.
+struct SeedJob {
+  HostGraph g{};
+  u32 k = 1, lb = 3, graph = 0;
+  std::vector<u32> order, seeds;
+  size_t next = 0;
+};
+SeedJob make_seed_job(const HostGraph& g, u32 k, u32 lb, u32 graph);
+}  // namespace gmk
+namespace gmk {
+// A partition built on a host thread: offsets relative to its own vectors.
+struct BuiltPart {
+  int W = 0;
+  PartDesc P{};
+  std::vector<u64> rows;  // nl rows of wr words, then nr rows of wl words
+  std::vector<u32> maps;  // left IDs, then right IDs
+  std::vector<u64> root;  // root state without its partition word (6W words)
+};
+
+// Branch-and-bound over partitions of mixed widths: one device state stack per
+// width (1..64 words), one expansion launch per non-empty stack per step, and
+// append-only device tables of partitions that may grow while states run.
+class BBEngine {
+ public:
+  BBEngine(gmk_ctx* c, const gmk_params* prm, RunState& rs, bool persistent);
+  int reserve(size_t rows_words, size_t maps_words, size_t nparts);
+  bool fits(const BuiltPart& b) const;
+  void add(BuiltPart& b);
+  int add_set(const PartitionSet& ps);
+  void reset_tables();  // only with no live state
+  u64 live() const;     // states on the stacks plus staged roots
+  int step();
+  int drain();
+
+ private:
+  static constexpr u32 kBatchMax = 1u << 16;
+  int lane_alloc(int l);
+  int upload();
+  void clear_stage();
+  gmk_ctx* c_;
+  const gmk_params* prm_;
+  RunState& rs_;
+  bool persistent_;
+  size_t rows_cap_ = 0, maps_cap_ = 0, parts_cap_ = 0;
+  size_t rows_used_ = 0, maps_used_ = 0, parts_used_ = 0;  // including staged entries
+  size_t rows_dev_ = 0, maps_dev_ = 0, parts_dev_ = 0;     // uploaded
+  std::vector<u64> rows_stage_;
+  std::vector<u32> maps_stage_;
+  std::vector<PartDesc> parts_stage_;
+  std::vector<u64> roots_stage_[7], rev_stage_[7];
+  u64 staged_roots_ = 0;
+  u64 top_[7] = {0, 0, 0, 0, 0, 0, 0};
+  u64 cap_[7] = {0, 0, 0, 0, 0, 0, 0};
+  double fanout_[7] = {2, 2, 2, 2, 2, 2, 2};
+};
+}  // namespace gmk
+
+int run_seed_jobs(gmk_ctx* c, std::vector<gmk::SeedJob>& jobs, const gmk_params* prm, gmk_stats* st,
+                  gmk::RunState& rs, bool one_job_per_run, double* ms_per_job);
+
 int gmk_run_bb(gmk_ctx* c, const gmk::PartitionSet& ps, const gmk_params* prm, gmk_stats* st, gmk::RunState& rs);
 int gmk_run_seeded(gmk_ctx* c, const gmk_params* prm, gmk_stats* st, gmk::RunState& rs);
 // Thin results with the tuple on side_a (0 left, 1 right); generated side >= min_b.
