@@ -59,9 +59,17 @@ def convert_edges(u: np.ndarray, v: np.ndarray) -> Graph:
     if u.size == 0:
         raise ValueError("empty edge list")
     ids = np.unique(np.concatenate([u, v]))
-    cu = np.searchsorted(ids, u)
-    cv = np.searchsorted(ids, v)
     nid = ids.size
+    # rank of each raw ID among the IDs in use (a dense table when the IDs are
+    # compact, binary search otherwise)
+    if int(ids[-1]) < 4 * nid + (1 << 20):
+        rank = np.zeros(int(ids[-1]) + 1, dtype=np.int64)
+        rank[ids] = np.arange(nid, dtype=np.int64)
+        cu = rank[u]
+        cv = rank[v]
+    else:
+        cu = np.searchsorted(ids, u)
+        cv = np.searchsorted(ids, v)
     seen_src = np.zeros(nid, dtype=bool)
     seen_dst = np.zeros(nid, dtype=bool)
     seen_src[cu] = True
@@ -92,10 +100,9 @@ def convert_edges(u: np.ndarray, v: np.ndarray) -> Graph:
     gmap[right_nodes] = n_left + np.arange(n_right)
     a = gmap[lu]
     b = gmap[rv]
-    src = np.concatenate([a, b])
-    dst = np.concatenate([b, a])
-    o = np.lexsort((dst, src))
-    src, dst = src[o], dst[o]
+    # pairs are distinct, so sorting src * n + dst orders them by (src, dst)
+    pair = np.sort(np.concatenate([a * n + b, b * n + a]))
+    src, dst = pair // n, pair % n
     off = np.zeros(n + 1, dtype=np.int64)
     np.add.at(off, src + 1, 1)
     off = np.cumsum(off)

@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <cstdio>
 #include <cstring>
 #include <deque>
 #include <string>
@@ -271,6 +272,7 @@ int BBEngine::step() {
     }
   }
   if (h.flags & FLAG_LIMIT) rs_.hit_limit = true;
+  rs_.report("branch-and-bound", std::min<u64>(h.res_count, rs_.limit), live());
   return GMK_OK;
 }
 
@@ -390,42 +392,66 @@ void build_tasks(std::vector<SeedJob>& jobs, const std::vector<BuildTask>& tasks
 }  // namespace gmk
 
 // Runs the seed partitions of the jobs (seeds of each job in degeneracy order).
-//   plain, one job:        one partition at a time, each drained before the next
-//                          (the per-partition processing of the reference);
-//   one_job_per_run:       jobs one after another, each drained before the next
-//                          (ms_per_job receives each job's device time);
-//   otherwise (batched):   all jobs share the stacks; partitions are fed while
-//                          the stacks run, jobs in order with a doubling quota.
+//   plain:    one partition at a time, each drained before the next is built
+//             (the reference's per-partition processing); jobs in order, and
+//             ms_per_job (optional) receives each job's device time;
+//   batched:  all jobs share the stacks; partitions are built on host threads
+//             while the stacks run, jobs in order with a doubling quota.
 int run_seed_jobs(gmk_ctx* c, std::vector<SeedJob>& jobs, const gmk_params* prm, gmk_stats* st, RunState& rs,
-                  bool one_job_per_run, double* ms_per_job) {
+                  double* ms_per_job) {
   const bool plain = prm->mode == GMK_MODE_PLAIN;
-  const bool per_part = plain && !one_job_per_run;
   BBEngine eng(c, prm, rs, !plain);
   int r;
   if ((r = eng.reserve((size_t)1 << 27, (size_t)1 << 25, (size_t)1 << 21))) return r;  // 1 GiB of rows
   u32 nmax = 0, ngraphs = 0;
   for (auto& j : jobs) { nmax = std::max(nmax, j.g.n); ngraphs = std::max(ngraphs, j.graph + 1); }
   const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
-  std::vector<Worker> workers(per_part ? 1 : std::min(16u, hw));
+  std::vector<Worker> workers(plain ? 1 : std::min(16u, hw));
   for (Worker& w : workers) {
     w.cnt.assign(nmax, 0);
     w.mark.assign(nmax, 0);
     w.loc.assign(nmax, NONE);
   }
-  const u64 low_water = per_part ? 1 : 8192;   // feed when fewer states than this are live
-  const size_t want = per_part ? 1 : 2048;     // seeds examined per feed
-  std::vector<u32> quota(jobs.size(), 32);
   std::vector<u64> gcnt_h(ngraphs, 0);
-  std::deque<BuiltPart> carry;                 // built, waiting for table room
-  std::vector<BuildTask> tasks;
-  std::vector<BuildOut> outs;
-  size_t cur = 0;                              // first job not exhausted
-  double job_t0 = rs.ms_kernel[0];
   auto capped = [&](const SeedJob& j) { return rs.gcount && gcnt_h[j.graph] >= rs.gcap; };
   auto done = [&](const SeedJob& j) { return j.next >= j.seeds.size() || capped(j); };
+  std::vector<BuildTask> tasks;
+  std::vector<BuildOut> outs;
+  auto too_wide = [&](u32 seed) {
+    c->err = "seed partition of vertex " + std::to_string(seed) + " has a side above 4096 vertices";
+    return GMK_EUNSUPPORTED;
+  };
+
+  if (plain) {
+    for (size_t ji = 0; ji < jobs.size() && !rs.hit_limit; ++ji) {
+      SeedJob& j = jobs[ji];
+      const double t0 = rs.ms_kernel[0];
+      while (!done(j) && !rs.hit_limit) {
+        tasks.assign(1, BuildTask{(u32)ji, j.seeds[j.next++]});
+        build_tasks(jobs, tasks, outs, workers);
+        if (outs[0].too_wide) return too_wide(tasks[0].seed);
+        if (!outs[0].ok) continue;
+        ++st->partitions;
+        eng.add(outs[0].part);
+        if ((r = eng.drain())) return r;
+        eng.reset_tables();
+        if (rs.gcount) {
+          HIPCHK(c, hipMemcpyAsync(&gcnt_h[j.graph], rs.gcount + j.graph, 8, hipMemcpyDeviceToHost, c->stream));
+          HIPCHK(c, hipStreamSynchronize(c->stream));
+        }
+      }
+      if (ms_per_job) ms_per_job[ji] = rs.ms_kernel[0] - t0;
+    }
+    return GMK_OK;
+  }
+
+  const u64 low_water = 8192;  // feed when fewer states than this are live
+  const size_t want = 2048;    // seeds examined per feed
+  std::vector<u32> quota(jobs.size(), 32);
+  std::deque<BuiltPart> carry;  // built, waiting for table room
+  size_t cur = 0;               // jobs before cur are done
   for (;;) {
     if (rs.hit_limit) break;
-    // feed
     while (!carry.empty() && eng.fits(carry.front())) {
       eng.add(carry.front());
       carry.pop_front();
@@ -435,61 +461,28 @@ int run_seed_jobs(gmk_ctx* c, std::vector<SeedJob>& jobs, const gmk_params* prm,
         HIPCHK(c, hipMemcpyAsync(gcnt_h.data(), rs.gcount, (size_t)ngraphs * 8, hipMemcpyDeviceToHost, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
       }
-      while (cur < jobs.size() && done(jobs[cur])) {
-        if (one_job_per_run) break;
-        ++cur;
-      }
+      while (cur < jobs.size() && done(jobs[cur])) ++cur;
       tasks.clear();
-      if (per_part) {
-        // one partition: build seeds until one survives the reductions
-        while (cur < jobs.size() && !done(jobs[cur])) {
-          SeedJob& j = jobs[cur];
-          tasks.assign(1, BuildTask{(u32)cur, j.seeds[j.next++]});
-          build_tasks(jobs, tasks, outs, workers);
-          if (outs[0].too_wide) break;
-          if (outs[0].ok) break;
-          tasks.clear();
-        }
-        if (jobs.size() > cur && done(jobs[cur]) && tasks.empty()) ++cur;
-      } else {
-        const size_t last = one_job_per_run ? std::min(cur + 1, jobs.size()) : jobs.size();
-        for (size_t ji = cur; ji < last && tasks.size() < want; ++ji) {
-          SeedJob& j = jobs[ji];
-          if (done(j)) continue;
-          const size_t take = std::min<size_t>({(size_t)quota[ji], j.seeds.size() - j.next, want - tasks.size()});
-          for (size_t q = 0; q < take; ++q) tasks.push_back(BuildTask{(u32)ji, j.seeds[j.next++]});
-          quota[ji] = std::min<u32>(quota[ji] * 2, 1u << 16);
-        }
-        if (!tasks.empty()) build_tasks(jobs, tasks, outs, workers);
+      for (size_t ji = cur; ji < jobs.size() && tasks.size() < want; ++ji) {
+        SeedJob& j = jobs[ji];
+        if (done(j)) continue;
+        const size_t take = std::min<size_t>({(size_t)quota[ji], j.seeds.size() - j.next, want - tasks.size()});
+        for (size_t q = 0; q < take; ++q) tasks.push_back(BuildTask{(u32)ji, j.seeds[j.next++]});
+        quota[ji] = std::min<u32>(quota[ji] * 2, 1u << 16);
       }
+      if (!tasks.empty()) build_tasks(jobs, tasks, outs, workers);
       for (size_t i = 0; i < tasks.size(); ++i) {
         BuildOut& o = outs[i];
-        if (o.too_wide) {
-          c->err = "seed partition of vertex " + std::to_string(tasks[i].seed) + " has a side above 4096 vertices";
-          return GMK_EUNSUPPORTED;
-        }
+        if (o.too_wide) return too_wide(tasks[i].seed);
         if (!o.ok) continue;
-        st->partitions += 1;
+        ++st->partitions;
         if (carry.empty() && eng.fits(o.part)) eng.add(o.part);
         else carry.push_back(std::move(o.part));
       }
     }
     if (eng.live() == 0) {
-      if (!carry.empty()) {  // tables full and every state drained: start over
-        eng.reset_tables();
-        continue;
-      }
-      eng.reset_tables();
-      if (one_job_per_run) {
-        if (cur >= jobs.size()) break;
-        if (done(jobs[cur])) {
-          if (ms_per_job) ms_per_job[cur] = rs.ms_kernel[0] - job_t0;
-          job_t0 = rs.ms_kernel[0];
-          ++cur;
-          if (cur >= jobs.size()) break;
-        }
-        continue;
-      }
+      eng.reset_tables();  // nothing references the tables any more
+      if (!carry.empty()) continue;
       bool more = false;
       for (size_t ji = cur; ji < jobs.size() && !more; ++ji) more = !done(jobs[ji]);
       if (!more) break;

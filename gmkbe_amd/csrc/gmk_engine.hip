@@ -159,6 +159,17 @@ void gmk::RunState::progress(u64 count) {
   prog_ms.push_back(now - t0_ms);
 }
 
+void gmk::RunState::report(const char* phase, u64 count, u64 live) {
+  if (!verbose) return;
+  const double now =
+      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+  if (now - last_print_ms < 10000.0) return;
+  last_print_ms = now;
+  fprintf(stderr, "[gmk] %.1f s %s: %llu results, %.0f ms on device, %llu launches, %llu live states\n",
+          (now - t0_ms) / 1000.0, phase, (unsigned long long)count, ms_expand, (unsigned long long)launches,
+          (unsigned long long)live);
+}
+
 extern "C" int gmk_fetch_progress(gmk_ctx* c, uint64_t* counts, double* ms, uint32_t cap, uint32_t* n) {
   if (!c) return GMK_EINVAL;
   const uint32_t len = (uint32_t)c->prog_count.size();
@@ -203,6 +214,7 @@ extern "C" int gmk_enumerate(gmk_ctx* c, const gmk_params* prm, gmk_stats* st) {
   rs.t0_ms = std::chrono::duration<double, std::milli>(t0.time_since_epoch()).count();
   rs.keep = prm->keep_results != 0;
   rs.limit = prm->limit ? prm->limit : ~0ull;
+  rs.verbose = prm->verbose != 0;
   int r;
   if ((r = ensure(c, c->counters, sizeof(RunCounters)))) return r;
   HIPCHK(c, hipMemsetAsync(c->counters.p, 0, sizeof(RunCounters), c->stream));
@@ -303,6 +315,7 @@ extern "C" int gmk_enumerate_batch(gmk_ctx* c, const gmk_graph_desc* graphs, uin
   rs.t0_ms = std::chrono::duration<double, std::milli>(t0.time_since_epoch()).count();
   rs.keep = false;
   rs.limit = ~0ull;
+  rs.verbose = prm->verbose != 0;
   rs.gcount = (unsigned long long*)c->gcnt.p;
   rs.gfp = (u64*)((char*)c->gcnt.p + (size_t)n_graphs * 8);
   rs.gcap = prm->limit ? prm->limit : ~0ull;
@@ -354,7 +367,7 @@ extern "C" int gmk_enumerate_batch(gmk_ctx* c, const gmk_graph_desc* graphs, uin
       ++st->partitions;
     }
     std::vector<double> ms(jobs.size(), 0.0);
-    if ((r = run_seed_jobs(c, jobs, prm, st, rs, true, ms.data()))) return r;
+    if ((r = run_seed_jobs(c, jobs, prm, st, rs, ms.data()))) return r;
     for (size_t q = 0; q < jobs.size(); ++q) per_graph[job_graph[q]].ms = ms[q];
   } else {
     gmk_params p2 = *prm;
@@ -378,7 +391,7 @@ extern "C" int gmk_enumerate_batch(gmk_ctx* c, const gmk_graph_desc* graphs, uin
       if ((r = eng.drain())) return r;
       st->partitions += global_graphs.size();
     }
-    if ((r = run_seed_jobs(c, jobs, &p2, st, rs, false, nullptr))) return r;
+    if ((r = run_seed_jobs(c, jobs, &p2, st, rs, nullptr))) return r;
   }
   HIPCHK(c, hipMemcpyAsync(c->h_cnt, c->counters.p, sizeof(RunCounters), hipMemcpyDeviceToHost, c->stream));
   std::vector<u64> gc((size_t)n_graphs * 3);

@@ -1,5 +1,6 @@
 // Host-side structures shared by the engine's translation units.
 #pragma once
+#include <chrono>
 #include <string>
 #include <vector>
 
@@ -37,6 +38,10 @@ struct RunState {
   std::vector<double> prog_ms;
   double t0_ms = 0;
   void progress(u64 count);
+  // verbose runs: a line on stderr at most every 10 s of wall time
+  bool verbose = false;
+  double last_print_ms = -1e30;
+  void report(const char* phase, u64 count, u64 live);
 };
 
 // Whole graph as one partition (local IDs = global IDs, right side shifted).
@@ -179,7 +184,7 @@ class BBEngine {
 }  // namespace gmk
 
 int run_seed_jobs(gmk_ctx* c, std::vector<gmk::SeedJob>& jobs, const gmk_params* prm, gmk_stats* st,
-                  gmk::RunState& rs, bool one_job_per_run, double* ms_per_job);
+                  gmk::RunState& rs, double* ms_per_job);
 
 int gmk_run_bb(gmk_ctx* c, const gmk::PartitionSet& ps, const gmk_params* prm, gmk_stats* st, gmk::RunState& rs);
 int gmk_run_seeded(gmk_ctx* c, const gmk_params* prm, gmk_stats* st, gmk::RunState& rs);

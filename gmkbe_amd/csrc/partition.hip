@@ -95,5 +95,5 @@ int gmk_run_seeded(gmk_ctx* c, const gmk_params* prm, gmk_stats* st, RunState& r
   HostGraph g{c->n, c->n_left, c->off.data(), c->adj.data()};
   std::vector<SeedJob> jobs;
   jobs.push_back(make_seed_job(g, k, lb, 0));
-  return run_seed_jobs(c, jobs, prm, st, rs, false, nullptr);
+  return run_seed_jobs(c, jobs, prm, st, rs, nullptr);
 }

@@ -9,15 +9,57 @@
 namespace gmk {
 namespace thin {
 
+// Upper bound on min |free \ N(w)| over w in side A outside L, from a bounded
+// scan: the candidates are the neighbours of the free member of least degree
+// (among the first 16), and each candidate's misses are counted by binary
+// search until they reach the best seen.  Returns k + 1 when no w with at most
+// k misses was seen.  Any value returned is attained by a real w, so a family
+// judged dominated with it is non-maximal (a witness exists); the bound only
+// decides how many families are recognised.  cand0/cstep split the candidates
+// between cooperating lanes (0/1 for one lane).
+GMK_HD u32 dominance_bound(const Graph& g, const u32* L, u32 j, u32 k, const u32* freel, u32 nfree, u32 cand0,
+                           u32 cstep, Cost& cost) {
+  if (nfree == 0) return (g.a_hi - g.a_lo) > j ? 0u : k + 1;
+  u32 vmin = freel[0], dmin = g.off[vmin + 1] - g.off[vmin];
+  for (u32 i = 1; i < nfree && i < 16; ++i) {
+    const u32 v = freel[i], dv = g.off[v + 1] - g.off[v];
+    if (dv < dmin) { vmin = v; dmin = dv; }
+  }
+  cost.bytes += 8ull * (nfree < 16 ? nfree : 16);
+  const u32* nv = g.adj + g.off[vmin];
+  u32 best = k + 1;
+  u32 probes = 0;  // work bound: binary searches per lane
+  for (u32 e = cand0; e < dmin && best > 0 && probes < 4096; e += cstep) {
+    const u32 w = nv[e];
+    cost.bytes += 4;
+    if (in_tuple(L, j, w)) continue;
+    const u32* nw = g.adj + g.off[w];
+    const u32 dw = g.off[w + 1] - g.off[w];
+    u32 miss = 0, i = 0;
+    for (; i < nfree && miss < best && probes < 4096; ++i) {
+      if (freel[i] == vmin) continue;
+      ++probes;
+      if (!contains(nw, dw, freel[i], cost)) ++miss;
+    }
+    if (i == nfree && miss < best) best = miss;
+  }
+  return best;
+}
+
 // Enumerates the maximal count vectors of a tuple and writes its families.
 //   mcnt[m]  number of side-B vertices of type m (m < 2^j; mcnt[full] unused)
 //   far      number of side-B vertices adjacent to no tuple member
 //   poolp[m] address of the list of type-m vertices (m = 0 is the free list)
 //   nlp, d   N(L) ascending (needed only when the far pool is feasible)
+//   dom      a w outside L misses dom free vertices (dominance_bound)
+// A family whose chosen vertices all miss fewer than k tuple members, and with
+// dom + (chosen vertices) <= k, is dropped: that w misses at most k vertices of
+// every R of the family and no vertex it misses is saturated, so w can be added
+// and no member is maximal.
 // JM bounds the tuple size (j <= JM) so the per-lane arrays have 2^JM entries.
 template <int JM, class Fams>
 GMK_HD void emit_families(const u32* L, u32 j, u32 k, u32 min_b, u32 tuple_idx, const u32* mcnt, u64 far,
-                          const u64* poolp, const u32* nlp, u32 d, Fams& fo, Cost& cost) {
+                          const u64* poolp, const u32* nlp, u32 d, u32 dom, Fams& fo, Cost& cost) {
   const u32 full = (1u << j) - 1u;
   const u32 nfree = mcnt[0];
   u8 tmask[(1 << JM)];
@@ -38,7 +80,7 @@ GMK_HD void emit_families(const u32* L, u32 j, u32 k, u32 min_b, u32 tuple_idx, 
     f->freep = (const u32*)poolp[0]; f->nlp = nlp; f->nl_size = d; f->tuple = tuple_idx;
   };
   if (nt == 0) {
-    if (nfree >= min_b && nfree > 0) {
+    if (nfree >= min_b && nfree > 0 && dom > k) {
       Family* f = fo.next();
       cost.bytes += sizeof(Family);
       if (f) { fill(f, nfree); f->ntypes = 0; f->total = 1; }
@@ -79,11 +121,17 @@ GMK_HD void emit_families(const u32* L, u32 j, u32 k, u32 min_b, u32 tuple_idx, 
       if (maximal) {
         u64 rs = nfree, total = 1;
         u32 used = 0;
+        bool unsat = true;  // every chosen vertex misses fewer than k members
         for (u32 q = 0; q < nt; ++q) {
           rs += c[q];
-          if (c[q]) { total = mul_sat(total, binom(psize_of(q), c[q])); ++used; }
+          if (c[q]) {
+            total = mul_sat(total, binom(psize_of(q), c[q]));
+            ++used;
+            if ((u32)popc(tmask[q]) >= k) unsat = false;
+          }
         }
-        if (rs >= min_b && rs > 0 && used <= (u32)MAXT) {
+        const bool dominated = unsat && (u64)dom + (rs - nfree) <= k;
+        if (rs >= min_b && rs > 0 && used <= (u32)MAXT && !dominated) {
           Family* f = fo.next();
           cost.bytes += sizeof(Family);
           if (f) {
@@ -133,7 +181,8 @@ GMK_HD bool build_families_t(const Graph& g, const u32* L, u32 j, u32 k, u32 min
     cost.bytes += 8;
     mcnt[0] = d;
     poolp[0] = (u64)ns;
-    emit_families<JM>(L, 1, k, min_b, tuple_idx, mcnt, (u64)(g.b_hi - g.b_lo) - d, poolp, ns, d, fo, cost);
+    const u32 dom = dominance_bound(g, L, 1, k, ns, d, 0, 1, cost);
+    emit_families<JM>(L, 1, k, min_b, tuple_idx, mcnt, (u64)(g.b_hi - g.b_lo) - d, poolp, ns, d, dom, fo, cost);
     return true;
   }
   u32 pos[JM], end[JM];
@@ -185,7 +234,8 @@ GMK_HD bool build_families_t(const Graph& g, const u32* L, u32 j, u32 k, u32 min
       pools[moff[m] + fillc[m]++] = y;
     }
   }
-  emit_families<JM>(L, j, k, min_b, tuple_idx, mcnt, (u64)(g.b_hi - g.b_lo) - d, poolp, nl, d, fo, cost);
+  const u32 dom = dominance_bound(g, L, j, k, (const u32*)poolp[0], mcnt[0], 0, 1, cost);
+  emit_families<JM>(L, j, k, min_b, tuple_idx, mcnt, (u64)(g.b_hi - g.b_lo) - d, poolp, nl, d, dom, fo, cost);
   return true;
 }
 

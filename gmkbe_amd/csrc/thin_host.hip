@@ -55,6 +55,8 @@ int run_chunk(ThinRun& T, const u32* tuples, u32 count, u32 j) {
     T.rs->launches_kernel[1]++;
   }
   T.rs->launches++;
+  T.rs->report(a.swap_sides ? "thin families (tuples on the right)" : "thin families (tuples on the left)",
+               std::min<u64>(c->h_cnt->res_count, T.rs->limit), 0);
   if (tc.tflags) {
     if (count == 1) { c->err = "one tuple's neighbour lists exceed the arena"; return GMK_EOVERFLOW; }
     const u32 h = count / 2;
@@ -85,6 +87,8 @@ int run_chunk(ThinRun& T, const u32* tuples, u32 count, u32 j) {
     T.rs->launches_kernel[2]++;
     T.rs->launches++;
     T.rs->progress(std::min<u64>(c->h_cnt->res_count, T.rs->limit));
+    T.rs->report(a.swap_sides ? "thin (tuples on the right)" : "thin (tuples on the left)",
+                 std::min<u64>(c->h_cnt->res_count, T.rs->limit), 0);
     if (c->h_cnt->flags & FLAG_RES_OVERFLOW) { c->err = "result buffer overflow (raise result_capacity_words)"; return GMK_EOVERFLOW; }
     if (c->h_cnt->flags & FLAG_LIMIT) { T.stop = true; T.rs->hit_limit = true; return GMK_OK; }
   }
@@ -139,8 +143,14 @@ int gmk_run_thin(gmk_ctx* c, const gmk_params* prm, gmk_stats* st, RunState& rs,
 
   const u32 a_lo = a.g.a_lo, a_hi = a.g.a_hi;
   const u32* off = c->off.data();
-  // seeds per block: plain mode takes one seed at a time
-  const u32 block = prm->mode == GMK_MODE_PLAIN ? 1u : 4096u;
+  // Tuples come in blocks of 4096 first vertices, tuple size outermost within a
+  // block.  Plain mode takes the same order one first vertex per launch
+  // (per-seed processing); the other modes fill each launch.
+  const u32 block = 4096u;
+  const bool per_seed = prm->mode == GMK_MODE_PLAIN;
+  // Launch sizes ramp up from 64 tuples (x4 per launch) to TCH, so a run that
+  // reaches its limit within the first tuples stops early; the order is the same.
+  u32 chunk = 64;
   std::vector<u32> buf;
   buf.reserve((size_t)TCH * thin::MAXJ);
   for (u32 s0 = a_lo; s0 < a_hi && !T.stop; s0 += block) {
@@ -153,6 +163,7 @@ int gmk_run_thin(gmk_ctx* c, const gmk_params* prm, gmk_stats* st, RunState& rs,
         if (count == 0) return GMK_OK;
         int rr = run_chunk(T, buf.data(), count, j);
         buf.clear(); count = 0; est = 0;
+        chunk = std::min<u32>(chunk * 4, TCH);
         return rr;
       };
       for (u32 s = s0; s < s1 && !T.stop; ++s) {
@@ -163,7 +174,7 @@ int gmk_run_thin(gmk_ctx* c, const gmk_params* prm, gmk_stats* st, RunState& rs,
         for (;;) {
           u64 need = 0;
           for (u32 i = 0; i < j; ++i) need += 2ull * (off[idx[i] + 1] - off[idx[i]]);
-          if (count == TCH || est + need > ARENA) { if ((r = flush())) return r; if (T.stop) break; }
+          if (count >= chunk || est + need > ARENA) { if ((r = flush())) return r; if (T.stop) break; }
           buf.insert(buf.end(), idx, idx + j);
           ++count;
           est += need;
@@ -174,6 +185,7 @@ int gmk_run_thin(gmk_ctx* c, const gmk_params* prm, gmk_stats* st, RunState& rs,
           ++idx[p];
           for (u32 q = p + 1; q < j; ++q) idx[q] = idx[q - 1] + 1;
         }
+        if (per_seed && (r = flush())) return r;
       }
       if (!T.stop && (r = flush())) return r;
     }

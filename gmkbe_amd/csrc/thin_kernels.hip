@@ -137,11 +137,15 @@ __global__ __launch_bounds__(256) void thin_build_wave(ThinArgs a) {
     u32 mcnt[1 << J];
     u32 d = 0;
     for (u32 mm = 0; mm <= full; ++mm) { mcnt[mm] = (u32)__shfl((int)cnt_lane, (int)mm, 64); d += mcnt[mm]; }
+    // the free list (type 0) starts the tuple's pools; lanes split the witness scan
+    const u32* freel = a.arena + s_base[wv] + s_off[wv][0];
+    u32 dom = thin::dominance_bound(a.g, L, j, a.k, freel, mcnt[0], (u32)lane, 64u, cost);
+    dom = (u32)wave_min_u64((u64)dom);
     if (lane == 0) {
       u64 poolp[1 << J];
       for (u32 mm = 0; mm <= full; ++mm) poolp[mm] = (u64)(a.arena + s_base[wv] + s_off[wv][mm]);
       DevFamilies fo{a.fams, a.fam_cap, a.fam_n, a.tflags};
-      thin::emit_families<J>(L, j, a.k, a.min_b, t, mcnt, (u64)(a.g.b_hi - a.g.b_lo) - d, poolp, nullptr, d, fo,
+      thin::emit_families<J>(L, j, a.k, a.min_b, t, mcnt, (u64)(a.g.b_hi - a.g.b_lo) - d, poolp, nullptr, d, dom, fo,
                              cost);
     }
   }

@@ -200,9 +200,10 @@ class Engine:
 
     def enumerate(self, k: int = 1, theta: int = 1, limit: int = 0, mode: str = "plain",
                   partition: str = "auto", keep: bool = True, capacity_words: int = 0, dedup: bool = False,
-                  dedup_load: float = 0.5) -> Result:
-        prm = Params(k, theta, limit, MODES[mode], PARTS[partition], int(keep), 0, capacity_words, int(dedup),
-                     int(round(dedup_load * 1000)))
+                  dedup_load: float = 0.5, verbose: bool = False) -> Result:
+        """verbose: the library prints a progress line to stderr every 10 s."""
+        prm = Params(k, theta, limit, MODES[mode], PARTS[partition], int(keep), int(verbose), capacity_words,
+                     int(dedup), int(round(dedup_load * 1000)))
         st = Stats()
         self._check(self._L.gmk_enumerate(self._h, ctypes.byref(prm), ctypes.byref(st)), "gmk_enumerate")
         words = None
@@ -250,11 +251,12 @@ def _fp(a):
 
 
 def _engine_batch_methods():
-    def enumerate_batch(self, graphs, limit: int = 0, mode: str = "batched", order=None):
-        """graphs: list of (Graph, k, theta).  Returns (per-graph stats dicts, total stats dict)."""
+    def enumerate_batch(self, graphs, limit: int = 0, mode: str = "batched", order=None, verbose: bool = False):
+        """graphs: list of (Graph, k, theta).  Returns (per-graph stats dicts, total stats dict).
+        verbose: the library prints a progress line to stderr every 10 s."""
         keep: list = []
         arr = (GraphDesc * len(graphs))(*[_graph_desc(g, k, t, keep) for g, k, t in graphs])
-        prm = Params(0, 1, limit, MODES[mode], PARTS["global"], 0, 0, 0, 0, 0)
+        prm = Params(0, 1, limit, MODES[mode], PARTS["global"], 0, int(verbose), 0, 0, 0)
         per = (GraphStats * len(graphs))()
         tot = Stats()
         op = None

@@ -17,6 +17,9 @@ Workloads (sizes are the ones the benchmarks and tests use):
 """
 from __future__ import annotations
 
+import os
+from concurrent.futures import ThreadPoolExecutor
+
 import numpy as np
 
 from .convert import convert_edges
@@ -55,30 +58,61 @@ def pareto_weights(n: int, alpha: float, total: float, cap: float | None, rng) -
     return x
 
 
+def _sample_cdf(cdf: np.ndarray, x: np.ndarray) -> np.ndarray:
+    """searchsorted(cdf, x, side='right'), split over threads (numpy releases
+    the GIL inside the search)."""
+    n = x.size
+    parts = min(16, max(1, (os.cpu_count() or 1)), max(1, n >> 20))
+    if parts == 1:
+        return np.searchsorted(cdf, x, side="right")
+    out = np.empty(n, dtype=np.int64)
+    bounds = np.linspace(0, n, parts + 1).astype(np.int64)
+
+    def work(i):
+        lo, hi = bounds[i], bounds[i + 1]
+        out[lo:hi] = np.searchsorted(cdf, x[lo:hi], side="right")
+
+    with ThreadPoolExecutor(parts) as ex:
+        list(ex.map(work, range(parts)))
+    return out
+
+
 def chung_lu_edges(n_u: int, n_v: int, m: int, alpha: float, seed: int, cap: float | None = None,
                    alpha_v: float | None = None) -> tuple[np.ndarray, np.ndarray]:
     """Chung-Lu style bipartite graph: endpoints drawn independently with
     probability proportional to Pareto expected degrees on each side, repeated
-    draws collapsed, until m distinct edges exist (first m kept in draw order)."""
+    draws collapsed, until m distinct edges exist (first m kept in draw order).
+
+    Draws come in rounds of 1.15x the edges still missing (+1024); a round's
+    keys are deduplicated within the round (first occurrence) and against a
+    sorted set of the keys kept so far."""
     rng = _rng(seed)
     wu = pareto_weights(n_u, alpha, m, cap, rng)
     wv = pareto_weights(n_v, alpha if alpha_v is None else alpha_v, m, cap, rng)
     cu = np.cumsum(wu); cu /= cu[-1]
     cv = np.cumsum(wv); cv /= cv[-1]
-    keys = np.empty(0, dtype=np.int64)
+    seen = np.empty(0, dtype=np.int64)  # sorted keys kept so far
+    kept = []
+    total = 0
     need = m
     for _ in range(64):
         batch = int(need * 1.15) + 1024
-        a = np.minimum(np.searchsorted(cu, rng.random(batch), side="right"), n_u - 1)
-        b = np.minimum(np.searchsorted(cv, rng.random(batch), side="right"), n_v - 1)
-        k = a.astype(np.int64) * n_v + b
-        keys = np.concatenate([keys, k])
-        _, first = np.unique(keys, return_index=True)
-        keys = keys[np.sort(first)]
-        if keys.size >= m:
-            keys = keys[:m]
+        a = np.minimum(_sample_cdf(cu, rng.random(batch)), n_u - 1)
+        b = np.minimum(_sample_cdf(cv, rng.random(batch)), n_v - 1)
+        k = a * n_v + b
+        _, first = np.unique(k, return_index=True)
+        first.sort()
+        k = k[first]
+        if seen.size:
+            pos = np.minimum(np.searchsorted(seen, k), seen.size - 1)
+            k = k[seen[pos] != k]
+        kept.append(k)
+        total += k.size
+        if total >= m:
             break
-        need = m - keys.size
+        seen = np.sort(np.concatenate([seen, k]), kind="stable")
+        need = m - total
+    keys = np.concatenate(kept)[:m]
     u = (keys // n_v).astype(np.uint32)
     v = (keys % n_v + n_u).astype(np.uint32)
     return u, v

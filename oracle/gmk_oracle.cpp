@@ -685,4 +685,33 @@ long long or_validate(u32 nLeft, u32 N, const u32* off, const u32* adj, u32 k, u
 
 u64 or_fingerprint(const u32* L, u32 nl, const u32* R, u32 nr) { return fp_of(L, nl, R, nr); }
 
+// Fingerprint of every record of a stream into out (capacity cap); returns the
+// record count, or -1 when the stream is malformed or out is too small.
+long long or_fingerprints(const u32* words, u64 nwords, u64* out, u64 cap) {
+  u64 i = 0, n = 0;
+  while (i < nwords) {
+    const u32 nl = words[i];
+    if (i + 1 + nl >= nwords) return -1;
+    const u32 nr = words[i + 1 + nl];
+    if (i + 2 + nl + nr > nwords || n >= cap) return -1;
+    out[n++] = fp_of(words + i + 1, nl, words + i + 2 + nl, nr);
+    i += 2 + nl + nr;
+  }
+  return (long long)n;
+}
+
+// Offsets (in words) of the records of a stream; returns the record count or -1.
+long long or_record_offsets(const u32* words, u64 nwords, u64* out, u64 cap) {
+  u64 i = 0, n = 0;
+  while (i < nwords) {
+    if (n >= cap) return -1;
+    out[n++] = i;
+    const u32 nl = words[i];
+    if (i + 1 + nl >= nwords) return -1;
+    const u32 nr = words[i + 1 + nl];
+    i += 2 + nl + nr;
+  }
+  return i == nwords ? (long long)n : -1;
+}
+
 }  // extern "C"

@@ -94,3 +94,21 @@ extern "C" long long thin_model(unsigned n, unsigned n_left, const unsigned* off
   if (!words.empty()) memcpy(*out_words, words.data(), sizeof(u32) * words.size());
   return count;
 }
+
+// Number of one-vertex tuples of side_a with a dominating vertex: some w outside
+// the tuple misses at most k of the tuple's free list (dominance_bound <= k).
+// Lets the tests check that graphs meant to exercise the family skip do.
+extern "C" long long thin_dominated(unsigned n, unsigned n_left, const unsigned* off, const unsigned* adj, unsigned k,
+                                    int side_a) {
+  Graph g;
+  g.off = off; g.adj = adj;
+  if (side_a == 0) { g.a_lo = 0; g.a_hi = n_left; g.b_lo = n_left; g.b_hi = n; }
+  else { g.a_lo = n_left; g.a_hi = n; g.b_lo = 0; g.b_hi = n_left; }
+  long long c = 0;
+  for (u32 s = g.a_lo; s < g.a_hi; ++s) {
+    Cost cost;
+    const u32 L[1] = {s};
+    if (dominance_bound(g, L, 1, k, adj + off[s], off[s + 1] - off[s], 0, 1, cost) <= k) ++c;
+  }
+  return c;
+}

@@ -44,6 +44,11 @@ def lib():
         L.or_fingerprint.restype = ctypes.c_uint64
         L.or_fingerprint.argtypes = [u32p, ctypes.c_uint32, u32p, ctypes.c_uint32]
         L.or_free.argtypes = [ctypes.c_void_p]
+        u64p = P(ctypes.c_uint64)
+        L.or_fingerprints.restype = ctypes.c_longlong
+        L.or_fingerprints.argtypes = [u32p, ctypes.c_uint64, u64p, ctypes.c_uint64]
+        L.or_record_offsets.restype = ctypes.c_longlong
+        L.or_record_offsets.argtypes = [u32p, ctypes.c_uint64, u64p, ctypes.c_uint64]
         _lib = L
     return _lib
 
@@ -104,6 +109,33 @@ def validate(g, k, theta, words):
     return 0 if r == 0 else (int(r), int(why.value))
 
 
+def fingerprints(words):
+    """Per-record fingerprints (same function as the device digest), uint64."""
+    words = np.ascontiguousarray(words, dtype=np.uint32)
+    cap = max(1, words.size // 2)
+    out = np.empty(cap, dtype=np.uint64)
+    n = lib().or_fingerprints(_p(words), words.size, out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), cap)
+    assert n >= 0, "malformed record stream"
+    return out[:n]
+
+
+def sample_records(words, count, seed=0):
+    """A record stream holding `count` records spread over the stream."""
+    words = np.ascontiguousarray(words, dtype=np.uint32)
+    cap = max(1, words.size // 2)
+    offs = np.empty(cap, dtype=np.uint64)
+    n = lib().or_record_offsets(_p(words), words.size, offs.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), cap)
+    assert n >= 0, "malformed record stream"
+    offs = offs[:n].astype(np.int64)
+    pick = np.sort(np.random.default_rng(seed).choice(n, size=min(count, n), replace=False))
+    parts = []
+    for i in pick:
+        o = offs[i]
+        nl = int(words[o]); nr = int(words[o + 1 + nl])
+        parts.append(words[o:o + 2 + nl + nr])
+    return np.concatenate(parts) if parts else np.zeros(0, dtype=np.uint32)
+
+
 def records(words):
     """Iterate (L tuple, R tuple) over a record stream."""
     i, n = 0, len(words)
@@ -118,8 +150,7 @@ MODEL_PATH = os.path.join(ROOT, "tests", "model", "_build", "libthinmodel.so")
 _model = None
 
 
-def thin_model(g, k, theta, side_a, min_b):
-    """CPU model of the thin-result kernels (tests/model/thin_model.cpp)."""
+def _model_init():
     global _model
     if _model is None:
         _model = ctypes.CDLL(MODEL_PATH)
@@ -128,6 +159,21 @@ def thin_model(g, k, theta, side_a, min_b):
         _model.thin_model.restype = ctypes.c_longlong
         _model.thin_model.argtypes = [ctypes.c_uint32, ctypes.c_uint32, u32p, u32p, ctypes.c_uint32, ctypes.c_uint32,
                                       ctypes.c_int, ctypes.c_uint32, P(u32p), P(ctypes.c_uint64)]
+        _model.thin_dominated.restype = ctypes.c_longlong
+        _model.thin_dominated.argtypes = [ctypes.c_uint32, ctypes.c_uint32, u32p, u32p, ctypes.c_uint32, ctypes.c_int]
+
+
+def thin_dominated(g, k, side_a):
+    """One-vertex tuples of side_a with a dominating vertex (tests/model/thin_model.cpp)."""
+    _model_init()
+    off = np.ascontiguousarray(g.off, dtype=np.uint32)
+    adj = np.ascontiguousarray(g.adj, dtype=np.uint32)
+    return int(_model.thin_dominated(g.n, g.n_left, _p(off), _p(adj), k, side_a))
+
+
+def thin_model(g, k, theta, side_a, min_b):
+    """CPU model of the thin-result kernels (tests/model/thin_model.cpp)."""
+    _model_init()
     off = np.ascontiguousarray(g.off, dtype=np.uint32)
     adj = np.ascontiguousarray(g.adj, dtype=np.uint32)
     wp = ctypes.POINTER(ctypes.c_uint32)()
