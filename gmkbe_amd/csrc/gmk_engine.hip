@@ -392,6 +392,8 @@ extern "C" int gmk_enumerate_batch(gmk_ctx* c, const gmk_graph_desc* graphs, uin
       st->partitions += global_graphs.size();
     }
     if ((r = run_seed_jobs(c, jobs, &p2, st, rs, nullptr))) return r;
+    // graphs share the device in this mode; a lone graph's time is the run's
+    if (n_graphs == 1) per_graph[0].ms = rs.ms_kernel[0];
   }
   HIPCHK(c, hipMemcpyAsync(c->h_cnt, c->counters.p, sizeof(RunCounters), hipMemcpyDeviceToHost, c->stream));
   std::vector<u64> gc((size_t)n_graphs * 3);

@@ -189,14 +189,19 @@ extern "C" int gmk_graph_features(const gmk_graph_desc* g, float out[GMK_MLP_IN]
   const double nl = g->n_left, nr = (double)(g->n - g->n_left);
   const double e = g->off[g->n] / 2.0;
   uint32_t maxd = 0;
-  for (uint32_t v = 0; v < g->n; ++v) maxd = std::max(maxd, g->off[v + 1] - g->off[v]);
+  double two_hop = 0;  // paths of length two: sum over vertices of C(deg, 2)
+  for (uint32_t v = 0; v < g->n; ++v) {
+    const uint32_t d = g->off[v + 1] - g->off[v];
+    maxd = std::max(maxd, d);
+    two_hop += 0.5 * (double)d * (double)(d > 0 ? d - 1 : 0);
+  }
   out[0] = (float)std::log1p(nl);
   out[1] = (float)std::log1p(nr);
   out[2] = (float)std::log1p(e);
   out[3] = (float)std::log1p(nl > 0 ? e / nl : 0);
   out[4] = (float)std::log1p(nr > 0 ? e / nr : 0);
   out[5] = (float)std::log1p((double)maxd);
-  out[6] = (float)(nl > 0 && nr > 0 ? e / (nl * nr) : 0);
+  out[6] = (float)std::log1p(two_hop);
   out[7] = (float)g->k;
   return GMK_OK;
 }
@@ -224,29 +229,71 @@ extern "C" int gmk_mlp_train(gmk_ctx* c, const float* X, const float* y, uint32_
   double ymean = 0;
   for (uint32_t s = 0; s < n; ++s) ymean += y[s];
   m.b3 = (float)(ymean / n);
-  MlpBuffers b;
+  // Early stopping: with 20 or more samples a seeded fifth is held out; every 5
+  // epochs the held-out error is measured and the best parameters are kept.
+  std::vector<uint32_t> perm(n);
+  for (uint32_t s = 0; s < n; ++s) perm[s] = s;
+  const uint32_t nv = n >= 20 ? n / 5 : 0, nt = n - nv;
+  if (nv) std::shuffle(perm.begin(), perm.end(), std::mt19937(seed ^ 0x5EEDu));
+  std::vector<float> xs_all;
+  standardise(m, X, n, xs_all);
+  std::vector<float> xt((size_t)nt * IN), yt(nt), xv((size_t)nv * IN), yv(nv);
+  for (uint32_t s = 0; s < n; ++s) {
+    const uint32_t src = perm[s];
+    float* dx = s < nt ? &xt[(size_t)s * IN] : &xv[(size_t)(s - nt) * IN];
+    std::copy(&xs_all[(size_t)src * IN], &xs_all[(size_t)src * IN] + IN, dx);
+    (s < nt ? yt[s] : yv[s - nt]) = y[src];
+  }
+  MlpBuffers b, bv;
   int r;
-  if ((r = alloc_mlp(c, b, n))) return r;
-  std::vector<float> P(NPARAM), xs;
+  if ((r = alloc_mlp(c, b, nt))) return r;
+  if (nv && (r = alloc_mlp(c, bv, nv))) return r;
+  std::vector<float> P(NPARAM), best(NPARAM);
   pack(m, P.data());
-  standardise(m, X, n, xs);
+  best = P;
   HIPCHK(c, hipMemcpyAsync(b.P, P.data(), NPARAM * 4, hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipMemsetAsync(b.M, 0, NPARAM * 4, c->stream));
   HIPCHK(c, hipMemsetAsync(b.V, 0, NPARAM * 4, c->stream));
-  HIPCHK(c, hipMemcpyAsync(b.X, xs.data(), (size_t)n * IN * 4, hipMemcpyHostToDevice, c->stream));
-  HIPCHK(c, hipMemcpyAsync(b.T, y, (size_t)n * 4, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(b.X, xt.data(), (size_t)nt * IN * 4, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(b.T, yt.data(), (size_t)nt * 4, hipMemcpyHostToDevice, c->stream));
+  if (nv) HIPCHK(c, hipMemcpyAsync(bv.X, xv.data(), (size_t)nv * IN * 4, hipMemcpyHostToDevice, c->stream));
+  double best_err = 1e300;
+  std::vector<float> pv(nv);
+  auto score = [&]() -> int {  // held-out mean squared error of the current parameters
+    hipLaunchKernelGGL(mlp_forward, dim3((nv + 255) / 256), dim3(256), 0, c->stream, b.P, bv.X, nv, bv.Z1, bv.Z2,
+                       bv.Y);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipMemcpyAsync(pv.data(), bv.Y, (size_t)nv * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    double err = 0;
+    for (uint32_t s = 0; s < nv; ++s) err += (double)(pv[s] - yv[s]) * (pv[s] - yv[s]);
+    if (err < best_err) {
+      best_err = err;
+      HIPCHK(c, hipMemcpyAsync(best.data(), b.P, NPARAM * 4, hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    return GMK_OK;
+  };
+  if (nv && (r = score())) return r;
   float b1t = 1.f, b2t = 1.f;
   for (uint32_t e = 0; e < epochs; ++e) {
-    if ((r = run_fb(c, b, n, true))) return r;
+    if ((r = run_fb(c, b, nt, true))) return r;
     b1t *= 0.9f;
     b2t *= 0.999f;
     hipLaunchKernelGGL(mlp_adam, dim3((NPARAM + 255) / 256), dim3(256), 0, c->stream, b.P, b.G, b.M, b.V, lr, b1t, b2t);
     HIPCHK(c, hipGetLastError());
+    if (nv && (e % 5 == 4 || e + 1 == epochs) && (r = score())) return r;
   }
-  if ((r = run_fb(c, b, n, false))) return r;
+  if (nv) HIPCHK(c, hipMemcpyAsync(b.P, best.data(), NPARAM * 4, hipMemcpyHostToDevice, c->stream));
+  // final loss over every sample
+  MlpBuffers ba;
+  if ((r = alloc_mlp(c, ba, n))) return r;
+  HIPCHK(c, hipMemcpyAsync(ba.X, xs_all.data(), (size_t)n * IN * 4, hipMemcpyHostToDevice, c->stream));
+  hipLaunchKernelGGL(mlp_forward, dim3((n + 255) / 256), dim3(256), 0, c->stream, b.P, ba.X, n, ba.Z1, ba.Z2, ba.Y);
+  HIPCHK(c, hipGetLastError());
   std::vector<float> yh(n);
   HIPCHK(c, hipMemcpyAsync(P.data(), b.P, NPARAM * 4, hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(c, hipMemcpyAsync(yh.data(), b.Y, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(yh.data(), ba.Y, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   unpack(P.data(), m);
   if (loss) {

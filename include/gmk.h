@@ -165,12 +165,14 @@ typedef struct {
 
 /* Features of a graph work item: log(1+nL), log(1+nR), log(1+|E|),
  * log(1+mean left degree), log(1+mean right degree), log(1+max degree),
- * density |E|/(nL nR), k. */
+ * log(1+paths of length two) (sum over vertices of C(deg, 2)), k. */
 int gmk_graph_features(const gmk_graph_desc* g, float out[GMK_MLP_IN]);
 
 /* Full-batch Adam on mean squared error; X is n x GMK_MLP_IN row-major (raw
- * features), y the targets.  Standardisation is fitted on X.  *loss receives
- * the final training loss. */
+ * features), y the targets.  Standardisation is fitted on X.  With n >= 20 a
+ * seeded fifth of the samples is held out and the parameters with the lowest
+ * held-out error (checked every 5 epochs) are returned (early stopping).
+ * *loss receives the mean squared error over all n samples. */
 int gmk_mlp_train(gmk_ctx* ctx, const float* X, const float* y, uint32_t n, uint32_t epochs, float lr, uint32_t seed,
                   gmk_mlp* model, float* loss);
 int gmk_mlp_predict(gmk_ctx* ctx, const gmk_mlp* model, const float* X, uint32_t n, float* y);
