@@ -1,0 +1,142 @@
+"""Autotuner for the branch-and-bound expansion launch parameters, per size bucket.
+
+The expansion kernel runs one wave per search state, either one workgroup per
+state or as a persistent grid of waves per compute unit that takes states from
+a device counter.  Which form, how many waves, and how many states go into one
+launch depend on the state's bitset width W (1..64 words): wide states hold
+more LDS per wave (fewer resident waves) and their searches branch wider.
+
+Each size bucket is a calibration batch of Erdos-Renyi graphs whose seed
+partitions have one bitset width (both sides near 64 W vertices, degree about
+sqrt(n), so a seed's 2-hop neighbourhood covers its side).  Every candidate
+setting runs the batch in batched mode; the fastest per bucket is kept for
+that width (the widest bucket also sets W = 64).  Every candidate must return
+the same per-graph counts, and the same fingerprints for graphs below the
+result cap: the tuning changes launches, not results.  The dedup table's load factor is tuned on the 50k x 30k workload.
+
+    python -m gmkbe_amd.autotune [--out gmkbe_amd/tune/mi355x.json]
+
+The table is versioned (gmk_tuning.version) and loaded by every Engine.
+"""
+from __future__ import annotations
+
+import argparse
+import datetime
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+from . import gen
+from .engine import TUNE_TABLE, TUNING_VERSION, Engine
+
+# (lane = log2 W, side low, side high)
+BUCKETS = [(0, 40, 60), (1, 90, 125), (2, 180, 250), (3, 360, 500), (4, 720, 1000), (5, 1450, 2000)]
+WAVES = (0, 4, 8, 16, 32)
+BATCH = (16384, 65536)
+DEDUP_LOADS = (300, 500, 700)
+
+
+def bucket_graphs(lo: int, hi: int, count: int, seed: int):
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        a, b = int(rng.integers(lo, hi + 1)), int(rng.integers(lo, hi + 1))
+        p = min(0.5, 1.2 / np.sqrt((a + b) / 2))
+        out.append((gen.er(a, b, p, seed * 1000 + i), 1, 3))
+    return out
+
+
+def _log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def tune(graphs_per_bucket: int = 24, cap: int = 1000, reps: int = 2) -> dict:
+    eng = Engine(0, tuning=None)
+    base = eng.get_tuning()
+    chosen_w, chosen_b = list(base["waves_per_cu"]), list(base["batch_states"])
+    trials = []
+    for lane, lo, hi in BUCKETS:
+        graphs = bucket_graphs(lo, hi, graphs_per_bucket, 7000 + lane)
+        ref = None
+        best = None
+        for w in WAVES:
+            for b in BATCH:
+                t = dict(base)
+                t["waves_per_cu"] = [w] * 7
+                t["batch_states"] = [b] * 7
+                eng.set_tuning(t)
+                eng.enumerate_batch(graphs[:4], limit=cap, mode="batched")  # warm-up
+                times = []
+                for _ in range(reps):
+                    t0 = time.perf_counter()
+                    per, tot = eng.enumerate_batch(graphs, limit=cap, mode="batched")
+                    times.append(time.perf_counter() - t0)
+                # counts always match; the sets of graphs below their cap must too
+                # (which results a capped graph keeps depends on the work order)
+                sig = [(p["count"],) + ((0, 0) if p["hit_cap"] else (p["fp_sum"], p["fp_xor"])) for p in per]
+                if ref is None:
+                    ref = sig
+                if sig != ref:
+                    raise RuntimeError(f"bucket W={1 << lane}: results changed with waves={w} batch={b}")
+                s = min(times)
+                trials.append({"lane": lane, "W": 1 << lane, "waves_per_cu": w, "batch_states": b, "s": s,
+                               "results": tot["count"], "states": tot["states"]})
+                _log(f"[autotune] W={1 << lane:2d} waves/CU={w:2d} batch={b:5d}: {s * 1e3:8.2f} ms "
+                     f"({tot['states']} states)")
+                if best is None or s < best[0]:
+                    best = (s, w, b)
+        chosen_w[lane], chosen_b[lane] = best[1], best[2]
+        _log(f"[autotune] W={1 << lane}: waves/CU={best[1]} batch={best[2]}")
+    chosen_w[6], chosen_b[6] = chosen_w[5], chosen_b[5]
+
+    # dedup table load factor on the 50k x 30k workload (seeded, batched, r = 10^6)
+    g = gen.workload("cl50kx30k")
+    eng.set_graph(g)
+    t = dict(base)
+    t["waves_per_cu"], t["batch_states"] = chosen_w, chosen_b
+    eng.set_tuning(t)
+    best_load = None
+    for load in DEDUP_LOADS:
+        times = []
+        for _ in range(reps + 1):
+            t0 = time.perf_counter()
+            r = eng.enumerate(k=1, theta=1, limit=1_000_000, mode="batched", partition="seeded", keep=False,
+                              dedup=True, dedup_load=load / 1000.0)
+            times.append(time.perf_counter() - t0)
+        s = min(times[1:])
+        trials.append({"dedup_load_permille": load, "s": s, "probes": r.stats["dedup_probes"],
+                       "inserts": r.stats["dedup_inserts"]})
+        _log(f"[autotune] dedup load {load / 1000:.1f}: {s * 1e3:.2f} ms, "
+             f"{r.stats['dedup_probes'] / max(1, r.stats['dedup_inserts']):.3f} probes/insert")
+        if best_load is None or s < best_load[0]:
+            best_load = (s, load)
+    eng.close()
+    return {
+        "version": TUNING_VERSION,
+        "target": "gfx950 (MI355X)",
+        "generated": datetime.datetime.now(datetime.timezone.utc).isoformat(timespec="seconds"),
+        "buckets": [{"lane": lane, "W": 1 << lane, "sides": [lo, hi]} for lane, lo, hi in BUCKETS],
+        "tuning": {"version": TUNING_VERSION, "waves_per_cu": chosen_w, "batch_states": chosen_b,
+                   "dedup_load_permille": best_load[1]},
+        "trials": trials,
+    }
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser(prog="python -m gmkbe_amd.autotune")
+    ap.add_argument("--out", default=TUNE_TABLE)
+    ap.add_argument("--graphs", type=int, default=24, help="graphs per size bucket")
+    a = ap.parse_args(argv)
+    table = tune(a.graphs)
+    os.makedirs(os.path.dirname(a.out), exist_ok=True)
+    with open(a.out, "w") as f:
+        json.dump(table, f, indent=1)
+    print(json.dumps(table["tuning"]))
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())

@@ -213,7 +213,8 @@ int BBEngine::step() {
     // more parents than the free stack can hold at that fan-out, so a search
     // that branches widely drifts toward depth-first instead of overflowing.
     const u64 by_room = (u64)((double)room / (2.0 * std::max(fanout_[l], 1.0)));
-    B[l] = (u32)std::max<u64>(1, std::min<u64>({top_[l], (u64)kBatchMax, by_room}));
+    const u64 bmax = std::min<u64>(kBatchMax, c_->tune.batch_states[l]);
+    B[l] = (u32)std::max<u64>(1, std::min<u64>({top_[l], bmax, by_room}));
     out_cap[l] = (u32)std::min<u64>(cap_[l] / 2, room);
     // A parent whose children do not fit is requeued; with the stack nearly full
     // only the top (deepest) states run, and the step fails below when none of
@@ -228,7 +229,8 @@ int BBEngine::step() {
     a.out_n = &dc->lane[l][0];
     a.retry_n = &dc->lane[l][1];
     a.work_next = &dc->lane[l][2];
-    HIPCHK(c_, launch_bb_expand(W, B[l], s, a, persistent_ ? c_->persistent_waves : 0u));
+    const u32 waves = persistent_ ? c_->n_cu * c_->tune.waves_per_cu[l] : 0u;
+    HIPCHK(c_, launch_bb_expand(W, B[l], s, a, waves));
     ++launched;
   }
   HIPCHK(c_, hipEventRecord(c_->ev1, s));

@@ -30,7 +30,7 @@ extern "C" int gmk_create(int device, gmk_ctx** out) {
   {
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
-      c->persistent_waves = (u32)prop.multiProcessorCount * 16;
+      c->n_cu = (u32)prop.multiProcessorCount;
   }
   if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
       hipHostMalloc((void**)&c->h_cnt, sizeof(RunCounters), hipHostMallocDefault) != hipSuccess ||
@@ -39,6 +39,29 @@ extern "C" int gmk_create(int device, gmk_ctx** out) {
     return GMK_EDEVICE;
   }
   *out = c;
+  return GMK_OK;
+}
+
+extern "C" int gmk_set_tuning(gmk_ctx* c, const gmk_tuning* t) {
+  if (!c || !t) return GMK_EINVAL;
+  if (t->version != GMK_TUNING_VERSION) { c->err = "tuning table version mismatch"; return GMK_EINVAL; }
+  for (int l = 0; l < 7; ++l) {
+    if (t->waves_per_cu[l] > 64 || t->batch_states[l] < 64 || t->batch_states[l] > 65536) {
+      c->err = "tuning value out of range";
+      return GMK_EINVAL;
+    }
+  }
+  if (t->dedup_load_permille < 100 || t->dedup_load_permille > 900) {
+    c->err = "dedup load out of range";
+    return GMK_EINVAL;
+  }
+  c->tune = *t;
+  return GMK_OK;
+}
+
+extern "C" int gmk_get_tuning(gmk_ctx* c, gmk_tuning* t) {
+  if (!c || !t) return GMK_EINVAL;
+  *t = c->tune;
   return GMK_OK;
 }
 
@@ -230,7 +253,7 @@ extern "C" int gmk_enumerate(gmk_ctx* c, const gmk_params* prm, gmk_stats* st) {
   c->res_words = 0;
   if (prm->dedup) {
     const u64 expect = prm->limit ? prm->limit : (1ull << 25);
-    const double lf = (prm->dedup_load_permille ? prm->dedup_load_permille : 500) / 1000.0;
+    const double lf = (prm->dedup_load_permille ? prm->dedup_load_permille : c->tune.dedup_load_permille) / 1000.0;
     u64 cap = 1024;
     while ((double)cap * lf < (double)expect) cap <<= 1;
     if ((r = ensure(c, c->dedup, cap * 8))) return r;

@@ -83,7 +83,9 @@ struct DeviceVec {
 
 struct gmk_ctx {
   int device = 0;
-  u32 persistent_waves = 256 * 16;  // waves of the persistent expansion kernel (16 per CU)
+  u32 n_cu = 256;                   // compute units of the device
+  gmk_tuning tune{GMK_TUNING_VERSION, {16, 16, 16, 16, 16, 16, 16},
+                  {65536, 65536, 65536, 65536, 65536, 65536, 65536}, 500};
   hipStream_t stream = nullptr;
   std::string err;
   // graph (host copy kept for partition construction)
@@ -161,7 +163,7 @@ class BBEngine {
   int drain();
 
  private:
-  static constexpr u32 kBatchMax = 1u << 16;
+  static constexpr u32 kBatchMax = 1u << 16;  // largest states per launch (retry buffer size)
   int lane_alloc(int l);
   int upload();
   void clear_stage();

@@ -9,6 +9,7 @@ is missing or no GPU is present -- there is no CPU path.
 from __future__ import annotations
 
 import ctypes
+import json
 import dataclasses
 import os
 
@@ -64,6 +65,33 @@ class GraphDesc(ctypes.Structure):
 class GraphStats(ctypes.Structure):
     _fields_ = [("count", ctypes.c_uint64), ("fp_sum", ctypes.c_uint64), ("fp_xor", ctypes.c_uint64),
                 ("ms", ctypes.c_double), ("hit_cap", ctypes.c_int32), ("pad", ctypes.c_int32)]
+
+
+class Tuning(ctypes.Structure):
+    """gmk_tuning: expansion launch parameters per bitset width (2^l words)."""
+    _fields_ = [("version", ctypes.c_uint32), ("waves_per_cu", ctypes.c_uint32 * 7),
+                ("batch_states", ctypes.c_uint32 * 7), ("dedup_load_permille", ctypes.c_uint32)]
+
+    def as_dict(self) -> dict:
+        return {"version": self.version, "waves_per_cu": list(self.waves_per_cu),
+                "batch_states": list(self.batch_states), "dedup_load_permille": self.dedup_load_permille}
+
+    @classmethod
+    def from_dict(cls, d: dict) -> "Tuning":
+        t = cls()
+        t.version = int(d.get("version", TUNING_VERSION))
+        for i, v in enumerate(d["waves_per_cu"]):
+            t.waves_per_cu[i] = int(v)
+        for i, v in enumerate(d["batch_states"]):
+            t.batch_states[i] = int(v)
+        t.dedup_load_permille = int(d.get("dedup_load_permille", 500))
+        return t
+
+
+TUNING_VERSION = 1
+# Table written by gmkbe_amd/autotune.py for this device family; loaded by
+# every Engine when present.
+TUNE_TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tune", "mi355x.json")
 
 
 class Mlp(ctypes.Structure):
@@ -137,6 +165,10 @@ def load_library():
     L.gmk_mlp_grad.restype = ctypes.c_int
     L.gmk_set_cost_model.argtypes = [ctypes.c_void_p, P(Mlp)]
     L.gmk_set_cost_model.restype = ctypes.c_int
+    L.gmk_set_tuning.argtypes = [ctypes.c_void_p, P(Tuning)]
+    L.gmk_set_tuning.restype = ctypes.c_int
+    L.gmk_get_tuning.argtypes = [ctypes.c_void_p, P(Tuning)]
+    L.gmk_get_tuning.restype = ctypes.c_int
     _lib = L
     return L
 
@@ -154,7 +186,10 @@ class Result:
 class Engine:
     """One device context (gmk_create).  Not thread-safe; one run at a time."""
 
-    def __init__(self, device: int = 0):
+    def __init__(self, device: int = 0, tuning: str | dict | None = TUNE_TABLE):
+        """tuning: a tuning table (path or dict from gmkbe_amd/autotune.py);
+        the committed table for this device family by default, library
+        defaults when it is absent or None."""
         self._L = load_library()
         h = ctypes.c_void_p()
         rc = self._L.gmk_create(device, ctypes.byref(h))
@@ -162,6 +197,21 @@ class Engine:
             raise GmkError(f"gmk_create(device={device}) failed: {STATUS.get(rc, rc)}")
         self._h = h
         self.graph: Graph | None = None
+        if isinstance(tuning, str):
+            if os.path.exists(tuning):
+                with open(tuning) as f:
+                    self.set_tuning(json.load(f)["tuning"])
+        elif tuning is not None:
+            self.set_tuning(tuning)
+
+    def set_tuning(self, t) -> None:
+        t = t if isinstance(t, Tuning) else Tuning.from_dict(t)
+        self._check(self._L.gmk_set_tuning(self._h, ctypes.byref(t)), "gmk_set_tuning")
+
+    def get_tuning(self) -> dict:
+        t = Tuning()
+        self._check(self._L.gmk_get_tuning(self._h, ctypes.byref(t)), "gmk_get_tuning")
+        return t.as_dict()
 
     def close(self) -> None:
         if getattr(self, "_h", None):
@@ -200,14 +250,16 @@ class Engine:
 
     def enumerate(self, k: int = 1, theta: int = 1, limit: int = 0, mode: str = "plain",
                   partition: str = "auto", keep: bool = True, capacity_words: int = 0, dedup: bool = False,
-                  dedup_load: float = 0.5, verbose: bool = False) -> Result:
-        """verbose: the library prints a progress line to stderr every 10 s."""
+                  dedup_load: float = 0.5, verbose: bool = False, fetch: bool = True) -> Result:
+        """keep: write result records (in device memory); fetch: copy them to
+        the host (Result.words, None otherwise).  verbose: the library prints a
+        progress line to stderr every 10 s."""
         prm = Params(k, theta, limit, MODES[mode], PARTS[partition], int(keep), int(verbose), capacity_words,
                      int(dedup), int(round(dedup_load * 1000)))
         st = Stats()
         self._check(self._L.gmk_enumerate(self._h, ctypes.byref(prm), ctypes.byref(st)), "gmk_enumerate")
         words = None
-        if keep:
+        if keep and fetch:
             nw = ctypes.c_uint64()
             self._check(self._L.gmk_fetch_results(self._h, None, 0, ctypes.byref(nw)), "gmk_fetch_results")
             words = np.empty(int(nw.value), dtype=np.uint32)

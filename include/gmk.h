@@ -183,6 +183,22 @@ int gmk_mlp_grad(gmk_ctx* ctx, const gmk_mlp* model, const float* X, const float
 /* Cost model used by GMK_MODE_LEARNED (null: the size heuristic). */
 int gmk_set_cost_model(gmk_ctx* ctx, const gmk_mlp* model);
 
+/* ---- Tuning (no counterpart in the reference) -----------------------------
+ * Launch parameters of the branch-and-bound expansion per bitset width
+ * (index l: 2^l words, l = 0..6), as chosen per size bucket by the autotuner
+ * (gmkbe_amd/autotune.py, table under gmkbe_amd/tune/).  Results do not depend
+ * on them.  Defaults: 16 persistent waves per CU, 65536 states per launch,
+ * dedup load 0.5. */
+#define GMK_TUNING_VERSION 1
+typedef struct {
+  uint32_t version;                 /* GMK_TUNING_VERSION */
+  uint32_t waves_per_cu[7];         /* batched modes: persistent waves per CU; 0 = one workgroup per state */
+  uint32_t batch_states[7];         /* states per expansion launch, 64 .. 65536 */
+  uint32_t dedup_load_permille;     /* dedup table load when gmk_params leaves it 0 (100 .. 900) */
+} gmk_tuning;
+int gmk_set_tuning(gmk_ctx* ctx, const gmk_tuning* tuning);
+int gmk_get_tuning(gmk_ctx* ctx, gmk_tuning* tuning);
+
 #ifdef __cplusplus
 }
 #endif
