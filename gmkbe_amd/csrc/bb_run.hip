@@ -25,6 +25,19 @@ using namespace gmk;
 
 namespace gmk {
 hipError_t launch_bb_expand(int W, u32 n_states, hipStream_t s, const ExpandArgs& a, u32 persistent_waves);
+struct RowBuildArgs {
+  const PartDesc* parts;
+  const u32* pids;
+  u32 n;
+  const u32* maps;
+  const u32* off;
+  const u32* adj;
+  const u64* off_base;
+  const u64* adj_base;
+  u64* rows;
+  u64* bytes;
+};
+hipError_t launch_build_part_rows(const RowBuildArgs& a, hipStream_t s);
 
 namespace {
 constexpr int NLANE = 7;  // widths 1, 2, 4, 8, 16, 32, 64 words
@@ -50,8 +63,15 @@ int BBEngine::reserve(size_t rows_words, size_t maps_words, size_t nparts) {
 }
 
 bool BBEngine::fits(const BuiltPart& b) const {
-  return rows_used_ + b.rows.size() <= rows_cap_ && maps_used_ + b.maps.size() <= maps_cap_ &&
+  return rows_used_ + b.row_words() <= rows_cap_ && maps_used_ + b.maps.size() <= maps_cap_ &&
          parts_used_ + 1 <= parts_cap_;
+}
+
+void BBEngine::set_row_source(const u32* off, const u32* adj, const u64* off_base, const u64* adj_base) {
+  src_off_ = off;
+  src_adj_ = adj;
+  src_off_base_ = off_base;
+  src_adj_base_ = adj_base;
 }
 
 void BBEngine::add(BuiltPart& b) {
@@ -61,10 +81,18 @@ void BBEngine::add(BuiltPart& b) {
   P.rows_r += rows_used_;
   P.map_l += maps_used_;
   P.map_r += maps_used_;
-  rows_stage_.insert(rows_stage_.end(), b.rows.begin(), b.rows.end());
+  if (b.device_rows) {
+    // the row range is reserved here and filled on the device before any state runs
+    rowbuild_.push_back(pid);
+  } else {
+    // the stage covers [rows_dev_, rows_used_): zeros over device-built ranges
+    // (the build kernel runs after the copy)
+    rows_stage_.resize(rows_used_ - rows_dev_, 0);
+    rows_stage_.insert(rows_stage_.end(), b.rows.begin(), b.rows.end());
+  }
   maps_stage_.insert(maps_stage_.end(), b.maps.begin(), b.maps.end());
   parts_stage_.push_back(P);
-  rows_used_ += b.rows.size();
+  rows_used_ += b.row_words();
   maps_used_ += b.maps.size();
   ++parts_used_;
   std::vector<u64>& rt = roots_stage_[lane_of(b.W)];
@@ -88,6 +116,7 @@ int BBEngine::add_set(const PartitionSet& ps) {
     P.map_r += maps_used_;
     parts_stage_.push_back(P);
   }
+  rows_stage_.resize(rows_used_ - rows_dev_, 0);
   rows_stage_.insert(rows_stage_.end(), ps.rows.begin(), ps.rows.end());
   maps_stage_.insert(maps_stage_.end(), ps.maps.begin(), ps.maps.end());
   rows_used_ += ps.rows.size();
@@ -135,6 +164,9 @@ int BBEngine::lane_alloc(int l) {
 // first-fed root on top, expanded first).
 int BBEngine::upload() {
   hipStream_t s = c_->stream;
+  // Host-built rows are staged contiguously from rows_dev_; device-built rows
+  // only reserve their range (rows_stage_ then stays empty, or holds zeros in
+  // those ranges when host rows come after them).
   if (!rows_stage_.empty())
     HIPCHK(c_, hipMemcpyAsync((u64*)c_->rows.p + rows_dev_, rows_stage_.data(), rows_stage_.size() * 8,
                               hipMemcpyHostToDevice, s));
@@ -144,9 +176,23 @@ int BBEngine::upload() {
   if (!parts_stage_.empty())
     HIPCHK(c_, hipMemcpyAsync((PartDesc*)c_->parts.p + parts_dev_, parts_stage_.data(),
                               parts_stage_.size() * sizeof(PartDesc), hipMemcpyHostToDevice, s));
-  rows_dev_ += rows_stage_.size();
+  rows_dev_ = rows_used_;
   maps_dev_ += maps_stage_.size();
   parts_dev_ += parts_stage_.size();
+  if (!rowbuild_.empty()) {
+    if (!src_off_) { c_->err = "no graph for device-built partition rows"; return GMK_EINVAL; }
+    int r;
+    if ((r = ensure(c_, c_->rowpids, rowbuild_.size() * 4))) return r;
+    HIPCHK(c_, hipMemcpyAsync(c_->rowpids.p, rowbuild_.data(), rowbuild_.size() * 4, hipMemcpyHostToDevice, s));
+    RunCounters* dc = (RunCounters*)c_->counters.p;
+    RowBuildArgs ra{(const PartDesc*)c_->parts.p, (const u32*)c_->rowpids.p, (u32)rowbuild_.size(),
+                    (const u32*)c_->maps.p, src_off_, src_adj_, src_off_base_, src_adj_base_, (u64*)c_->rows.p,
+                    &dc->stat[3 + 3]};
+    HIPCHK(c_, hipEventRecord(c_->ev2, s));
+    HIPCHK(c_, launch_build_part_rows(ra, s));
+    HIPCHK(c_, hipEventRecord(c_->ev3, s));
+    rows_built_ = true;
+  }
   for (int l = 0; l < NLANE; ++l) {
     std::vector<u64>& rt = roots_stage_[l];
     if (rt.empty()) continue;
@@ -170,6 +216,7 @@ int BBEngine::upload() {
 
 // Host staging buffers are released after the stream has synchronised.
 void BBEngine::clear_stage() {
+  rowbuild_.clear();
   rows_stage_.clear();
   maps_stage_.clear();
   parts_stage_.clear();
@@ -238,6 +285,13 @@ int BBEngine::step() {
   HIPCHK(c_, hipStreamSynchronize(s));
   clear_stage();
   float ms = 0;
+  if (rows_built_) {
+    HIPCHK(c_, hipEventElapsedTime(&ms, c_->ev2, c_->ev3));
+    rs_.ms_kernel[3] += ms;
+    rs_.launches_kernel[3] += 1;
+    rs_.launches += 1;
+    rows_built_ = false;
+  }
   HIPCHK(c_, hipEventElapsedTime(&ms, c_->ev0, c_->ev1));
   rs_.ms_expand += ms;
   rs_.ms_kernel[0] += ms;
@@ -290,9 +344,13 @@ int BBEngine::drain() {
 // Partition construction off the main thread.
 // ---------------------------------------------------------------------------
 
-// Rows, maps and root state of a seed partition, with offsets relative to the
-// part's own vectors.  Returns false when a side exceeds 4096 vertices.
-static bool make_built(const SeedJob& job, const SeedPartition& sp, std::vector<u32>& loc, BuiltPart& b) {
+static void build_host_rows(const HostGraph& g, const SeedPartition& sp, std::vector<u32>& loc, BuiltPart& b);
+
+// Maps and root state of a seed partition (and its rows unless the device
+// builds them), with offsets relative to the part's own vectors.  Returns
+// false when a side exceeds 4096 vertices.
+static bool make_built(const SeedJob& job, const SeedPartition& sp, std::vector<u32>& loc, BuiltPart& b,
+                       bool device_rows) {
   const HostGraph& g = job.g;
   const u32 nl = (u32)sp.left.size(), nr = (u32)sp.right.size();
   const u32 wl = (nl + 63) / 64, wr = (nr + 63) / 64;
@@ -313,6 +371,25 @@ static bool make_built(const SeedJob& job, const SeedPartition& sp, std::vector<
   b.maps.resize((size_t)nl + nr);
   std::copy(sp.left.begin(), sp.left.end(), b.maps.begin());
   std::copy(sp.right.begin(), sp.right.end(), b.maps.begin() + nl);
+  b.device_rows = device_rows;
+  b.rows.clear();
+  if (!device_rows) build_host_rows(g, sp, loc, b);
+  // root: S = {seed}, C / X from the flags (0 = C, 1 = X, 2 = S)
+  b.root.assign(6 * (size_t)W, 0);
+  for (int side = 0; side < 2; ++side) {
+    const std::vector<uint8_t>& fl = side ? sp.rflag : sp.lflag;
+    for (u32 i = 0; i < fl.size(); ++i) {
+      const int which = fl[i] == 2 ? 0 : (fl[i] == 0 ? 1 : 2);
+      b.root[(side * 3 + which) * W + i / 64] |= 1ull << (i % 64);
+    }
+  }
+  return true;
+}
+
+// Host copy of the rows (the whole-graph partition and tests use it).
+static void build_host_rows(const HostGraph& g, const SeedPartition& sp, std::vector<u32>& loc, BuiltPart& b) {
+  const PartDesc& P = b.P;
+  const u32 nl = P.nl, nr = P.nr, wl = P.wl, wr = P.wr;
   for (u32 i = 0; i < nl; ++i) loc[sp.left[i]] = i;
   for (u32 i = 0; i < nr; ++i) loc[sp.right[i]] = i;
   b.rows.assign((size_t)nl * wr + (size_t)nr * wl, 0);
@@ -334,16 +411,6 @@ static bool make_built(const SeedJob& job, const SeedPartition& sp, std::vector<
   }
   for (u32 v : sp.left) loc[v] = NONE;
   for (u32 v : sp.right) loc[v] = NONE;
-  // root: S = {seed}, C / X from the flags (0 = C, 1 = X, 2 = S)
-  b.root.assign(6 * (size_t)W, 0);
-  for (int side = 0; side < 2; ++side) {
-    const std::vector<uint8_t>& fl = side ? sp.rflag : sp.lflag;
-    for (u32 i = 0; i < fl.size(); ++i) {
-      const int which = fl[i] == 2 ? 0 : (fl[i] == 0 ? 1 : 2);
-      b.root[(side * 3 + which) * W + i / 64] |= 1ull << (i % 64);
-    }
-  }
-  return true;
 }
 
 namespace {
@@ -365,7 +432,7 @@ struct Worker {
 
 // Builds the partitions of tasks[i] into outs[i] on up to nthreads threads.
 void build_tasks(std::vector<SeedJob>& jobs, const std::vector<BuildTask>& tasks, std::vector<BuildOut>& outs,
-                 std::vector<Worker>& workers) {
+                 std::vector<Worker>& workers, bool device_rows) {
   outs.clear();
   outs.resize(tasks.size());
   std::atomic<size_t> next{0};
@@ -375,7 +442,7 @@ void build_tasks(std::vector<SeedJob>& jobs, const std::vector<BuildTask>& tasks
       if (i >= tasks.size()) break;
       SeedJob& job = jobs[tasks[i].job];
       if (!build_seed_partition(job.g, job.order, tasks[i].seed, job.k, job.lb, w.sp, w.cnt, w.mark)) continue;
-      if (make_built(job, w.sp, w.loc, outs[i].part)) outs[i].ok = true;
+      if (make_built(job, w.sp, w.loc, outs[i].part, device_rows)) outs[i].ok = true;
       else outs[i].too_wide = true;
     }
   };
@@ -414,6 +481,36 @@ int run_seed_jobs(gmk_ctx* c, std::vector<SeedJob>& jobs, const gmk_params* prm,
     w.mark.assign(nmax, 0);
     w.loc.assign(nmax, NONE);
   }
+  // CSR of every job's graph for the device row builder: the context's own
+  // graph, or the batch's graphs concatenated
+  std::vector<u64> base(2 * (size_t)ngraphs, 0);
+  std::vector<u32> coff, cadj;
+  bool ctx_graph = true;
+  for (auto& j : jobs) ctx_graph = ctx_graph && j.g.off == c->off.data() && j.graph == 0;
+  if (ctx_graph) {
+    if ((r = ensure(c, c->bcsr_base, 16))) return r;
+    HIPCHK(c, hipMemcpyAsync(c->bcsr_base.p, base.data(), 16, hipMemcpyHostToDevice, c->stream));
+    eng.set_row_source(c->d_off, c->d_adj, (const u64*)c->bcsr_base.p, (const u64*)c->bcsr_base.p + 1);
+  } else {
+    std::vector<char> seen(ngraphs, 0);
+    for (auto& j : jobs) {
+      if (seen[j.graph]) continue;
+      seen[j.graph] = 1;
+      base[j.graph] = coff.size();
+      base[ngraphs + j.graph] = cadj.size();
+      coff.insert(coff.end(), j.g.off, j.g.off + j.g.n + 1);
+      cadj.insert(cadj.end(), j.g.adj, j.g.adj + j.g.off[j.g.n]);
+    }
+    if ((r = ensure(c, c->bcsr_off, std::max<size_t>(4, coff.size() * 4)))) return r;
+    if ((r = ensure(c, c->bcsr_adj, std::max<size_t>(4, cadj.size() * 4)))) return r;
+    if ((r = ensure(c, c->bcsr_base, base.size() * 8))) return r;
+    HIPCHK(c, hipMemcpyAsync(c->bcsr_off.p, coff.data(), coff.size() * 4, hipMemcpyHostToDevice, c->stream));
+    if (!cadj.empty())
+      HIPCHK(c, hipMemcpyAsync(c->bcsr_adj.p, cadj.data(), cadj.size() * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->bcsr_base.p, base.data(), base.size() * 8, hipMemcpyHostToDevice, c->stream));
+    eng.set_row_source((const u32*)c->bcsr_off.p, (const u32*)c->bcsr_adj.p, (const u64*)c->bcsr_base.p,
+                       (const u64*)c->bcsr_base.p + ngraphs);
+  }
   std::vector<u64> gcnt_h(ngraphs, 0);
   auto capped = [&](const SeedJob& j) { return rs.gcount && gcnt_h[j.graph] >= rs.gcap; };
   auto done = [&](const SeedJob& j) { return j.next >= j.seeds.size() || capped(j); };
@@ -430,7 +527,7 @@ int run_seed_jobs(gmk_ctx* c, std::vector<SeedJob>& jobs, const gmk_params* prm,
       const double t0 = rs.ms_kernel[0];
       while (!done(j) && !rs.hit_limit) {
         tasks.assign(1, BuildTask{(u32)ji, j.seeds[j.next++]});
-        build_tasks(jobs, tasks, outs, workers);
+        build_tasks(jobs, tasks, outs, workers, true);
         if (outs[0].too_wide) return too_wide(tasks[0].seed);
         if (!outs[0].ok) continue;
         ++st->partitions;
@@ -472,7 +569,7 @@ int run_seed_jobs(gmk_ctx* c, std::vector<SeedJob>& jobs, const gmk_params* prm,
         for (size_t q = 0; q < take; ++q) tasks.push_back(BuildTask{(u32)ji, j.seeds[j.next++]});
         quota[ji] = std::min<u32>(quota[ji] * 2, 1u << 16);
       }
-      if (!tasks.empty()) build_tasks(jobs, tasks, outs, workers);
+      if (!tasks.empty()) build_tasks(jobs, tasks, outs, workers, true);
       for (size_t i = 0; i < tasks.size(); ++i) {
         BuildOut& o = outs[i];
         if (o.too_wide) return too_wide(tasks[i].seed);

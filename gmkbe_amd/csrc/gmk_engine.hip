@@ -34,7 +34,8 @@ extern "C" int gmk_create(int device, gmk_ctx** out) {
   }
   if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
       hipHostMalloc((void**)&c->h_cnt, sizeof(RunCounters), hipHostMallocDefault) != hipSuccess ||
-      hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+      hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
+      hipEventCreate(&c->ev2) != hipSuccess || hipEventCreate(&c->ev3) != hipSuccess) {
     delete c;
     return GMK_EDEVICE;
   }
@@ -70,7 +71,8 @@ extern "C" void gmk_destroy(gmk_ctx* c) {
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
   for (DeviceVec* v : {&c->rows, &c->maps, &c->parts, &c->res, &c->counters, &c->tuples, &c->arena, &c->fams,
-                        &c->prefix, &c->tcnt, &c->dedup, &c->gcnt})
+                        &c->prefix, &c->tcnt, &c->dedup, &c->gcnt, &c->bcsr_off, &c->bcsr_adj, &c->bcsr_base,
+                        &c->rowpids})
     if (v->p) (void)hipFree(v->p);
   for (int l = 0; l < 7; ++l)
     for (DeviceVec* v : {&c->lane_stack[l], &c->lane_next[l], &c->lane_retry[l]})
@@ -80,6 +82,8 @@ extern "C" void gmk_destroy(gmk_ctx* c) {
   if (c->h_cnt) (void)hipHostFree(c->h_cnt);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->ev2) (void)hipEventDestroy(c->ev2);
+  if (c->ev3) (void)hipEventDestroy(c->ev3);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }

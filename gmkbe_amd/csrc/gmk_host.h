@@ -103,7 +103,8 @@ struct gmk_ctx {
   bool order_valid = false;
   RunCounters* h_cnt = nullptr;  // pinned
   u64 res_words = 0;             // record stream length of the last run
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
+  DeviceVec bcsr_off, bcsr_adj, bcsr_base, rowpids;  // CSRs of a batch's graphs; partitions whose rows are built
   std::vector<u64> prog_count;   // progress log of the last run
   std::vector<double> prog_ms;
   bool has_model = false;        // cost model for GMK_MODE_LEARNED
@@ -142,9 +143,11 @@ namespace gmk {
 struct BuiltPart {
   int W = 0;
   PartDesc P{};
-  std::vector<u64> rows;  // nl rows of wr words, then nr rows of wl words
-  std::vector<u32> maps;  // left IDs, then right IDs
-  std::vector<u64> root;  // root state without its partition word (6W words)
+  bool device_rows = false;  // rows are built on the device from the graph's CSR
+  std::vector<u64> rows;     // otherwise: nl rows of wr words, then nr rows of wl words
+  std::vector<u32> maps;     // left IDs, then right IDs
+  std::vector<u64> root;     // root state without its partition word (6W words)
+  u64 row_words() const { return device_rows ? (u64)P.nl * P.wr + (u64)P.nr * P.wl : rows.size(); }
 };
 
 // Branch-and-bound over partitions of mixed widths: one device state stack per
@@ -159,6 +162,9 @@ class BBEngine {
   int add_set(const PartitionSet& ps);
   void reset_tables();  // only with no live state
   u64 live() const;     // states on the stacks plus staged roots
+  // CSR of every graph a device-built partition may come from (device
+  // pointers; per graph the start of its offsets and lists)
+  void set_row_source(const u32* off, const u32* adj, const u64* off_base, const u64* adj_base);
   int step();
   int drain();
 
@@ -178,6 +184,10 @@ class BBEngine {
   std::vector<u32> maps_stage_;
   std::vector<PartDesc> parts_stage_;
   std::vector<u64> roots_stage_[7], rev_stage_[7];
+  std::vector<u32> rowbuild_;  // staged partitions whose rows the device builds
+  bool rows_built_ = false;    // a row build was launched in this step
+  const u32 *src_off_ = nullptr, *src_adj_ = nullptr;
+  const u64 *src_off_base_ = nullptr, *src_adj_base_ = nullptr;
   u64 staged_roots_ = 0;
   u64 top_[7] = {0, 0, 0, 0, 0, 0, 0};
   u64 cap_[7] = {0, 0, 0, 0, 0, 0, 0};
