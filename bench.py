@@ -191,17 +191,28 @@ def cpu_baseline(g, k, theta, sample):
 
 
 def run_steps(eng, cfg, mode, partition, steps, warmup, sync):
+    """Timed steps write every result record to device memory; the records are
+    not copied to the host inside the timed region."""
     for _ in range(warmup):
-        eng.enumerate(k=cfg["k"], theta=cfg["theta"], limit=cfg["limit"], mode=mode, partition=partition, keep=True)
+        eng.enumerate(k=cfg["k"], theta=cfg["theta"], limit=cfg["limit"], mode=mode, partition=partition, keep=True,
+                      fetch=False)
     sync()
     t0 = time.perf_counter()
     last = None
     for _ in range(steps):
         last = eng.enumerate(k=cfg["k"], theta=cfg["theta"], limit=cfg["limit"], mode=mode, partition=partition,
-                             keep=True)
+                             keep=True, fetch=False)
     sync()
     dt = time.perf_counter() - t0
     return dt, last
+
+
+def with_fetch(eng, cfg, mode, partition):
+    """One step that also copies the records to the host (PCIe included)."""
+    t0 = time.perf_counter()
+    r = eng.enumerate(k=cfg["k"], theta=cfg["theta"], limit=cfg["limit"], mode=mode, partition=partition, keep=True)
+    dt = time.perf_counter() - t0
+    return {"value": r.count / dt, "ms_per_step": dt * 1e3, "record_words": int(r.words.size)}
 
 
 def main() -> int:
@@ -213,6 +224,7 @@ def main() -> int:
     ap.add_argument("--partition", default="seeded", choices=["auto", "global", "seeded"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-mode run")
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain-mode run (profiling the batched kernels)")
     ap.add_argument("--batch-sample", type=int, default=100, help="er_batch: graphs in the speedup comparison")
     ap.add_argument("--train-graphs", type=int, default=200, help="er_batch: graphs that train the cost model")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -253,12 +265,16 @@ def main() -> int:
     eng.set_graph(g)
     # plain: per-partition (per-seed) launches, the reference's way of working;
     # batched (the default mode): full launches, persistent expansion kernel
-    log("[bench] plain mode")
-    dtp, lastp = run_steps(eng, cfg, "plain", args.partition, args.steps, args.warmup, sync)
-    vp = args.steps * lastp.stats["count"] / dtp
-    plain = {"value": vp, "ms_per_step": dtp / args.steps * 1e3, "count": lastp.stats["count"]}
-    log(f"[bench] plain: {lastp.stats['count']} results/step, {dtp / args.steps * 1e3:.2f} ms/step, {vp:.3e} biplexes/s")
-    mode, dt, last = "plain", dtp, lastp
+    plain, vp = None, None
+    mode, dt, last = None, None, None
+    if not args.no_plain or args.no_batched:
+        log("[bench] plain mode")
+        dtp, lastp = run_steps(eng, cfg, "plain", args.partition, args.steps, args.warmup, sync)
+        vp = args.steps * lastp.stats["count"] / dtp
+        plain = {"value": vp, "ms_per_step": dtp / args.steps * 1e3, "count": lastp.stats["count"]}
+        log(f"[bench] plain: {lastp.stats['count']} results/step, {dtp / args.steps * 1e3:.2f} ms/step, "
+            f"{vp:.3e} biplexes/s")
+        mode, dt, last = "plain", dtp, lastp
     if not args.no_batched:
         log("[bench] batched mode")
         dt, last = run_steps(eng, cfg, "batched", args.partition, args.steps, args.warmup, sync)
@@ -266,9 +282,12 @@ def main() -> int:
     st = last.stats
     count = st["count"]
     value = args.steps * count / dt
-    plain["speedup_of_batched"] = value / vp
-    log(f"[bench] {mode}: {count} results/step, {dt / args.steps * 1e3:.2f} ms/step, {value:.3e} biplexes/s "
-        f"({value / vp:.2f}x plain)")
+    if plain:
+        plain["speedup_of_batched"] = value / vp
+    log(f"[bench] {mode}: {count} results/step, {dt / args.steps * 1e3:.2f} ms/step, {value:.3e} biplexes/s"
+        + (f" ({value / vp:.2f}x plain)" if plain else ""))
+    pcie = with_fetch(eng, cfg, mode, args.partition)
+    log(f"[bench] {mode} with the records copied to the host: {pcie['value']:.3e} biplexes/s")
     eng.close()
 
     # dominant kernel of the measured run
@@ -310,6 +329,7 @@ def main() -> int:
         "kernels": {names[q]: {"ms": ms_k[q], "launches": st["launches_kernel"][q], "algo_bytes": st["bytes_kernel"][q]}
                     for q in range(4)},
         "plain": plain,
+        "with_records_to_host": pcie,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
