@@ -146,7 +146,24 @@ struct Family {
   u32 psize[MAXT];
   u64 pool[MAXT];     // address of the pool, FARP for the far pool
   u64 total;          // number of concrete R (product of binomials), saturating
+  u32 fpick[4];       // min(k + 1, nfree) free vertices of least degree
+  u32 nfpick;
 };
+
+// The min(k + 1, nfree) free vertices of least degree (ties: first in the list).
+GMK_HD u32 free_picks(const u32* off, const u32* freel, u32 nfree, u32 k, u32* pick, Cost& cost) {
+  u32 pdeg[4];
+  u32 np = 0;
+  for (u32 i = 0; i < nfree; ++i) {
+    const u32 y = freel[i], d = off[y + 1] - off[y];
+    if (np < k + 1) { pick[np] = y; pdeg[np] = d; ++np; continue; }
+    u32 worst = 0;
+    for (u32 q = 1; q < np; ++q) if (pdeg[q] > pdeg[worst]) worst = q;
+    if (d < pdeg[worst]) { pick[worst] = y; pdeg[worst] = d; }
+  }
+  cost.bytes += 12ull * nfree;
+  return np;
+}
 
 // The y of index i in pool t of family f.
 GMK_HD u32 pool_member(const Family& f, u32 t, u64 i, u32 b_lo, Cost& cost) {
@@ -173,6 +190,8 @@ GMK_HD bool in_tuple(const u32* L, u32 j, u32 w) {
 // Does w miss at most k vertices of R, and is w adjacent to every saturated one?
 // `known` is a member of R already known to be adjacent to w (the vertex whose
 // list produced w as a candidate), or NONE.  Members are searched in N(w).
+// NMAX bounds nch at compile time so the chosen vertices stay in registers.
+template <int NMAX>
 GMK_HD bool addable(const Graph& g, u32 w, const u32* freel, u32 nfree, const u32* ch, const u32* ctsz, u32 nch,
                     u32 k, u32 known, Cost& cost) {
   const u32 ow = g.off[w], dw = g.off[w + 1] - ow;
@@ -181,14 +200,13 @@ GMK_HD bool addable(const Graph& g, u32 w, const u32* freel, u32 nfree, const u3
   const u32 rs = nfree + nch;
   if (dw + k < rs) return false;
   u32 miss = 0;
-  for (u32 i = 0; i < nch; ++i) {
-    const u32 y = ch[i];
-    if (y == known) continue;
-    if (!contains(nw, dw, y, cost)) {
-      if (ctsz[i] >= k) return false;  // saturated member missed
-      if (++miss > k) return false;
-    }
+  bool out = false;  // a saturated member is missed, or more than k members
+#pragma unroll
+  for (int i = 0; i < NMAX; ++i) {
+    if ((u32)i < nch && !out && ch[i] != known && !contains(nw, dw, ch[i], cost))
+      out = ctsz[i] >= k || ++miss > k;
   }
+  if (out) return false;
   // free vertices have type 0 (< k for k >= 1): only their number matters
   if (nfree <= dw) {
     for (u32 i = 0; i < nfree; ++i)
@@ -205,24 +223,64 @@ GMK_HD bool addable(const Graph& g, u32 w, const u32* freel, u32 nfree, const u3
   return true;
 }
 
-// Returns true iff (L, R) is maximal on side A.
-GMK_HD bool side_a_maximal(const Graph& g, const u32* L, u32 j, const u32* freel, u32 nfree, const u32* ch,
-                           const u32* ctsz, u32 nch, u32 k, Cost& cost) {
+// Returns true iff (L, R) is maximal on side A.  fpick holds min(k + 1, nfree)
+// free vertices (free_picks): with no saturated member, an addable w misses at
+// most k members of R, so it is adjacent to one of any k + 1 members; the k + 1
+// members of least degree among the chosen vertices and fpick give the
+// candidates.
+template <int NMAX>
+GMK_HD bool side_a_maximal(const Graph& g, const u32* L, u32 j, const u32* freel, u32 nfree, const u32* fpick,
+                           u32 nfpick, const u32* ch, const u32* ctsz, u32 nch, u32 k, Cost& cost) {
   const u32 rs = nfree + nch;
   // saturated chosen vertex with the smallest degree
   u32 best = 0xFFFFFFFFu, bestdeg = 0xFFFFFFFFu;
-  for (u32 i = 0; i < nch; ++i)
+#pragma unroll
+  for (int i = 0; i < NMAX; ++i) {
+    if ((u32)i >= nch) break;
     if (ctsz[i] >= k) {
       const u32 d = g.off[ch[i] + 1] - g.off[ch[i]];
       cost.bytes += 8;
       if (d < bestdeg) { bestdeg = d; best = ch[i]; }
     }
+  }
   if (best != 0xFFFFFFFFu) {
+    // An addable w is adjacent to `best` (saturated) and to all but at most k
+    // members of R, so it is a neighbour of one of any k + 1 other members.
+    // When the k + 1 other members of least degree have fewer neighbours in
+    // all than `best`, their lists give the candidates (kept if adjacent to
+    // `best`); otherwise N(best) does.
+    u32 pk[4], pd[4];
+    u32 np = 0;
+    auto take = [&](u32 y) {
+      const u32 d = g.off[y + 1] - g.off[y];
+      cost.bytes += 8;
+      if (np < k + 1) { pk[np] = y; pd[np] = d; ++np; return; }
+      u32 worst = 0;
+      for (u32 q = 1; q < np; ++q) if (pd[q] > pd[worst]) worst = q;
+      if (d < pd[worst]) { pk[worst] = y; pd[worst] = d; }
+    };
+#pragma unroll
+    for (int i = 0; i < NMAX; ++i)
+      if ((u32)i < nch && ch[i] != best) take(ch[i]);
+    for (u32 i = 0; i < nfpick; ++i) take(fpick[i]);
+    u64 psum = 0;
+    for (u32 q = 0; q < np; ++q) psum += pd[q];
+    if (np == k + 1 && psum < bestdeg) {
+      const u32* nb = g.adj + g.off[best];
+      for (u32 q = 0; q < np; ++q)
+        for (u32 e = g.off[pk[q]]; e < g.off[pk[q] + 1]; ++e) {
+          const u32 w = g.adj[e];
+          cost.bytes += 4;
+          if (in_tuple(L, j, w) || !contains(nb, bestdeg, w, cost)) continue;
+          if (addable<NMAX>(g, w, freel, nfree, ch, ctsz, nch, k, pk[q], cost)) return false;
+        }
+      return true;
+    }
     for (u32 e = g.off[best]; e < g.off[best + 1]; ++e) {
       const u32 w = g.adj[e];
       cost.bytes += 4;
       if (in_tuple(L, j, w)) continue;
-      if (addable(g, w, freel, nfree, ch, ctsz, nch, k, best, cost)) return false;
+      if (addable<NMAX>(g, w, freel, nfree, ch, ctsz, nch, k, best, cost)) return false;
     }
     return true;
   }
@@ -244,15 +302,18 @@ GMK_HD bool side_a_maximal(const Graph& g, const u32* L, u32 j, const u32* freel
       if (d < pdeg[worst]) { pick[worst] = y; pdeg[worst] = d; }
     }
   };
-  for (u32 i = 0; i < nch; ++i) consider(ch[i]);
-  cost.bytes += 4ull * nfree;
-  for (u32 i = 0; i < nfree; ++i) consider(freel[i]);
+#pragma unroll
+  for (int i = 0; i < NMAX; ++i) {
+    if ((u32)i >= nch) break;
+    consider(ch[i]);
+  }
+  for (u32 i = 0; i < nfpick; ++i) consider(fpick[i]);
   for (u32 q = 0; q < np; ++q)
     for (u32 e = g.off[pick[q]]; e < g.off[pick[q] + 1]; ++e) {
       const u32 w = g.adj[e];
       cost.bytes += 4;
       if (in_tuple(L, j, w)) continue;
-      if (addable(g, w, freel, nfree, ch, ctsz, nch, k, 0xFFFFFFFFu, cost)) return false;
+      if (addable<NMAX>(g, w, freel, nfree, ch, ctsz, nch, k, 0xFFFFFFFFu, cost)) return false;
     }
   return true;
 }

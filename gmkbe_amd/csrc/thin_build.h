@@ -59,7 +59,8 @@ GMK_HD u32 dominance_bound(const Graph& g, const u32* L, u32 j, u32 k, const u32
 // JM bounds the tuple size (j <= JM) so the per-lane arrays have 2^JM entries.
 template <int JM, class Fams>
 GMK_HD void emit_families(const u32* L, u32 j, u32 k, u32 min_b, u32 tuple_idx, const u32* mcnt, u64 far,
-                          const u64* poolp, const u32* nlp, u32 d, u32 dom, Fams& fo, Cost& cost) {
+                          const u64* poolp, const u32* nlp, u32 d, u32 dom, const u32* fpick, u32 nfpick, Fams& fo,
+                          Cost& cost) {
   const u32 full = (1u << j) - 1u;
   const u32 nfree = mcnt[0];
   u8 tmask[(1 << JM)];
@@ -78,6 +79,8 @@ GMK_HD void emit_families(const u32* L, u32 j, u32 k, u32 min_b, u32 tuple_idx, 
     for (u32 i = 0; i < j; ++i) f->L[i] = L[i];
     f->j = j; f->nfree = nfree; f->rsize = (u32)rs;
     f->freep = (const u32*)poolp[0]; f->nlp = nlp; f->nl_size = d; f->tuple = tuple_idx;
+    for (u32 q = 0; q < nfpick; ++q) f->fpick[q] = fpick[q];
+    f->nfpick = nfpick;
   };
   if (nt == 0) {
     if (nfree >= min_b && nfree > 0 && dom > k) {
@@ -182,7 +185,10 @@ GMK_HD bool build_families_t(const Graph& g, const u32* L, u32 j, u32 k, u32 min
     mcnt[0] = d;
     poolp[0] = (u64)ns;
     const u32 dom = dominance_bound(g, L, 1, k, ns, d, 0, 1, cost);
-    emit_families<JM>(L, 1, k, min_b, tuple_idx, mcnt, (u64)(g.b_hi - g.b_lo) - d, poolp, ns, d, dom, fo, cost);
+    u32 fpick[4];
+    const u32 nfp = free_picks(g.off, ns, d, k, fpick, cost);
+    emit_families<JM>(L, 1, k, min_b, tuple_idx, mcnt, (u64)(g.b_hi - g.b_lo) - d, poolp, ns, d, dom, fpick, nfp,
+                      fo, cost);
     return true;
   }
   u32 pos[JM], end[JM];
@@ -235,7 +241,10 @@ GMK_HD bool build_families_t(const Graph& g, const u32* L, u32 j, u32 k, u32 min
     }
   }
   const u32 dom = dominance_bound(g, L, j, k, (const u32*)poolp[0], mcnt[0], 0, 1, cost);
-  emit_families<JM>(L, j, k, min_b, tuple_idx, mcnt, (u64)(g.b_hi - g.b_lo) - d, poolp, nl, d, dom, fo, cost);
+  u32 fpick[4];
+  const u32 nfp = free_picks(g.off, (const u32*)poolp[0], mcnt[0], k, fpick, cost);
+  emit_families<JM>(L, j, k, min_b, tuple_idx, mcnt, (u64)(g.b_hi - g.b_lo) - d, poolp, nl, d, dom, fpick, nfp, fo,
+                    cost);
   return true;
 }
 

@@ -33,14 +33,17 @@ struct DevFamilies {
 };
 
 // One lane per tuple: one-vertex tuples (O(1): the lists are the CSR row) and
-// tuples whose far pool needs N(L) in ascending order (sequential merge).
+// tuples whose far pool needs N(L) in ascending order (sequential merge).  J is
+// the tuple size (j = 1, or j <= k <= 3), so the per-lane type arrays have 2^J
+// entries.
+template <int J>
 __global__ __launch_bounds__(256) void thin_build_families(ThinArgs a) {
   const u32 t = blockIdx.x * blockDim.x + threadIdx.x;
   thin::Cost cost;
   if (t < a.ntuples) {
     DevArena ar{a.arena, a.arena_cap, a.arena_used};
     DevFamilies fo{a.fams, a.fam_cap, a.fam_n, a.tflags};
-    if (!thin::build_families(a.g, a.tuples + (u64)t * a.j, a.j, a.k, a.min_b, t, ar, fo, cost))
+    if (!thin::build_families_t<J>(a.g, a.tuples + (u64)t * J, J, a.k, a.min_b, t, ar, fo, cost))
       atomicOr(a.tflags, FLAG_RES_OVERFLOW);
   }
   const u64 b = wave_sum_u64(cost.bytes);
@@ -141,12 +144,37 @@ __global__ __launch_bounds__(256) void thin_build_wave(ThinArgs a) {
     const u32* freel = a.arena + s_base[wv] + s_off[wv][0];
     u32 dom = thin::dominance_bound(a.g, L, j, a.k, freel, mcnt[0], (u32)lane, 64u, cost);
     dom = (u32)wave_min_u64((u64)dom);
+    // the k+1 free vertices of least degree (ties: first in the list): each
+    // lane keeps its best k+1 of a strided share, then k+1 rounds of a wave
+    // minimum take the overall best
+    const u32 npk = a.k + 1 < 4 ? a.k + 1 : 4;
+    u64 best[4] = {~0ull, ~0ull, ~0ull, ~0ull};
+    for (u32 i = lane; i < mcnt[0]; i += 64) {
+      const u32 y = freel[i];
+      u64 key = ((u64)(a.g.off[y + 1] - a.g.off[y]) << 32) | i;
+      cost.bytes += 12;
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        if ((u32)q < npk && key < best[q]) { const u64 tmp = best[q]; best[q] = key; key = tmp; }
+    }
+    u32 fpick[4];
+    u32 nfp = 0, head = 0;
+    for (u32 q = 0; q < npk; ++q) {
+      u64 mine = ~0ull;
+#pragma unroll
+      for (int h = 0; h < 4; ++h)
+        if ((u32)h == head) mine = best[h];
+      const u64 m = wave_min_u64(mine);
+      if (m == ~0ull) break;
+      if (mine == m) ++head;
+      fpick[nfp++] = freel[(u32)(m & 0xFFFFFFFFu)];
+    }
     if (lane == 0) {
       u64 poolp[1 << J];
       for (u32 mm = 0; mm <= full; ++mm) poolp[mm] = (u64)(a.arena + s_base[wv] + s_off[wv][mm]);
       DevFamilies fo{a.fams, a.fam_cap, a.fam_n, a.tflags};
-      thin::emit_families<J>(L, j, a.k, a.min_b, t, mcnt, (u64)(a.g.b_hi - a.g.b_lo) - d, poolp, nullptr, d, dom, fo,
-                             cost);
+      thin::emit_families<J>(L, j, a.k, a.min_b, t, mcnt, (u64)(a.g.b_hi - a.g.b_lo) - d, poolp, nullptr, d, dom,
+                             fpick, nfp, fo, cost);
     }
   }
   const u64 b = wave_sum_u64(cost.bytes);
@@ -185,11 +213,18 @@ __global__ __launch_bounds__(1024) void thin_scan_totals(const thin::Family* fam
   if (nfam == 0 && t == 0) prefix[0] = 0;
 }
 
+// K: misses allowed per vertex (1..3).  A family chooses at most 2 K^2 vertices
+// besides its free ones (each costs budget of at least one of <= 2K members),
+// so the chosen vertices live in registers: every access uses a constant index.
+template <int K>
 __global__ __launch_bounds__(256) void thin_check(ThinArgs a) {
+  constexpr int NM = 2 * K * K;
   const u64 r = a.r0 + (u64)blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63;
   bool emit = false;
-  u32 ch[thin::MAXJ * 3], ctsz[thin::MAXJ * 3];
+  u32 ch[NM], ctsz[NM];
+#pragma unroll
+  for (int q = 0; q < NM; ++q) { ch[q] = 0; ctsz[q] = 0; }
   u32 nch = 0;
   const thin::Family* f = nullptr;
   thin::Cost cost;
@@ -210,13 +245,16 @@ __global__ __launch_bounds__(256) void thin_check(ThinArgs a) {
       thin::unrank(rt, f->cnt[t], f->psize[t], idx);
       const u32 tsz = thin::popc(f->mask[t]);
       for (u32 q = 0; q < f->cnt[t]; ++q) {
-        ch[nch] = thin::pool_member(*f, t, idx[q], a.g.b_lo, cost);
-        ctsz[nch] = tsz;
+        const u32 y = thin::pool_member(*f, t, idx[q], a.g.b_lo, cost);
+#pragma unroll
+        for (int s2 = 0; s2 < NM; ++s2)
+          if ((u32)s2 == nch) { ch[s2] = y; ctsz[s2] = tsz; }
         ++nch;
       }
     }
     cost.bytes += 8ull * 8 + 16ull * f->ntypes;  // prefix probes, family header and type fields
-    emit = thin::side_a_maximal(a.g, f->L, f->j, f->freep, f->nfree, ch, ctsz, nch, a.k, cost);
+    emit = thin::side_a_maximal<NM>(a.g, f->L, f->j, f->freep, f->nfree, f->fpick, f->nfpick, ch, ctsz, nch, a.k,
+                                    cost);
   }
   // per-lane emission (records are independent)
   u64 h = 0;
@@ -226,7 +264,9 @@ __global__ __launch_bounds__(256) void thin_check(ThinArgs a) {
     const u32* fr0 = f->freep;
     for (u32 i = 0; i < f->j; ++i) h += mix64(f->L[i]);
     for (u32 i = 0; i < f->nfree; ++i) h += mix64(fr0[i]);
-    for (u32 i = 0; i < nch; ++i) h += mix64(ch[i]);
+#pragma unroll
+    for (int i = 0; i < NM; ++i)
+      if ((u32)i < nch) h += mix64(ch[i]);
     cost.bytes += 4ull * f->nfree;
     fpv = fmix64(h + (u64)(a.swap_sides ? nr0 : f->j) * FP_A + (u64)(a.swap_sides ? f->j : nr0) * FP_B);
     if (!dedup_insert(a.dedup, fpv, a.flags, blockIdx.x)) emit = false;
@@ -238,9 +278,13 @@ __global__ __launch_bounds__(256) void thin_check(ThinArgs a) {
       emit = false;
     } else {
       if (slot + 1 == a.limit) atomicOr(a.flags, FLAG_LIMIT);
-      // sort the chosen vertices (few) for the merge with the free list
-      for (u32 i = 1; i < nch; ++i)
-        for (u32 q = i; q > 0 && ch[q - 1] > ch[q]; --q) { const u32 tmp = ch[q]; ch[q] = ch[q - 1]; ch[q - 1] = tmp; }
+      // sort the chosen vertices (few) for the merge with the free list:
+      // odd-even transposition over the first nch slots
+#pragma unroll
+      for (int pass = 0; pass < NM; ++pass)
+#pragma unroll
+        for (int q = pass & 1; q + 1 < NM; q += 2)
+          if ((u32)(q + 1) < nch && ch[q] > ch[q + 1]) { const u32 tmp = ch[q]; ch[q] = ch[q + 1]; ch[q + 1] = tmp; }
       const u32 nr = f->nfree + nch;
       const u32* fr = f->freep;
       const u32 nl = f->j;
@@ -262,9 +306,13 @@ __global__ __launch_bounds__(256) void thin_check(ThinArgs a) {
           for (u32 i = 0; i < nl; ++i) a.res[at_t + i] = f->L[i];
           u32 p = 0, q = 0;
           for (u32 i = 0; i < nr; ++i) {
+            u32 cq = 0xFFFFFFFFu;  // ch[q], or none
+#pragma unroll
+            for (int s2 = 0; s2 < NM; ++s2)
+              if ((u32)s2 == q && q < nch) cq = ch[s2];
             u32 v;
-            if (q >= nch || (p < f->nfree && fr[p] < ch[q])) v = fr[p++];
-            else v = ch[q++];
+            if (q >= nch || (p < f->nfree && fr[p] < cq)) v = fr[p++];
+            else { v = cq; ++q; }
             a.res[at_r + i] = v;
           }
         }
@@ -294,8 +342,15 @@ hipError_t launch_thin_build(const ThinArgs& a, hipStream_t s) {
       case 5: hipLaunchKernelGGL(thin_build_wave<5>, g, b, 0, s, a); break;
       default: hipLaunchKernelGGL(thin_build_wave<6>, g, b, 0, s, a); break;
     }
-  } else
-    hipLaunchKernelGGL(thin_build_families, dim3((a.ntuples + 255) / 256), dim3(256), 0, s, a);
+  } else {
+    const dim3 g((a.ntuples + 255) / 256), b(256);
+    switch (a.j) {
+      case 1: hipLaunchKernelGGL(thin_build_families<1>, g, b, 0, s, a); break;
+      case 2: hipLaunchKernelGGL(thin_build_families<2>, g, b, 0, s, a); break;
+      case 3: hipLaunchKernelGGL(thin_build_families<3>, g, b, 0, s, a); break;
+      default: return hipErrorInvalidValue;  // j <= k <= 3 here
+    }
+  }
   return hipGetLastError();
 }
 
@@ -307,7 +362,13 @@ hipError_t launch_thin_scan(const thin::Family* fams, u32 nfam, u64* prefix, hip
 hipError_t launch_thin_check(const ThinArgs& a, hipStream_t s) {
   if (a.r1 <= a.r0) return hipSuccess;
   const u64 n = a.r1 - a.r0;
-  hipLaunchKernelGGL(thin_check, dim3((u32)((n + 255) / 256)), dim3(256), 0, s, a);
+  const dim3 g((u32)((n + 255) / 256)), b(256);
+  switch (a.k) {
+    case 1: hipLaunchKernelGGL(thin_check<1>, g, b, 0, s, a); break;
+    case 2: hipLaunchKernelGGL(thin_check<2>, g, b, 0, s, a); break;
+    case 3: hipLaunchKernelGGL(thin_check<3>, g, b, 0, s, a); break;
+    default: return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 
