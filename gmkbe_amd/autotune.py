@@ -55,7 +55,7 @@ def _log(msg: str) -> None:
 
 def tune(graphs_per_bucket: int = 24, cap: int = 1000, reps: int = 2) -> dict:
     eng = Engine(0, tuning=None)
-    base = eng.get_tuning()
+    base = eng.get_tuning()  # the library's built-in table; only the swept fields change
     chosen_w, chosen_b = list(base["waves_per_cu"]), list(base["batch_states"])
     trials = []
     for lane, lo, hi in BUCKETS:

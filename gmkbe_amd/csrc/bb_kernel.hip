@@ -42,7 +42,22 @@ struct __align__(16) Smem {
   u64 prefix[W];           // pivot candidates already branched on
   uint16_t deg[2][64 * W];
   uint8_t ms[2][64 * W];
+  uint8_t widx[W];         // words of the opposite side that the current pass reads
 };
+
+// Lists in sm.widx the words w < wb with mask_a[w] | mask_b[w] != 0; returns
+// their number.  Words outside the list contribute nothing to the counters.
+template <int W>
+__device__ __forceinline__ u32 word_list(Smem<W>& sm, const u64* ma, const u64* mb, u32 wb) {
+  const int lane = threadIdx.x & 63;
+  const bool nz = (u32)lane < wb && ((ma[lane] | (mb ? mb[lane] : 0)) != 0);
+  const u64 bal = __ballot(nz);
+  if (nz) sm.widx[__popcll(bal & ((1ull << lane) - 1ull))] = (uint8_t)lane;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+  return (u32)__popcll(bal);
+}
 
 __device__ __forceinline__ u64* setp(u64* base, int side, int which, int W) {
   return base + (side * 3 + which) * W;
@@ -61,15 +76,18 @@ __device__ void side_stats(Smem<W>& sm, const PartDesc& P, const u64* rows, int 
   const u64* Xa = setp(sm.set, a, 2, W);
   const u64* Sb = setp(sm.set, a ^ 1, 0, W);
   const u64* Cb = setp(sm.set, a ^ 1, 1, W);
+  const u32 nw = word_list<W>(sm, Cb, Sb, wb);
   u32 tests = 0;
   for (u32 i = 0; i < wa; ++i) {
-    const u32 v = i * 64 + lane;
     const u64 memb = Sa[i] | Ca[i] | Xa[i];
+    if (memb == 0) continue;  // the counters of non-members are never read
+    const u32 v = i * 64 + lane;
     const bool m = v < na && ((memb >> lane) & 1ull);
     u32 d = 0, s = 0;
     if (m) {
       const u64* row = rowsA + (u64)v * wb;
-      for (u32 w = 0; w < wb; ++w) {
+      for (u32 q = 0; q < nw; ++q) {
+        const u32 w = sm.widx[q];
         const u64 r = row[w];
         d += __popcll(r & Cb[w]);
         s += __popcll(Sb[w] & ~r);
@@ -80,7 +98,8 @@ __device__ void side_stats(Smem<W>& sm, const PartDesc& P, const u64* rows, int 
     sm.ms[a][v] = (uint8_t)(s > 255 ? 255 : s);
   }
   st.intersections += tests;
-  st.bytes += (u64)tests * wb * 8;
+  st.bytes += (u64)tests * nw * 8;
+  __builtin_amdgcn_wave_barrier();  // widx is rewritten by the next pass
 }
 
 // Recompute deg only (after C changed).
@@ -95,21 +114,27 @@ __device__ void side_deg(Smem<W>& sm, const PartDesc& P, const u64* rows, int a,
   const u64* Ca = setp(sm.set, a, 1, W);
   const u64* Xa = setp(sm.set, a, 2, W);
   const u64* Cb = setp(sm.set, a ^ 1, 1, W);
+  const u32 nw = word_list<W>(sm, Cb, nullptr, wb);
   u32 tests = 0;
   for (u32 i = 0; i < wa; ++i) {
-    const u32 v = i * 64 + lane;
     const u64 memb = Sa[i] | Ca[i] | Xa[i];
+    if (memb == 0) continue;
+    const u32 v = i * 64 + lane;
     const bool m = v < na && ((memb >> lane) & 1ull);
     u32 d = 0;
     if (m) {
       const u64* row = rowsA + (u64)v * wb;
-      for (u32 w = 0; w < wb; ++w) d += __popcll(row[w] & Cb[w]);
+      for (u32 q = 0; q < nw; ++q) {
+        const u32 w = sm.widx[q];
+        d += __popcll(row[w] & Cb[w]);
+      }
       ++tests;
     }
     sm.deg[a][v] = (uint16_t)d;
   }
   st.intersections += tests;
-  st.bytes += (u64)tests * wb * 8;
+  st.bytes += (u64)tests * nw * 8;
+  __builtin_amdgcn_wave_barrier();
 }
 
 template <int W>

@@ -84,8 +84,9 @@ struct DeviceVec {
 struct gmk_ctx {
   int device = 0;
   u32 n_cu = 256;                   // compute units of the device
-  gmk_tuning tune{GMK_TUNING_VERSION, {16, 16, 16, 16, 16, 16, 16},
-                  {65536, 65536, 65536, 65536, 65536, 65536, 65536}, 500};
+  // launch table measured on MI355X by gmkbe_amd/autotune.py (gmkbe_amd/tune/mi355x.json)
+  gmk_tuning tune{GMK_TUNING_VERSION, {0, 0, 0, 0, 0, 0, 0},
+                  {16384, 16384, 65536, 16384, 16384, 65536, 65536}, 500};
   hipStream_t stream = nullptr;
   std::string err;
   // graph (host copy kept for partition construction)

@@ -9,7 +9,8 @@
 //   -m <0|1>             verbose (per-kernel statistics)
 //   -g <n>               accepted and ignored, as in the reference
 // Additions: -t <theta> (size lower bound, default 1), --mode plain|batched|learned,
-// --partition auto|global|seeded.
+// --partition auto|global|seeded, --tune <table.json> (launch table written by
+// gmkbe_amd/autotune.py; the library's built-in MI355X table otherwise).
 //
 // Records, as written by the reference:
 //   "Output n, cost Time: t ms" at the milestones {10^i < r} and r
@@ -113,10 +114,41 @@ fs::path record_path(const fs::path& graph, const std::string& suffix) {
   return bin.parent_path() / (bin.stem().string() + suffix + ".json");
 }
 
+// Reads the "tuning" object of an autotuner table: the arrays waves_per_cu and
+// batch_states (7 numbers each) and dedup_load_permille.
+bool read_tuning(const std::string& path, gmk_tuning& t) {
+  std::ifstream in(path);
+  if (!in) return false;
+  std::stringstream ss;
+  ss << in.rdbuf();
+  const std::string s = ss.str();
+  size_t at = s.find("\"tuning\"");
+  if (at == std::string::npos) return false;
+  auto numbers = [&](const char* key, uint32_t* out, int n) -> bool {
+    size_t k = s.find(std::string("\"") + key + "\"", at);
+    if (k == std::string::npos) return false;
+    size_t i = s.find_first_of("[0123456789", k + strlen(key) + 2);
+    if (i == std::string::npos) return false;
+    if (s[i] == '[') ++i;
+    for (int q = 0; q < n; ++q) {
+      while (i < s.size() && (s[i] == ' ' || s[i] == ',' || s[i] == '\n')) ++i;
+      char* end = nullptr;
+      const unsigned long v = strtoul(s.c_str() + i, &end, 10);
+      if (end == s.c_str() + i) return false;
+      out[q] = (uint32_t)v;
+      i = end - s.c_str();
+    }
+    return true;
+  };
+  t.version = GMK_TUNING_VERSION;
+  return numbers("waves_per_cu", t.waves_per_cu, 7) && numbers("batch_states", t.batch_states, 7) &&
+         numbers("dedup_load_permille", &t.dedup_load_permille, 1);
+}
+
 void usage() {
   fprintf(stderr,
           "gmkbpe -f <graph.bin> [-k 1] [-r 10000000] [-d 0] [-q 0] [-m 0] [-t 1] "
-          "[--mode plain|batched|learned] [--partition auto|global|seeded]\n");
+          "[--mode plain|batched|learned] [--partition auto|global|seeded] [--tune table.json]\n");
 }
 
 }  // namespace
@@ -127,6 +159,7 @@ int main(int argc, char** argv) {
   int q = 0, m = 0, dev = 0;
   unsigned k = 1, theta = 1;
   int mode = GMK_MODE_BATCHED, part = GMK_PART_AUTO;
+  std::string tune_path;
   for (int i = 1; i < argc; ++i) {
     std::string a = argv[i];
     auto val = [&]() -> const char* {
@@ -147,6 +180,8 @@ int main(int argc, char** argv) {
     } else if (a == "--partition") {
       std::string v = val();
       part = v == "global" ? GMK_PART_GLOBAL : v == "seeded" ? GMK_PART_SEEDED : GMK_PART_AUTO;
+    } else if (a == "--tune") {
+      tune_path = val();
     } else if (a == "-h" || a == "--help") { usage(); return 0; }
     else { fprintf(stderr, "unknown option %s\n", a.c_str()); usage(); return 1; }
   }
@@ -161,6 +196,14 @@ int main(int argc, char** argv) {
   }
   gmk_ctx* ctx = nullptr;
   if (gmk_create(dev, &ctx) != GMK_OK) { fprintf(stderr, "Error: cannot open device %d\n", dev); return 1; }
+  if (!tune_path.empty()) {
+    gmk_tuning t{};
+    if (!read_tuning(tune_path, t) || gmk_set_tuning(ctx, &t) != GMK_OK) {
+      fprintf(stderr, "Error: cannot use tuning table %s: %s\n", tune_path.c_str(), gmk_last_error(ctx));
+      gmk_destroy(ctx);
+      return 1;
+    }
+  }
   printf("Using GPU id: %d\n", dev);
   uint32_t n = 0, nl = 0;
   uint64_t nnz = 0;

@@ -187,8 +187,8 @@ int gmk_set_cost_model(gmk_ctx* ctx, const gmk_mlp* model);
  * Launch parameters of the branch-and-bound expansion per bitset width
  * (index l: 2^l words, l = 0..6), as chosen per size bucket by the autotuner
  * (gmkbe_amd/autotune.py, table under gmkbe_amd/tune/).  Results do not depend
- * on them.  Defaults: 16 persistent waves per CU, 65536 states per launch,
- * dedup load 0.5. */
+ * on them.  Defaults: the table measured on MI355X (one workgroup per state;
+ * 16384 or 65536 states per launch by width; dedup load 0.5). */
 #define GMK_TUNING_VERSION 1
 typedef struct {
   uint32_t version;                 /* GMK_TUNING_VERSION */

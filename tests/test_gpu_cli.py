@@ -56,3 +56,26 @@ def test_cli_limit_milestones_and_time_record(tmp_path):
     assert p.returncode == 0, p.stderr
     rec = json.load(open(tmp_path / "er200x300.graph.gpu1.json"))
     assert rec["1"]["100"] == 5 * 3600 * 1000 and "100000" in rec["1"]
+
+
+def test_cli_tuning_table(tmp_path):
+    """--tune loads an autotuner table; results do not depend on it, and a
+    table the library refuses ends the run with an error."""
+    c = [c for c in GC.cases(min_k=1) if c["graph"] == "er40x50" and c["k"] == 1 and c["theta"] == 1][0]
+    src = os.path.join(GC.HERE, GC._G["graphs"]["er40x50"])
+    g = str(tmp_path / "er40x50.graph.bin")
+    shutil.copy(src, g)
+    table = json.load(open(engine.TUNE_TABLE))
+    alt = dict(table, tuning=dict(table["tuning"], waves_per_cu=[8] * 7, batch_states=[256] * 7))
+    bad = dict(table, tuning=dict(table["tuning"], batch_states=[1] * 7))
+    for name, t, ok in (("table.json", table, True), ("alt.json", alt, True), ("bad.json", bad, False)):
+        path = str(tmp_path / name)
+        json.dump(t, open(path, "w"))
+        print(f"[gpu] gmkbpe --tune {name}", flush=True)
+        p = subprocess.run([CLI, "-f", g, "-k", "1", "-r", "0", "-q", "1", "--tune", path], capture_output=True,
+                           text=True, timeout=120)
+        if ok:
+            assert p.returncode == 0, p.stderr
+            assert GC.digest_matches(c, O.digest(_records_from_lines(p.stdout)))
+        else:
+            assert p.returncode != 0 and "tuning" in p.stderr
