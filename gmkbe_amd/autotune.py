@@ -12,7 +12,9 @@ sqrt(n), so a seed's 2-hop neighbourhood covers its side).  Every candidate
 setting runs the batch in batched mode; the fastest per bucket is kept for
 that width (the widest bucket also sets W = 64).  Every candidate must return
 the same per-graph counts, and the same fingerprints for graphs below the
-result cap: the tuning changes launches, not results.  The dedup table's load factor is tuned on the 50k x 30k workload.
+result cap: the tuning changes launches, not results.  The thin checks' launch
+form is tuned per k on graphs shaped like the workloads of that k, and the
+dedup table's load factor on the 50k x 30k workload.
 
     python -m gmkbe_amd.autotune [--out gmkbe_amd/tune/mi355x.json]
 
@@ -92,11 +94,38 @@ def tune(graphs_per_bucket: int = 24, cap: int = 1000, reps: int = 2) -> dict:
         _log(f"[autotune] W={1 << lane}: waves/CU={best[1]} batch={best[2]}")
     chosen_w[6], chosen_b[6] = chosen_w[5], chosen_b[5]
 
-    # dedup table load factor on the 50k x 30k workload (seeded, batched, r = 10^6)
-    g = gen.workload("cl50kx30k")
-    eng.set_graph(g)
+    # thin checks per k: one lane per rank over the grid, or waves taking chunks
+    # of ranks; seeded, batched, r = 10^6, on a graph shaped like the workload
+    # each k is measured on: k=1 the 50k x 30k Chung-Lu graph, k=2 a 400k x 200k
+    # Chung-Lu graph (exponent 2.0, uncapped hubs), k=3 a 100k x 100k graph with
+    # planted near-bicliques
+    proxies = {1: gen.workload("cl50kx30k"), 2: gen.chung_lu(400_000, 200_000, 4_000_000, 2.0, gen.SEED_C4 + 7),
+               3: gen.planted(100_000, 100_000, 1_000_000, 2.2, gen.SEED_C5 + 7)}
+    chunked = list(base.get("check_chunked", [0, 0, 0, 1]))
+    for k in (1, 2, 3):
+        eng.set_graph(proxies[k])
+        best_c = None
+        for form in (0, 1):
+            t = dict(base)
+            t["waves_per_cu"], t["batch_states"] = chosen_w, chosen_b
+            t["check_chunked"] = [form] * 4
+            eng.set_tuning(t)
+            times = []
+            for _ in range(reps + 1):
+                t0 = time.perf_counter()
+                r = eng.enumerate(k=k, theta=1, limit=1_000_000, mode="batched", partition="seeded", keep=False)
+                times.append(time.perf_counter() - t0)
+            s = min(times[1:])
+            trials.append({"k": k, "check_chunked": form, "s": s, "results": r.count})
+            _log(f"[autotune] thin checks k={k} {'chunked' if form else 'one rank per lane'}: {s * 1e3:.2f} ms")
+            if best_c is None or s < best_c[0]:
+                best_c = (s, form)
+        chunked[k] = best_c[1]
+
+    # dedup table load factor on the 50k x 30k workload
+    eng.set_graph(proxies[1])
     t = dict(base)
-    t["waves_per_cu"], t["batch_states"] = chosen_w, chosen_b
+    t["waves_per_cu"], t["batch_states"], t["check_chunked"] = chosen_w, chosen_b, chunked
     eng.set_tuning(t)
     best_load = None
     for load in DEDUP_LOADS:
@@ -120,7 +149,7 @@ def tune(graphs_per_bucket: int = 24, cap: int = 1000, reps: int = 2) -> dict:
         "generated": datetime.datetime.now(datetime.timezone.utc).isoformat(timespec="seconds"),
         "buckets": [{"lane": lane, "W": 1 << lane, "sides": [lo, hi]} for lane, lo, hi in BUCKETS],
         "tuning": {"version": TUNING_VERSION, "waves_per_cu": chosen_w, "batch_states": chosen_b,
-                   "dedup_load_permille": best_load[1]},
+                   "dedup_load_permille": best_load[1], "check_chunked": chunked},
         "trials": trials,
     }
 

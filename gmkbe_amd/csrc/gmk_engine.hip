@@ -56,6 +56,8 @@ extern "C" int gmk_set_tuning(gmk_ctx* c, const gmk_tuning* t) {
     c->err = "dedup load out of range";
     return GMK_EINVAL;
   }
+  for (int q = 0; q < 4; ++q)
+    if (t->check_chunked[q] > 1) { c->err = "tuning value out of range"; return GMK_EINVAL; }
   c->tune = *t;
   return GMK_OK;
 }

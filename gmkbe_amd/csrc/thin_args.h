@@ -19,6 +19,7 @@ struct ThinArgs {
   u32 fam_cap;
   u32* fam_n;
   u32* tflags;          // family-construction overflow flags
+  u32* work;            // chunk counter of a check launch (zeroed before it)
   u32* flags;           // run flags (limit, result overflow)
   // T2
   const u64* prefix;    // nfam + 1 prefix sums of family totals
@@ -37,7 +38,7 @@ struct ThinArgs {
 };
 
 hipError_t launch_thin_build(const ThinArgs& a, hipStream_t s);
-hipError_t launch_thin_check(const ThinArgs& a, hipStream_t s);
+hipError_t launch_thin_check(const ThinArgs& a, hipStream_t s, u32 max_blocks, bool chunked);
 hipError_t launch_thin_scan(const thin::Family* fams, u32 nfam, u64* prefix, hipStream_t s);
 
 }  // namespace gmk

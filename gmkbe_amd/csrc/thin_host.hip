@@ -17,6 +17,7 @@ namespace {
 struct ThinCounters {
   u32 fam_n, tflags;
   u64 arena_used;
+  u32 work, pad;  // chunk counter of the check kernel
 };
 
 constexpr u32 TCH = 1u << 16;          // tuples per family-construction launch
@@ -75,8 +76,9 @@ int run_chunk(ThinRun& T, const u32* tuples, u32 count, u32 j) {
   for (u64 r0 = 0; r0 < total; r0 += RCH) {
     a.r0 = r0;
     a.r1 = std::min(total, r0 + RCH);
+    HIPCHK(c, hipMemsetAsync(&((ThinCounters*)c->tcnt.p)->work, 0, sizeof(u32), s));
     HIPCHK(c, hipEventRecord(c->ev0, s));
-    HIPCHK(c, launch_thin_check(a, s));
+    HIPCHK(c, launch_thin_check(a, s, c->n_cu * 8, c->tune.check_chunked[a.k < 4 ? a.k : 3] != 0));
     HIPCHK(c, hipEventRecord(c->ev1, s));
     HIPCHK(c, hipMemcpyAsync(c->h_cnt, dc, offsetof(RunCounters, fp), hipMemcpyDeviceToHost, s));
     HIPCHK(c, hipStreamSynchronize(s));
@@ -127,6 +129,7 @@ int gmk_run_thin(gmk_ctx* c, const gmk_params* prm, gmk_stats* st, RunState& rs,
   a.fam_cap = FCAP;
   a.fam_n = &tc->fam_n;
   a.tflags = &tc->tflags;
+  a.work = &tc->work;
   RunCounters* dc = (RunCounters*)c->counters.p;
   a.flags = &dc->flags;
   a.prefix = (const u64*)c->prefix.p;

@@ -213,22 +213,20 @@ __global__ __launch_bounds__(1024) void thin_scan_totals(const thin::Family* fam
   if (nfam == 0 && t == 0) prefix[0] = 0;
 }
 
+// Checks rank r (one lane): unrank its combination, test maximality, emit.
 // K: misses allowed per vertex (1..3).  A family chooses at most 2 K^2 vertices
 // besides its free ones (each costs budget of at least one of <= 2K members),
 // so the chosen vertices live in registers: every access uses a constant index.
 template <int K>
-__global__ __launch_bounds__(256) void thin_check(ThinArgs a) {
+__device__ __forceinline__ void check_rank(const ThinArgs& a, u64 r, thin::Cost& cost, u64& n_emit, u64& n_tested) {
   constexpr int NM = 2 * K * K;
-  const u64 r = a.r0 + (u64)blockIdx.x * blockDim.x + threadIdx.x;
-  const int lane = threadIdx.x & 63;
   bool emit = false;
   u32 ch[NM], ctsz[NM];
 #pragma unroll
   for (int q = 0; q < NM; ++q) { ch[q] = 0; ctsz[q] = 0; }
   u32 nch = 0;
   const thin::Family* f = nullptr;
-  thin::Cost cost;
-  if (r < a.r1 && !(__hip_atomic_load(a.flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & FLAG_LIMIT)) {
+  if (r < a.r1) {
     // family containing rank r
     u32 lo = 0, hi = a.nfam;
     while (hi - lo > 1) {
@@ -319,16 +317,51 @@ __global__ __launch_bounds__(256) void thin_check(ThinArgs a) {
       }
     }
   }
-  (void)lane;
-  const u64 e = emit ? 1 : 0;
-  const u64 tested = (r < a.r1 && f) ? 1 : 0;
-  const u64 se = wave_sum_u64(e), stt = wave_sum_u64(tested), sb = wave_sum_u64(cost.bytes);
-  if (lane == 0 && stt) {
+  n_emit += emit ? 1 : 0;
+  n_tested += (r < a.r1 && f) ? 1 : 0;
+}
+
+template <int K>
+__device__ __forceinline__ void flush_check_stats(const ThinArgs& a, const thin::Cost& cost, u64 n_emit,
+                                                  u64 n_tested) {
+  const u64 se = wave_sum_u64(n_emit), stt = wave_sum_u64(n_tested), sb = wave_sum_u64(cost.bytes);
+  if ((threadIdx.x & 63) == 0 && stt) {
     u64* sp = a.stat + ((blockIdx.x * 4 + (threadIdx.x >> 6)) % NSTRIPE) * NSTAT;
     atomicAdd((unsigned long long*)&sp[1], (unsigned long long)stt);
     atomicAdd((unsigned long long*)&sp[2], (unsigned long long)se);
     atomicAdd((unsigned long long*)&sp[5], (unsigned long long)sb);
   }
+}
+
+// One lane per rank over the whole grid.
+template <int K>
+__global__ __launch_bounds__(256) void thin_check(ThinArgs a) {
+  const u64 r = a.r0 + (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  u64 n_emit = 0, n_tested = 0;
+  thin::Cost cost;
+  if (!(__hip_atomic_load(a.flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & FLAG_LIMIT))
+    check_rank<K>(a, r, cost, n_emit, n_tested);
+  flush_check_stats<K>(a, cost, n_emit, n_tested);
+}
+
+// Chunked form: a fixed grid of waves takes chunks of 64 consecutive ranks from
+// a device counter until the launch's range is done, so a wave held by an
+// expensive rank does not hold back the chunks after it.
+template <int K>
+__global__ __launch_bounds__(256) void thin_check_chunked(ThinArgs a) {
+  const int lane = threadIdx.x & 63;
+  u64 n_emit = 0, n_tested = 0;
+  thin::Cost cost;
+  for (;;) {
+    u32 chunk = 0;
+    if (lane == 0) chunk = atomicAdd(a.work, 1u);
+    chunk = (u32)wave_bcast_u64(chunk, 0);
+    const u64 c0 = a.r0 + (u64)chunk * 64;
+    if (c0 >= a.r1) break;
+    if (__hip_atomic_load(a.flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & FLAG_LIMIT) break;
+    check_rank<K>(a, c0 + lane, cost, n_emit, n_tested);
+  }
+  flush_check_stats<K>(a, cost, n_emit, n_tested);
 }
 
 hipError_t launch_thin_build(const ThinArgs& a, hipStream_t s) {
@@ -359,15 +392,27 @@ hipError_t launch_thin_scan(const thin::Family* fams, u32 nfam, u64* prefix, hip
   return hipGetLastError();
 }
 
-hipError_t launch_thin_check(const ThinArgs& a, hipStream_t s) {
+hipError_t launch_thin_check(const ThinArgs& a, hipStream_t s, u32 max_blocks, bool chunked) {
   if (a.r1 <= a.r0) return hipSuccess;
   const u64 n = a.r1 - a.r0;
-  const dim3 g((u32)((n + 255) / 256)), b(256);
-  switch (a.k) {
-    case 1: hipLaunchKernelGGL(thin_check<1>, g, b, 0, s, a); break;
-    case 2: hipLaunchKernelGGL(thin_check<2>, g, b, 0, s, a); break;
-    case 3: hipLaunchKernelGGL(thin_check<3>, g, b, 0, s, a); break;
-    default: return hipErrorInvalidValue;
+  const u64 need = (n + 255) / 256;
+  const dim3 b(256);
+  if (chunked) {
+    const dim3 g((u32)(need < max_blocks ? need : max_blocks));
+    switch (a.k) {
+      case 1: hipLaunchKernelGGL(thin_check_chunked<1>, g, b, 0, s, a); break;
+      case 2: hipLaunchKernelGGL(thin_check_chunked<2>, g, b, 0, s, a); break;
+      case 3: hipLaunchKernelGGL(thin_check_chunked<3>, g, b, 0, s, a); break;
+      default: return hipErrorInvalidValue;
+    }
+  } else {
+    const dim3 g((u32)need);
+    switch (a.k) {
+      case 1: hipLaunchKernelGGL(thin_check<1>, g, b, 0, s, a); break;
+      case 2: hipLaunchKernelGGL(thin_check<2>, g, b, 0, s, a); break;
+      case 3: hipLaunchKernelGGL(thin_check<3>, g, b, 0, s, a); break;
+      default: return hipErrorInvalidValue;
+    }
   }
   return hipGetLastError();
 }

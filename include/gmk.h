@@ -187,14 +187,15 @@ int gmk_set_cost_model(gmk_ctx* ctx, const gmk_mlp* model);
  * Launch parameters of the branch-and-bound expansion per bitset width
  * (index l: 2^l words, l = 0..6), as chosen per size bucket by the autotuner
  * (gmkbe_amd/autotune.py, table under gmkbe_amd/tune/).  Results do not depend
- * on them.  Defaults: the table measured on MI355X (one workgroup per state;
- * 16384 or 65536 states per launch by width; dedup load 0.5). */
-#define GMK_TUNING_VERSION 1
+ * on them.  Defaults: the table measured on MI355X (one workgroup per state,
+ * 16384 states per launch, dedup load 0.5, chunked thin checks for k = 3). */
+#define GMK_TUNING_VERSION 2
 typedef struct {
   uint32_t version;                 /* GMK_TUNING_VERSION */
   uint32_t waves_per_cu[7];         /* batched modes: persistent waves per CU; 0 = one workgroup per state */
   uint32_t batch_states[7];         /* states per expansion launch, 64 .. 65536 */
   uint32_t dedup_load_permille;     /* dedup table load when gmk_params leaves it 0 (100 .. 900) */
+  uint32_t check_chunked[4];        /* per k: thin checks as waves taking 64-rank chunks (1) or one lane per rank (0) */
 } gmk_tuning;
 int gmk_set_tuning(gmk_ctx* ctx, const gmk_tuning* tuning);
 int gmk_get_tuning(gmk_ctx* ctx, gmk_tuning* tuning);

@@ -35,5 +35,5 @@ def test_params_and_stats_layout_match_header():
     # the ctypes mirrors must have the C structs' sizes (x86-64 SysV layout)
     import ctypes
     assert ctypes.sizeof(engine.Params) == 48
-    assert ctypes.sizeof(engine.Tuning) == 4 + 4 * 7 + 4 * 7 + 4
+    assert ctypes.sizeof(engine.Tuning) == 4 + 4 * 7 + 4 * 7 + 4 + 4 * 4
     assert ctypes.sizeof(engine.Stats) == 8 * 13 + 8 + 32 * 3
