@@ -2,19 +2,21 @@
 
 The expansion kernel runs one wave per search state, either one workgroup per
 state or as a persistent grid of waves per compute unit that takes states from
-a device counter.  Which form, how many waves, and how many states go into one
-launch depend on the state's bitset width W (1..64 words): wide states hold
-more LDS per wave (fewer resident waves) and their searches branch wider.
+a device counter.  Which form and how many states go into one launch depend on
+the state's bitset width W (1..64 words): wide states hold more LDS per wave
+(fewer resident waves) and their searches branch wider.
 
-Each size bucket is a calibration batch of Erdos-Renyi graphs whose seed
-partitions have one bitset width (both sides near 64 W vertices, degree about
-sqrt(n), so a seed's 2-hop neighbourhood covers its side).  Every candidate
-setting runs the batch in batched mode; the fastest per bucket is kept for
-that width (the widest bucket also sets W = 64).  Every candidate must return
-the same per-graph counts, and the same fingerprints for graphs below the
-result cap: the tuning changes launches, not results.  The thin checks' launch
-form is tuned per k on graphs shaped like the workloads of that k, and the
-dedup table's load factor on the 50k x 30k workload.
+The size buckets are those of the many-graphs workload (sides 20-63, 64-255,
+256-1023, 1024-2000); a bucket's seed partitions have widths W = 1, 2-4, 8-16
+and 32-64 words.  Each bucket is calibrated on 24 graphs drawn like that
+workload (Erdos-Renyi, density log-uniform in [0.01, 0.2], k in {1, 2},
+theta = 2k+1, at most 1000 results per graph) from a seed of its own, with
+the largest side in the bucket.  Every candidate setting runs the batch in
+batched mode; the fastest is kept for the bucket's widths.  Every candidate
+must return the same per-graph counts, and the same fingerprints for graphs
+below the result cap: the tuning changes launches, not results.  The thin
+checks' launch form is tuned per k on graphs shaped like the workloads of
+that k, and the dedup table's load factor on the 50k x 30k workload.
 
     python -m gmkbe_amd.autotune [--out gmkbe_amd/tune/mi355x.json]
 
@@ -34,20 +36,22 @@ import numpy as np
 from . import gen
 from .engine import TUNE_TABLE, TUNING_VERSION, Engine
 
-# (lane = log2 W, side low, side high)
-BUCKETS = [(0, 40, 60), (1, 90, 125), (2, 180, 250), (3, 360, 500), (4, 720, 1000), (5, 1450, 2000)]
-WAVES = (0, 4, 8, 16, 32)
+# (largest side low, high, widths log2 W the bucket's partitions use)
+BUCKETS = [(20, 63, (0,)), (64, 255, (1, 2)), (256, 1023, (3, 4)), (1024, 2000, (5, 6))]
+WAVES = (0, 16)
 BATCH = (16384, 65536)
 DEDUP_LOADS = (300, 500, 700)
+CAL_SEED = gen.SEED_C3 + 2000
 
 
-def bucket_graphs(lo: int, hi: int, count: int, seed: int):
-    rng = np.random.default_rng(seed)
+def bucket_graphs(lo: int, hi: int, count: int):
+    """The first `count` graphs of the calibration draw whose largest side is in [lo, hi]."""
     out = []
-    for i in range(count):
-        a, b = int(rng.integers(lo, hi + 1)), int(rng.integers(lo, hi + 1))
-        p = min(0.5, 1.2 / np.sqrt((a + b) / 2))
-        out.append((gen.er(a, b, p, seed * 1000 + i), 1, 3))
+    for a, b, p, k, sd in gen.er_batch_specs(4000, CAL_SEED):
+        if lo <= max(a, b) <= hi:
+            out.append((gen.er(a, b, p, sd), k, 2 * k + 1))
+            if len(out) == count:
+                break
     return out
 
 
@@ -55,13 +59,13 @@ def _log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
-def tune(graphs_per_bucket: int = 24, cap: int = 1000, reps: int = 2) -> dict:
+def tune(graphs_per_bucket: int = 24, cap: int = 1000, reps: int = 1) -> dict:
     eng = Engine(0, tuning=None)
     base = eng.get_tuning()  # the library's built-in table; only the swept fields change
     chosen_w, chosen_b = list(base["waves_per_cu"]), list(base["batch_states"])
     trials = []
-    for lane, lo, hi in BUCKETS:
-        graphs = bucket_graphs(lo, hi, graphs_per_bucket, 7000 + lane)
+    for lo, hi, lanes in BUCKETS:
+        graphs = bucket_graphs(lo, hi, graphs_per_bucket)
         ref = None
         best = None
         for w in WAVES:
@@ -82,17 +86,18 @@ def tune(graphs_per_bucket: int = 24, cap: int = 1000, reps: int = 2) -> dict:
                 if ref is None:
                     ref = sig
                 if sig != ref:
-                    raise RuntimeError(f"bucket W={1 << lane}: results changed with waves={w} batch={b}")
+                    raise RuntimeError(f"bucket {lo}-{hi}: results changed with waves={w} batch={b}")
                 s = min(times)
-                trials.append({"lane": lane, "W": 1 << lane, "waves_per_cu": w, "batch_states": b, "s": s,
+                trials.append({"bucket": [lo, hi], "waves_per_cu": w, "batch_states": b, "s": s,
                                "results": tot["count"], "states": tot["states"]})
-                _log(f"[autotune] W={1 << lane:2d} waves/CU={w:2d} batch={b:5d}: {s * 1e3:8.2f} ms "
+                _log(f"[autotune] sides {lo}-{hi} waves/CU={w:2d} batch={b:5d}: {s * 1e3:9.2f} ms "
                      f"({tot['states']} states)")
                 if best is None or s < best[0]:
                     best = (s, w, b)
-        chosen_w[lane], chosen_b[lane] = best[1], best[2]
-        _log(f"[autotune] W={1 << lane}: waves/CU={best[1]} batch={best[2]}")
-    chosen_w[6], chosen_b[6] = chosen_w[5], chosen_b[5]
+        for lane in lanes:
+            chosen_w[lane], chosen_b[lane] = best[1], best[2]
+        _log(f"[autotune] sides {lo}-{hi} (W = {', '.join(str(1 << q) for q in lanes)}): waves/CU={best[1]} "
+             f"batch={best[2]}")
 
     # thin checks per k: one lane per rank over the grid, or waves taking chunks
     # of ranks; seeded, batched, r = 10^6, on a graph shaped like the workload
@@ -147,7 +152,7 @@ def tune(graphs_per_bucket: int = 24, cap: int = 1000, reps: int = 2) -> dict:
         "version": TUNING_VERSION,
         "target": "gfx950 (MI355X)",
         "generated": datetime.datetime.now(datetime.timezone.utc).isoformat(timespec="seconds"),
-        "buckets": [{"lane": lane, "W": 1 << lane, "sides": [lo, hi]} for lane, lo, hi in BUCKETS],
+        "buckets": [{"sides": [lo, hi], "W": [1 << q for q in lanes]} for lo, hi, lanes in BUCKETS],
         "tuning": {"version": TUNING_VERSION, "waves_per_cu": chosen_w, "batch_states": chosen_b,
                    "dedup_load_permille": best_load[1], "check_chunked": chunked},
         "trials": trials,

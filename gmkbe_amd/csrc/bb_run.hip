@@ -13,6 +13,7 @@
 #include <atomic>
 #include <chrono>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <string>
@@ -139,8 +140,55 @@ void BBEngine::reset_tables() {
 
 u64 BBEngine::live() const {
   u64 n = staged_roots_;
-  for (int l = 0; l < NLANE; ++l) n += top_[l];
+  for (int l = 0; l < NLANE; ++l) n += top_[l] + spill_n_[l];
   return n;
+}
+
+// Moves the bottom (shallowest) ceil(top/2) states of a nearly full stack to the
+// width's spill buffer (device memory, grown by doubling with its contents
+// copied) and slides the rest down; the stack then has at least half its
+// capacity free.
+int BBEngine::spill(int l) {
+  const u64 stride = 1 + 6ull * (1u << l), sb = stride * 8;
+  const u64 move = (top_[l] + 1) / 2;
+  if (spill_n_[l] + move > spill_cap_[l]) {
+    const u64 ncap = std::max<u64>({cap_[l], 2 * spill_cap_[l], spill_n_[l] + move});
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, ncap * sb);
+    if (e != hipSuccess) {
+      c_->err = "spill buffer: hipMalloc(" + std::to_string(ncap * sb) + " B) failed: " + hipGetErrorString(e);
+      return GMK_ENOMEM;
+    }
+    if (spill_n_[l])
+      HIPCHK(c_, hipMemcpyAsync(p, c_->lane_spill[l].p, spill_n_[l] * sb, hipMemcpyDeviceToDevice, c_->stream));
+    HIPCHK(c_, hipStreamSynchronize(c_->stream));
+    if (c_->lane_spill[l].p) (void)hipFree(c_->lane_spill[l].p);
+    c_->lane_spill[l].p = p;
+    c_->lane_spill[l].bytes = ncap * sb;
+    spill_cap_[l] = ncap;
+  }
+  u64* stack = (u64*)c_->lane_stack[l].p;
+  u64* sp = (u64*)c_->lane_spill[l].p;
+  HIPCHK(c_, hipMemcpyAsync(sp + spill_n_[l] * stride, stack, move * sb, hipMemcpyDeviceToDevice, c_->stream));
+  // [move, top) -> [0, top - move): top - move <= move, so the ranges do not overlap
+  if (top_[l] > move)
+    HIPCHK(c_, hipMemcpyAsync(stack, stack + move * stride, (top_[l] - move) * sb, hipMemcpyDeviceToDevice,
+                              c_->stream));
+  spill_n_[l] += move;
+  top_[l] -= move;
+  ++rs_.spills;
+  return GMK_OK;
+}
+
+// Brings spilled states back into an empty stack (at most half its capacity).
+int BBEngine::unspill(int l) {
+  const u64 stride = 1 + 6ull * (1u << l), sb = stride * 8;
+  const u64 m = std::min<u64>(spill_n_[l], cap_[l] / 2);
+  HIPCHK(c_, hipMemcpyAsync(c_->lane_stack[l].p, (u64*)c_->lane_spill[l].p + (spill_n_[l] - m) * stride, m * sb,
+                            hipMemcpyDeviceToDevice, c_->stream));
+  spill_n_[l] -= m;
+  top_[l] = m;
+  return GMK_OK;
 }
 
 int BBEngine::lane_alloc(int l) {
@@ -151,7 +199,14 @@ int BBEngine::lane_alloc(int l) {
   // (HBM3E holds 288 GB: wide searches branch on up to 4096 candidates); the
   // children buffer is half of it
   const u64 bytes = W <= 4 ? (4ull << 30) : W <= 16 ? (8ull << 30) : (16ull << 30);
-  const u64 cap = std::max<u64>(1 << 17, bytes / state_bytes);
+  u64 cap = std::max<u64>(1 << 17, bytes / state_bytes);
+  // GMK_STACK_STATES caps every stack (at least 4096 states, and four times
+  // the most children one state can have): small stacks exercise the spill
+  // path on small graphs
+  if (const char* e = getenv("GMK_STACK_STATES")) {
+    const u64 v = strtoull(e, nullptr, 10);
+    if (v >= 4096) cap = std::min(cap, std::max<u64>(v, 4 * (2ull * 64 * W + 2)));
+  }
   int r;
   if ((r = ensure(c_, c_->lane_stack[l], cap * state_bytes))) return r;
   if ((r = ensure(c_, c_->lane_next[l], cap / 2 * state_bytes))) return r;
@@ -252,13 +307,21 @@ int BBEngine::step() {
   HIPCHK(c_, hipEventRecord(c_->ev0, s));
   int launched = 0;
   for (int l = 0; l < NLANE; ++l) {
+    if (!top_[l] && spill_n_[l] && (r = unspill(l))) return r;
+    // Spill when less than a quarter of the stack (and less than twice the most
+    // children one state can have) is free: the stack is then at least half
+    // free, which the override's minimum keeps above the most children of one
+    // state, so every step expands at least its top state.
+    if (top_[l] > 1 && cap_[l] - top_[l] < std::max<u64>(cap_[l] / 4, 2 * (2ull * 64 * (1u << l) + 2)) &&
+        (r = spill(l)))
+      return r;
     if (!top_[l]) continue;
     const int W = 1 << l;
     const u64 stride = 1 + 6ull * W;
     const u64 room = cap_[l] - top_[l];
     // Fan-out of the last batch (children per parent): the next batch takes no
     // more parents than the free stack can hold at that fan-out, so a search
-    // that branches widely drifts toward depth-first instead of overflowing.
+    // that branches widely drifts toward depth-first.
     const u64 by_room = (u64)((double)room / (2.0 * std::max(fanout_[l], 1.0)));
     const u64 bmax = std::min<u64>(kBatchMax, c_->tune.batch_states[l]);
     B[l] = (u32)std::max<u64>(1, std::min<u64>({top_[l], bmax, by_room}));

@@ -77,7 +77,7 @@ extern "C" void gmk_destroy(gmk_ctx* c) {
                         &c->rowpids})
     if (v->p) (void)hipFree(v->p);
   for (int l = 0; l < 7; ++l)
-    for (DeviceVec* v : {&c->lane_stack[l], &c->lane_next[l], &c->lane_retry[l]})
+    for (DeviceVec* v : {&c->lane_stack[l], &c->lane_next[l], &c->lane_retry[l], &c->lane_spill[l]})
       if (v->p) (void)hipFree(v->p);
   if (c->d_off) (void)hipFree(c->d_off);
   if (c->d_adj) (void)hipFree(c->d_adj);
@@ -312,6 +312,7 @@ extern "C" int gmk_enumerate(gmk_ctx* c, const gmk_params* prm, gmk_stats* st) {
   st->fp_xor ^= rs.fp_xor_extra;
   c->res_words = rs.keep ? std::min<u64>(h.res_words, rs.res_cap) : 0;
   st->launches = rs.launches;
+  st->spills = rs.spills;
   st->ms_expand = rs.ms_expand;
   st->hit_limit = rs.hit_limit || h.res_count >= rs.limit;
   st->ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -449,6 +450,7 @@ extern "C" int gmk_enumerate_batch(gmk_ctx* c, const gmk_graph_desc* graphs, uin
     st->launches_kernel[q] = rs.launches_kernel[q];
   }
   st->launches = rs.launches;
+  st->spills = rs.spills;
   st->ms_expand = rs.ms_expand;
   st->ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   c->prog_count = std::move(rs.prog_count);

@@ -31,6 +31,7 @@ struct RunState {
   u64 launches = 0;
   bool hit_limit = false;
   u64 fp_sum_extra = 0, fp_xor_extra = 0;
+  u32 spills = 0;
   double ms_kernel[4] = {0, 0, 0, 0};
   u64 launches_kernel[4] = {0, 0, 0, 0};
   // progress log: results so far and elapsed ms after each launch
@@ -96,6 +97,7 @@ struct gmk_ctx {
   u32* d_adj = nullptr;
   // per-run buffers, grown on demand and reused
   DeviceVec lane_stack[7], lane_next[7], lane_retry[7];  // state stacks per bitset width
+  DeviceVec lane_spill[7];  // shallow states moved out of a nearly full stack
   DeviceVec rows, maps, parts, res, counters;
   DeviceVec tuples, arena, fams, prefix, tcnt;  // thin-result kernels
   DeviceVec dedup;                              // result fingerprint table
@@ -192,7 +194,11 @@ class BBEngine {
   u64 staged_roots_ = 0;
   u64 top_[7] = {0, 0, 0, 0, 0, 0, 0};
   u64 cap_[7] = {0, 0, 0, 0, 0, 0, 0};
-  double fanout_[7] = {2, 2, 2, 2, 2, 2, 2};
+  double fanout_[7] = {2, 2, 2, 2, 2, 2, 2};  // children per parent in the last batch, smoothed
+  u64 spill_n_[7] = {0, 0, 0, 0, 0, 0, 0};    // states in the spill buffer per width
+  u64 spill_cap_[7] = {0, 0, 0, 0, 0, 0, 0};  // its capacity in states
+  int spill(int l);
+  int unspill(int l);
 };
 }  // namespace gmk
 

@@ -213,10 +213,15 @@ GMK_HD bool addable(const Graph& g, u32 w, const u32* freel, u32 nfree, const u3
       if (!contains(nw, dw, freel[i], cost) && ++miss > k) return false;
     cost.bytes += 4ull * nfree;
   } else {
-    // count free vertices adjacent to w by scanning N(w)
+    // count free vertices adjacent to w by scanning N(w); stop once the rest of
+    // N(w) cannot bring the misses back to k
+    const u32 need = nfree - (k - miss);  // hits required (miss <= k here)
     u32 hit = 0;
+    for (u32 i = 0; i < dw; ++i) {
+      if (hit + (dw - i) < need) { cost.bytes += 4ull * i; return false; }
+      hit += contains(freel, nfree, nw[i], cost) ? 1 : 0;
+    }
     cost.bytes += 4ull * dw;
-    for (u32 i = 0; i < dw; ++i) hit += contains(freel, nfree, nw[i], cost) ? 1 : 0;
     miss += nfree - hit;
     if (miss > k) return false;
   }

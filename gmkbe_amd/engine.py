@@ -38,7 +38,7 @@ class Stats(ctypes.Structure):
                 ("states", ctypes.c_uint64), ("intersections", ctypes.c_uint64), ("algo_bytes", ctypes.c_uint64),
                 ("partitions", ctypes.c_uint64), ("launches", ctypes.c_uint64), ("dedup_inserts", ctypes.c_uint64),
                 ("dedup_probes", ctypes.c_uint64), ("dedup_dups", ctypes.c_uint64), ("ms", ctypes.c_double),
-                ("ms_expand", ctypes.c_double), ("hit_limit", ctypes.c_int32), ("pad", ctypes.c_int32),
+                ("ms_expand", ctypes.c_double), ("hit_limit", ctypes.c_int32), ("spills", ctypes.c_uint32),
                 ("ms_kernel", ctypes.c_double * 4), ("launches_kernel", ctypes.c_uint64 * 4),
                 ("bytes_kernel", ctypes.c_uint64 * 4)]
 

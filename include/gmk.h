@@ -75,7 +75,7 @@ typedef struct {
   double ms;               /* enumeration window (after graph upload) */
   double ms_expand;        /* device time of all enumeration kernels */
   int32_t hit_limit;
-  int32_t pad;
+  uint32_t spills;         /* times a state stack over 3/4 full moved its shallow half to the spill buffer */
   /* per kernel: 0 state expansion, 1 thin family construction,
    * 2 thin combination check, 3 partition construction */
   double ms_kernel[4];       /* device time (HIP events around each launch) */
