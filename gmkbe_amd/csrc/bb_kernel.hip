@@ -43,6 +43,7 @@ struct __align__(16) Smem {
   uint16_t deg[2][64 * W];
   uint8_t ms[2][64 * W];
   uint8_t widx[W];         // words of the opposite side that the current pass reads
+  u64 rm[2][W];            // candidates removed by the degree pruning of this state
 };
 
 // Lists in sm.widx the words w < wb with mask_a[w] | mask_b[w] != 0; returns
@@ -102,9 +103,10 @@ __device__ void side_stats(Smem<W>& sm, const PartDesc& P, const u64* rows, int 
   __builtin_amdgcn_wave_barrier();  // widx is rewritten by the next pass
 }
 
-// Recompute deg only (after C changed).
+// deg of the members of side a after the vertices rm_b left C_b:
+// deg -= |N(v) & rm_b|, read over the (few) words rm_b occupies.
 template <int W>
-__device__ void side_deg(Smem<W>& sm, const PartDesc& P, const u64* rows, int a, Stats& st) {
+__device__ void side_deg_sub(Smem<W>& sm, const PartDesc& P, const u64* rows, int a, Stats& st) {
   const int lane = threadIdx.x;
   const u32 na = a ? P.nr : P.nl;
   const u32 wa = a ? P.wr : P.wl;
@@ -113,24 +115,24 @@ __device__ void side_deg(Smem<W>& sm, const PartDesc& P, const u64* rows, int a,
   const u64* Sa = setp(sm.set, a, 0, W);
   const u64* Ca = setp(sm.set, a, 1, W);
   const u64* Xa = setp(sm.set, a, 2, W);
-  const u64* Cb = setp(sm.set, a ^ 1, 1, W);
-  const u32 nw = word_list<W>(sm, Cb, nullptr, wb);
+  const u64* Rb = sm.rm[a ^ 1];
+  const u32 nw = word_list<W>(sm, Rb, nullptr, wb);
+  if (nw == 0) return;
   u32 tests = 0;
   for (u32 i = 0; i < wa; ++i) {
     const u64 memb = Sa[i] | Ca[i] | Xa[i];
     if (memb == 0) continue;
     const u32 v = i * 64 + lane;
-    const bool m = v < na && ((memb >> lane) & 1ull);
-    u32 d = 0;
-    if (m) {
+    if (v < na && ((memb >> lane) & 1ull)) {
       const u64* row = rowsA + (u64)v * wb;
+      u32 d = 0;
       for (u32 q = 0; q < nw; ++q) {
         const u32 w = sm.widx[q];
-        d += __popcll(row[w] & Cb[w]);
+        d += __popcll(row[w] & Rb[w]);
       }
+      sm.deg[a][v] = (uint16_t)(sm.deg[a][v] - d);
       ++tests;
     }
-    sm.deg[a][v] = (uint16_t)d;
   }
   st.intersections += tests;
   st.bytes += (u64)tests * nw * 8;
@@ -323,6 +325,8 @@ __device__ void expand_one(Smem<W>& sm, const ExpandArgs& a, u32 sidx, Stats& st
   // degrees before any removal; removal only lowers degrees, so every removed
   // vertex is one the sequential pass would also be allowed to remove.
   bool pruned_any = false;
+  if (lane < W) { sm.rm[0][lane] = 0; sm.rm[1][lane] = 0; }
+  wave_barrier();
   for (int a_ = 0; a_ < 2; ++a_) {
     const u32 na = a_ ? P.nr : P.nl, wa = a_ ? P.wr : P.wl;
     u64* Ca = setp(sm.set, a_, 1, W);
@@ -337,7 +341,7 @@ __device__ void expand_one(Smem<W>& sm, const ExpandArgs& a, u32 sidx, Stats& st
       const u64 rmw = __ballot(rm);
       wave_barrier();
       if (rmw) {
-        if (lane == 0) Ca[i] &= ~rmw;
+        if (lane == 0) { Ca[i] &= ~rmw; sm.rm[a_][i] = rmw; }
         pruned_any = true;
       }
       wave_barrier();
@@ -345,8 +349,8 @@ __device__ void expand_one(Smem<W>& sm, const ExpandArgs& a, u32 sidx, Stats& st
   }
   if (pruned_any) {
     count_sets();
-    side_deg<W>(sm, P, a.rows, 0, st);
-    side_deg<W>(sm, P, a.rows, 1, st);
+    side_deg_sub<W>(sm, P, a.rows, 0, st);
+    side_deg_sub<W>(sm, P, a.rows, 1, st);
     wave_barrier();
   }
 
