@@ -1,6 +1,6 @@
 // Host side of the MI355X maximal k-biplex enumerator: context, graph upload,
-// partition construction, the batch loop that drains the device state stack, and
-// the C ABI declared in include/gmk.h.
+// tuning, the single-graph and many-graph entry points of the C ABI declared in
+// include/gmk.h (the state-stack engine is in bb_run.hip).
 #include "gmk_device.h"
 #include "gmk_host.h"
 
@@ -14,10 +14,6 @@
 #include "../../include/gmk.h"
 
 using namespace gmk;
-
-namespace gmk {
-hipError_t launch_bb_expand(int W, u32 n_states, hipStream_t s, const ExpandArgs& a, u32 persistent_waves);
-}
 
 extern "C" int gmk_create(int device, gmk_ctx** out) {
   if (!out) return GMK_EINVAL;

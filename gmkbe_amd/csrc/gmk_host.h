@@ -12,7 +12,7 @@ namespace gmk {
 
 // Partitions of one run, with their root search states.
 struct PartitionSet {
-  int W = 0;                     // bitset words per set (1, 2, 4, 8 or 16)
+  int W = 0;                     // bitset words per set (1, 2, 4, ..., 64)
   std::vector<PartDesc> parts;
   std::vector<u64> rows;         // row pool (bitset adjacency)
   std::vector<u32> maps;         // local -> global ID maps
