@@ -99,7 +99,7 @@ struct gmk_ctx {
   DeviceVec lane_stack[7], lane_next[7], lane_retry[7];  // state stacks per bitset width
   DeviceVec lane_spill[7];  // shallow states moved out of a nearly full stack
   DeviceVec rows, maps, parts, res, counters;
-  DeviceVec tuples, arena, fams, prefix, tcnt;  // thin-result kernels
+  DeviceVec tuples, arena, fams, prefix, tcnt, tdefer;  // thin-result kernels
   DeviceVec dedup;                              // result fingerprint table
   DeviceVec gcnt;                               // per-graph counters of a batch
   std::vector<u32> order;                        // degeneracy order (seeded runs)

@@ -228,14 +228,39 @@ GMK_HD bool addable(const Graph& g, u32 w, const u32* freel, u32 nfree, const u3
   return true;
 }
 
-// Returns true iff (L, R) is maximal on side A.  fpick holds min(k + 1, nfree)
-// free vertices (free_picks): with no saturated member, an addable w misses at
-// most k members of R, so it is adjacent to one of any k + 1 members; the k + 1
-// members of least degree among the chosen vertices and fpick give the
-// candidates.
+// Candidates for a vertex that could join (L, R): up to k + 1 lists of
+// vertices of side A (neighbour lists), each with a member already known to be
+// adjacent to its entries, and optionally a vertex every candidate must be
+// adjacent to.  `decided` >= 0 when no candidate is needed (1 = maximal).
+struct CandSegs {
+  const u32* lst[4];
+  u32 len[4];
+  u32 known[4];
+  u32 n;
+  u32 must;          // candidates must be adjacent to this vertex (NONE: no filter)
+  const u32* must_nb;
+  u32 must_deg;
+  int decided;
+};
+
+// The candidate lists of (L, R).  fpick holds min(k + 1, nfree) free vertices
+// (free_picks).
+//  - With a saturated member: an addable w is adjacent to the saturated member
+//    `best` of least degree and to all but k members of R, so it neighbours one
+//    of any k + 1 other members.  When the k + 1 other members of least degree
+//    have fewer neighbours in all than `best`, their lists give the candidates
+//    (kept if adjacent to `best`); otherwise N(best) does.
+//  - With none: an addable w misses at most k members of R, so it is adjacent
+//    to one of any k + 1 members; the k + 1 members of least degree among the
+//    chosen vertices and fpick give the candidates.
 template <int NMAX>
-GMK_HD bool side_a_maximal(const Graph& g, const u32* L, u32 j, const u32* freel, u32 nfree, const u32* fpick,
-                           u32 nfpick, const u32* ch, const u32* ctsz, u32 nch, u32 k, Cost& cost) {
+GMK_HD void candidate_segments(const Graph& g, u32 j, u32 nfree, const u32* fpick, u32 nfpick, const u32* ch,
+                               const u32* ctsz, u32 nch, u32 k, CandSegs& cs, Cost& cost) {
+  cs.n = 0;
+  cs.must = 0xFFFFFFFFu;
+  cs.must_nb = nullptr;
+  cs.must_deg = 0;
+  cs.decided = -1;
   const u32 rs = nfree + nch;
   // saturated chosen vertex with the smallest degree
   u32 best = 0xFFFFFFFFu, bestdeg = 0xFFFFFFFFu;
@@ -248,79 +273,75 @@ GMK_HD bool side_a_maximal(const Graph& g, const u32* L, u32 j, const u32* freel
       if (d < bestdeg) { bestdeg = d; best = ch[i]; }
     }
   }
-  if (best != 0xFFFFFFFFu) {
-    // An addable w is adjacent to `best` (saturated) and to all but at most k
-    // members of R, so it is a neighbour of one of any k + 1 other members.
-    // When the k + 1 other members of least degree have fewer neighbours in
-    // all than `best`, their lists give the candidates (kept if adjacent to
-    // `best`); otherwise N(best) does.
-    u32 pk[4], pd[4];
-    u32 np = 0;
-    auto take = [&](u32 y) {
-      const u32 d = g.off[y + 1] - g.off[y];
-      cost.bytes += 8;
-      if (np < k + 1) { pk[np] = y; pd[np] = d; ++np; return; }
-      u32 worst = 0;
-      for (u32 q = 1; q < np; ++q) if (pd[q] > pd[worst]) worst = q;
-      if (d < pd[worst]) { pk[worst] = y; pd[worst] = d; }
-    };
-#pragma unroll
-    for (int i = 0; i < NMAX; ++i)
-      if ((u32)i < nch && ch[i] != best) take(ch[i]);
-    for (u32 i = 0; i < nfpick; ++i) take(fpick[i]);
-    u64 psum = 0;
-    for (u32 q = 0; q < np; ++q) psum += pd[q];
-    if (np == k + 1 && psum < bestdeg) {
-      const u32* nb = g.adj + g.off[best];
-      for (u32 q = 0; q < np; ++q)
-        for (u32 e = g.off[pk[q]]; e < g.off[pk[q] + 1]; ++e) {
-          const u32 w = g.adj[e];
-          cost.bytes += 4;
-          if (in_tuple(L, j, w) || !contains(nb, bestdeg, w, cost)) continue;
-          if (addable<NMAX>(g, w, freel, nfree, ch, ctsz, nch, k, pk[q], cost)) return false;
-        }
-      return true;
-    }
-    for (u32 e = g.off[best]; e < g.off[best + 1]; ++e) {
-      const u32 w = g.adj[e];
-      cost.bytes += 4;
-      if (in_tuple(L, j, w)) continue;
-      if (addable<NMAX>(g, w, freel, nfree, ch, ctsz, nch, k, best, cost)) return false;
-    }
-    return true;
-  }
-  if (rs <= k) {
+  if (best == 0xFFFFFFFFu && rs <= k) {
     // every vertex of side A outside L can be added
-    return (u64)(g.a_hi - g.a_lo) <= j;
+    cs.decided = (u64)(g.a_hi - g.a_lo) <= j ? 1 : 0;
+    return;
   }
-  // neighbours of the k+1 lowest-degree members of R
-  u32 pick[4];
-  u32 pdeg[4];
+  // the k + 1 members of least degree (excluding `best`)
+  u32 pk[4], pd[4];
   u32 np = 0;
-  auto consider = [&](u32 y) {
+  auto take = [&](u32 y) {
     const u32 d = g.off[y + 1] - g.off[y];
     cost.bytes += 8;
-    if (np < k + 1) { pick[np] = y; pdeg[np] = d; ++np; }
-    else {
-      u32 worst = 0;
-      for (u32 q = 1; q < np; ++q) if (pdeg[q] > pdeg[worst]) worst = q;
-      if (d < pdeg[worst]) { pick[worst] = y; pdeg[worst] = d; }
-    }
+    if (np < k + 1) { pk[np] = y; pd[np] = d; ++np; return; }
+    u32 worst = 0;
+    for (u32 q = 1; q < np; ++q) if (pd[q] > pd[worst]) worst = q;
+    if (d < pd[worst]) { pk[worst] = y; pd[worst] = d; }
   };
 #pragma unroll
-  for (int i = 0; i < NMAX; ++i) {
-    if ((u32)i >= nch) break;
-    consider(ch[i]);
+  for (int i = 0; i < NMAX; ++i)
+    if ((u32)i < nch && ch[i] != best) take(ch[i]);
+  for (u32 i = 0; i < nfpick; ++i) take(fpick[i]);
+  u64 psum = 0;
+  for (u32 q = 0; q < np; ++q) psum += pd[q];
+  if (best != 0xFFFFFFFFu && !(np == k + 1 && psum < bestdeg)) {
+    cs.lst[0] = g.adj + g.off[best];
+    cs.len[0] = bestdeg;
+    cs.known[0] = best;
+    cs.n = 1;
+    return;
   }
-  for (u32 i = 0; i < nfpick; ++i) consider(fpick[i]);
-  for (u32 q = 0; q < np; ++q)
-    for (u32 e = g.off[pick[q]]; e < g.off[pick[q] + 1]; ++e) {
-      const u32 w = g.adj[e];
+  for (u32 q = 0; q < np; ++q) {
+    cs.lst[q] = g.adj + g.off[pk[q]];
+    cs.len[q] = pd[q];
+    cs.known[q] = pk[q];
+  }
+  cs.n = np;
+  if (best != 0xFFFFFFFFu) {
+    cs.must = best;
+    cs.must_nb = g.adj + g.off[best];
+    cs.must_deg = bestdeg;
+  }
+}
+
+// Can candidate w (from list `seg`) be added?
+template <int NMAX>
+GMK_HD bool candidate_joins(const Graph& g, const u32* L, u32 j, const u32* freel, u32 nfree, const u32* ch,
+                            const u32* ctsz, u32 nch, u32 k, const CandSegs& cs, u32 seg, u32 w, Cost& cost) {
+  if (in_tuple(L, j, w)) return false;
+  if (cs.must != 0xFFFFFFFFu && !contains(cs.must_nb, cs.must_deg, w, cost)) return false;
+  return addable<NMAX>(g, w, freel, nfree, ch, ctsz, nch, k, cs.known[seg], cost);
+}
+
+// 1 if (L, R) is maximal on side A, 0 if not; -1 when the first `budget`
+// candidates (in list order) cannot join and more remain: the caller tests the
+// rest another way.
+template <int NMAX>
+GMK_HD int side_a_maximal(const Graph& g, const u32* L, u32 j, const u32* freel, u32 nfree, const u32* fpick,
+                          u32 nfpick, const u32* ch, const u32* ctsz, u32 nch, u32 k, Cost& cost,
+                          u32 budget = 0xFFFFFFFFu) {
+  CandSegs cs;
+  candidate_segments<NMAX>(g, j, nfree, fpick, nfpick, ch, ctsz, nch, k, cs, cost);
+  if (cs.decided >= 0) return cs.decided;
+  u32 tested = 0;
+  for (u32 q = 0; q < cs.n; ++q)
+    for (u32 e = 0; e < cs.len[q]; ++e) {
+      if (tested++ == budget) return -1;
       cost.bytes += 4;
-      if (in_tuple(L, j, w)) continue;
-      if (addable<NMAX>(g, w, freel, nfree, ch, ctsz, nch, k, 0xFFFFFFFFu, cost)) return false;
+      if (candidate_joins<NMAX>(g, L, j, freel, nfree, ch, ctsz, nch, k, cs, q, cs.lst[q][e], cost)) return 0;
     }
-  return true;
+  return 1;
 }
 
 }  // namespace thin

@@ -20,6 +20,10 @@ struct ThinArgs {
   u32* fam_n;
   u32* tflags;          // family-construction overflow flags
   u32* work;            // chunk counter of a check launch (zeroed before it)
+  u32 budget;           // candidates a lane tests before deferring the rest of a rank
+  u64* defer;           // deferred ranks of a check launch (capacity: the launch's ranks)
+  u32* defer_n;         // their number (zeroed before the launch)
+  u32* dwork;           // next deferred rank to take (zeroed before the launch)
   u32* flags;           // run flags (limit, result overflow)
   // T2
   const u64* prefix;    // nfam + 1 prefix sums of family totals
@@ -39,6 +43,7 @@ struct ThinArgs {
 
 hipError_t launch_thin_build(const ThinArgs& a, hipStream_t s);
 hipError_t launch_thin_check(const ThinArgs& a, hipStream_t s, u32 max_blocks, bool chunked);
+hipError_t launch_thin_check_deferred(const ThinArgs& a, hipStream_t s, u32 blocks);
 hipError_t launch_thin_scan(const thin::Family* fams, u32 nfam, u64* prefix, hipStream_t s);
 
 }  // namespace gmk

@@ -17,13 +17,14 @@ namespace {
 struct ThinCounters {
   u32 fam_n, tflags;
   u64 arena_used;
-  u32 work, pad;  // chunk counter of the check kernel
+  u32 work, defer_n, dwork, pad;  // check kernels: chunk counter, deferred ranks, next deferred rank
 };
 
 constexpr u32 TCH = 1u << 16;          // tuples per family-construction launch
 constexpr u64 ARENA = 1ull << 27;      // arena words (512 MiB)
 constexpr u32 FCAP = 1u << 18;         // families per launch
 constexpr u64 RCH = 1ull << 22;        // ranks per check launch
+constexpr u32 DEFER_BUDGET = 64;       // candidates one lane tests before a rank goes to a whole wave
 
 struct ThinRun {
   gmk_ctx* c;
@@ -76,9 +77,11 @@ int run_chunk(ThinRun& T, const u32* tuples, u32 count, u32 j) {
   for (u64 r0 = 0; r0 < total; r0 += RCH) {
     a.r0 = r0;
     a.r1 = std::min(total, r0 + RCH);
-    HIPCHK(c, hipMemsetAsync(&((ThinCounters*)c->tcnt.p)->work, 0, sizeof(u32), s));
+    // work, defer_n and dwork are consecutive
+    HIPCHK(c, hipMemsetAsync(&((ThinCounters*)c->tcnt.p)->work, 0, 3 * sizeof(u32), s));
     HIPCHK(c, hipEventRecord(c->ev0, s));
     HIPCHK(c, launch_thin_check(a, s, c->n_cu * 8, c->tune.check_chunked[a.k < 4 ? a.k : 3] != 0));
+    HIPCHK(c, launch_thin_check_deferred(a, s, c->n_cu * 4));
     HIPCHK(c, hipEventRecord(c->ev1, s));
     HIPCHK(c, hipMemcpyAsync(c->h_cnt, dc, offsetof(RunCounters, fp), hipMemcpyDeviceToHost, s));
     HIPCHK(c, hipStreamSynchronize(s));
@@ -111,6 +114,7 @@ int gmk_run_thin(gmk_ctx* c, const gmk_params* prm, gmk_stats* st, RunState& rs,
   if ((r = ensure(c, c->fams, (size_t)FCAP * sizeof(Family)))) return r;
   if ((r = ensure(c, c->prefix, (size_t)(FCAP + 1) * 8))) return r;
   if ((r = ensure(c, c->tcnt, sizeof(ThinCounters)))) return r;
+  if ((r = ensure(c, c->tdefer, RCH * 8))) return r;
   ThinRun T;
   T.c = c; T.prm = prm; T.rs = &rs;
   ThinArgs& a = T.a;
@@ -130,6 +134,14 @@ int gmk_run_thin(gmk_ctx* c, const gmk_params* prm, gmk_stats* st, RunState& rs,
   a.fam_n = &tc->fam_n;
   a.tflags = &tc->tflags;
   a.work = &tc->work;
+  a.budget = DEFER_BUDGET;
+  // GMK_CHECK_BUDGET sets the candidates one lane tests before deferring (0
+  // defers every open rank): small budgets exercise the deferred pass on
+  // small graphs
+  if (const char* e = getenv("GMK_CHECK_BUDGET")) a.budget = (u32)std::min<unsigned long>(strtoul(e, nullptr, 10), 1u << 20);
+  a.defer = (u64*)c->tdefer.p;
+  a.defer_n = &tc->defer_n;
+  a.dwork = &tc->dwork;
   RunCounters* dc = (RunCounters*)c->counters.p;
   a.flags = &dc->flags;
   a.prefix = (const u64*)c->prefix.p;

@@ -213,114 +213,6 @@ __global__ __launch_bounds__(1024) void thin_scan_totals(const thin::Family* fam
   if (nfam == 0 && t == 0) prefix[0] = 0;
 }
 
-// Checks rank r (one lane): unrank its combination, test maximality, emit.
-// K: misses allowed per vertex (1..3).  A family chooses at most 2 K^2 vertices
-// besides its free ones (each costs budget of at least one of <= 2K members),
-// so the chosen vertices live in registers: every access uses a constant index.
-template <int K>
-__device__ __forceinline__ void check_rank(const ThinArgs& a, u64 r, thin::Cost& cost, u64& n_emit, u64& n_tested) {
-  constexpr int NM = 2 * K * K;
-  bool emit = false;
-  u32 ch[NM], ctsz[NM];
-#pragma unroll
-  for (int q = 0; q < NM; ++q) { ch[q] = 0; ctsz[q] = 0; }
-  u32 nch = 0;
-  const thin::Family* f = nullptr;
-  if (r < a.r1) {
-    // family containing rank r
-    u32 lo = 0, hi = a.nfam;
-    while (hi - lo > 1) {
-      const u32 mid = (lo + hi) >> 1;
-      if (a.prefix[mid] <= r) lo = mid; else hi = mid;
-    }
-    f = a.fams + lo;
-    u64 rr = r - a.prefix[lo];
-    for (u32 t = 0; t < f->ntypes; ++t) {
-      const u64 radix = thin::binom(f->psize[t], f->cnt[t]);
-      const u64 rt = rr % radix;
-      rr /= radix;
-      u32 idx[3];
-      thin::unrank(rt, f->cnt[t], f->psize[t], idx);
-      const u32 tsz = thin::popc(f->mask[t]);
-      for (u32 q = 0; q < f->cnt[t]; ++q) {
-        const u32 y = thin::pool_member(*f, t, idx[q], a.g.b_lo, cost);
-#pragma unroll
-        for (int s2 = 0; s2 < NM; ++s2)
-          if ((u32)s2 == nch) { ch[s2] = y; ctsz[s2] = tsz; }
-        ++nch;
-      }
-    }
-    cost.bytes += 8ull * 8 + 16ull * f->ntypes;  // prefix probes, family header and type fields
-    emit = thin::side_a_maximal<NM>(a.g, f->L, f->j, f->freep, f->nfree, f->fpick, f->nfpick, ch, ctsz, nch, a.k,
-                                    cost);
-  }
-  // per-lane emission (records are independent)
-  u64 h = 0;
-  u64 fpv = 0;
-  if (emit) {
-    const u32 nr0 = f->nfree + nch;
-    const u32* fr0 = f->freep;
-    for (u32 i = 0; i < f->j; ++i) h += mix64(f->L[i]);
-    for (u32 i = 0; i < f->nfree; ++i) h += mix64(fr0[i]);
-#pragma unroll
-    for (int i = 0; i < NM; ++i)
-      if ((u32)i < nch) h += mix64(ch[i]);
-    cost.bytes += 4ull * f->nfree;
-    fpv = fmix64(h + (u64)(a.swap_sides ? nr0 : f->j) * FP_A + (u64)(a.swap_sides ? f->j : nr0) * FP_B);
-    if (!dedup_insert(a.dedup, fpv, a.flags, blockIdx.x)) emit = false;
-  }
-  if (emit) {
-    const u64 slot = atomicAdd((unsigned long long*)a.res_count, 1ull);
-    if (slot >= a.limit) {
-      atomicOr(a.flags, FLAG_LIMIT);
-      emit = false;
-    } else {
-      if (slot + 1 == a.limit) atomicOr(a.flags, FLAG_LIMIT);
-      // sort the chosen vertices (few) for the merge with the free list:
-      // odd-even transposition over the first nch slots
-#pragma unroll
-      for (int pass = 0; pass < NM; ++pass)
-#pragma unroll
-        for (int q = pass & 1; q + 1 < NM; q += 2)
-          if ((u32)(q + 1) < nch && ch[q] > ch[q + 1]) { const u32 tmp = ch[q]; ch[q] = ch[q + 1]; ch[q + 1] = tmp; }
-      const u32 nr = f->nfree + nch;
-      const u32* fr = f->freep;
-      const u32 nl = f->j;
-      u64* fs = a.fp + ((blockIdx.x * 4 + (threadIdx.x >> 6)) % NSTRIPE) * 2;
-      atomicAdd((unsigned long long*)&fs[0], (unsigned long long)fpv);
-      atomicXor((unsigned long long*)&fs[1], (unsigned long long)fpv);
-      if (a.keep) {
-        const u64 len = 2ull + nl + nr;
-        cost.bytes += 4ull * len;
-        const u64 pos = atomicAdd((unsigned long long*)a.res_words, (unsigned long long)len);
-        if (pos + len > a.res_cap) {
-          atomicOr(a.flags, FLAG_RES_OVERFLOW);
-        } else {
-          // tuple side and generated side, left side first
-          u64 at_t = a.swap_sides ? pos + 1 + nr + 1 : pos + 1;
-          u64 at_r = a.swap_sides ? pos + 1 : pos + 1 + nl + 1;
-          a.res[a.swap_sides ? pos + 1 + nr : pos] = nl;
-          a.res[a.swap_sides ? pos : pos + 1 + nl] = nr;
-          for (u32 i = 0; i < nl; ++i) a.res[at_t + i] = f->L[i];
-          u32 p = 0, q = 0;
-          for (u32 i = 0; i < nr; ++i) {
-            u32 cq = 0xFFFFFFFFu;  // ch[q], or none
-#pragma unroll
-            for (int s2 = 0; s2 < NM; ++s2)
-              if ((u32)s2 == q && q < nch) cq = ch[s2];
-            u32 v;
-            if (q >= nch || (p < f->nfree && fr[p] < cq)) v = fr[p++];
-            else { v = cq; ++q; }
-            a.res[at_r + i] = v;
-          }
-        }
-      }
-    }
-  }
-  n_emit += emit ? 1 : 0;
-  n_tested += (r < a.r1 && f) ? 1 : 0;
-}
-
 template <int K>
 __device__ __forceinline__ void flush_check_stats(const ThinArgs& a, const thin::Cost& cost, u64 n_emit,
                                                   u64 n_tested) {
@@ -331,6 +223,179 @@ __device__ __forceinline__ void flush_check_stats(const ThinArgs& a, const thin:
     atomicAdd((unsigned long long*)&sp[2], (unsigned long long)se);
     atomicAdd((unsigned long long*)&sp[5], (unsigned long long)sb);
   }
+}
+
+// K: misses allowed per vertex (1..3).  A family chooses at most 2 K^2 vertices
+// besides its free ones (each costs budget of at least one of <= 2K members),
+// so the chosen vertices live in registers: every access uses a constant index.
+
+// Family of rank r and its chosen vertices (with their type sizes).
+template <int K>
+__device__ __forceinline__ const thin::Family* unrank_rank(const ThinArgs& a, u64 r, u32 (&ch)[2 * K * K],
+                                                          u32 (&ctsz)[2 * K * K], u32& nch, thin::Cost& cost) {
+  constexpr int NM = 2 * K * K;
+#pragma unroll
+  for (int q = 0; q < NM; ++q) { ch[q] = 0; ctsz[q] = 0; }
+  nch = 0;
+  u32 lo = 0, hi = a.nfam;
+  while (hi - lo > 1) {
+    const u32 mid = (lo + hi) >> 1;
+    if (a.prefix[mid] <= r) lo = mid; else hi = mid;
+  }
+  const thin::Family* f = a.fams + lo;
+  u64 rr = r - a.prefix[lo];
+  for (u32 t = 0; t < f->ntypes; ++t) {
+    const u64 radix = thin::binom(f->psize[t], f->cnt[t]);
+    const u64 rt = rr % radix;
+    rr /= radix;
+    u32 idx[3];
+    thin::unrank(rt, f->cnt[t], f->psize[t], idx);
+    const u32 tsz = thin::popc(f->mask[t]);
+    for (u32 q = 0; q < f->cnt[t]; ++q) {
+      const u32 y = thin::pool_member(*f, t, idx[q], a.g.b_lo, cost);
+#pragma unroll
+      for (int s2 = 0; s2 < NM; ++s2)
+        if ((u32)s2 == nch) { ch[s2] = y; ctsz[s2] = tsz; }
+      ++nch;
+    }
+  }
+  cost.bytes += 8ull * 8 + 16ull * f->ntypes;  // prefix probes, family header and type fields
+  return f;
+}
+
+// Emits the maximal (L, R) of family f with chosen vertices ch (one lane):
+// fingerprint, dedup, result slot under the limit, record.  Returns whether it
+// was emitted.
+template <int K>
+__device__ __forceinline__ bool emit_rank(const ThinArgs& a, const thin::Family* f, u32 (&ch)[2 * K * K], u32 nch,
+                                          u32 stripe, thin::Cost& cost) {
+  constexpr int NM = 2 * K * K;
+  const u32 nr0 = f->nfree + nch;
+  const u32* fr0 = f->freep;
+  u64 h = 0;
+  for (u32 i = 0; i < f->j; ++i) h += mix64(f->L[i]);
+  for (u32 i = 0; i < f->nfree; ++i) h += mix64(fr0[i]);
+#pragma unroll
+  for (int i = 0; i < NM; ++i)
+    if ((u32)i < nch) h += mix64(ch[i]);
+  cost.bytes += 4ull * f->nfree;
+  const u64 fpv = fmix64(h + (u64)(a.swap_sides ? nr0 : f->j) * FP_A + (u64)(a.swap_sides ? f->j : nr0) * FP_B);
+  if (!dedup_insert(a.dedup, fpv, a.flags, stripe)) return false;
+  const u64 slot = atomicAdd((unsigned long long*)a.res_count, 1ull);
+  if (slot >= a.limit) {
+    atomicOr(a.flags, FLAG_LIMIT);
+    return false;
+  }
+  if (slot + 1 == a.limit) atomicOr(a.flags, FLAG_LIMIT);
+  // sort the chosen vertices (few) for the merge with the free list:
+  // odd-even transposition over the first nch slots
+#pragma unroll
+  for (int pass = 0; pass < NM; ++pass)
+#pragma unroll
+    for (int q = pass & 1; q + 1 < NM; q += 2)
+      if ((u32)(q + 1) < nch && ch[q] > ch[q + 1]) { const u32 tmp = ch[q]; ch[q] = ch[q + 1]; ch[q + 1] = tmp; }
+  const u32 nr = f->nfree + nch;
+  const u32* fr = f->freep;
+  const u32 nl = f->j;
+  u64* fs = a.fp + (stripe % NSTRIPE) * 2;
+  atomicAdd((unsigned long long*)&fs[0], (unsigned long long)fpv);
+  atomicXor((unsigned long long*)&fs[1], (unsigned long long)fpv);
+  if (a.keep) {
+    const u64 len = 2ull + nl + nr;
+    cost.bytes += 4ull * len;
+    const u64 pos = atomicAdd((unsigned long long*)a.res_words, (unsigned long long)len);
+    if (pos + len > a.res_cap) {
+      atomicOr(a.flags, FLAG_RES_OVERFLOW);
+    } else {
+      // tuple side and generated side, left side first
+      u64 at_t = a.swap_sides ? pos + 1 + nr + 1 : pos + 1;
+      u64 at_r = a.swap_sides ? pos + 1 : pos + 1 + nl + 1;
+      a.res[a.swap_sides ? pos + 1 + nr : pos] = nl;
+      a.res[a.swap_sides ? pos : pos + 1 + nl] = nr;
+      for (u32 i = 0; i < nl; ++i) a.res[at_t + i] = f->L[i];
+      u32 p = 0, q = 0;
+      for (u32 i = 0; i < nr; ++i) {
+        u32 cq = 0xFFFFFFFFu;  // ch[q], or none
+#pragma unroll
+        for (int s2 = 0; s2 < NM; ++s2)
+          if ((u32)s2 == q && q < nch) cq = ch[s2];
+        u32 v;
+        if (q >= nch || (p < f->nfree && fr[p] < cq)) v = fr[p++];
+        else { v = cq; ++q; }
+        a.res[at_r + i] = v;
+      }
+    }
+  }
+  return true;
+}
+
+// Checks rank r (one lane): unrank its combination, test maximality, emit.  A
+// rank still open after its first a.budget candidates goes to the deferred
+// list (thin_check_deferred tests the rest, a wave per rank).
+template <int K>
+__device__ __forceinline__ void check_rank(const ThinArgs& a, u64 r, thin::Cost& cost, u64& n_emit, u64& n_tested) {
+  constexpr int NM = 2 * K * K;
+  if (r >= a.r1) return;
+  u32 ch[NM], ctsz[NM];
+  u32 nch = 0;
+  const thin::Family* f = unrank_rank<K>(a, r, ch, ctsz, nch, cost);
+  const int m = thin::side_a_maximal<NM>(a.g, f->L, f->j, f->freep, f->nfree, f->fpick, f->nfpick, ch, ctsz, nch,
+                                         a.k, cost, a.budget);
+  if (m < 0) {
+    const u32 i = atomicAdd(a.defer_n, 1u);
+    a.defer[i] = r;  // the list holds a launch's ranks
+    return;
+  }
+  ++n_tested;
+  if (m == 1 && emit_rank<K>(a, f, ch, nch, blockIdx.x * 4 + (threadIdx.x >> 6), cost)) ++n_emit;
+}
+
+// Deferred ranks, one wave each: every lane unranks the combination, the lanes
+// split its candidates after the first a.budget (which the first pass tested),
+// and lane 0 emits it when none can join.
+// A fixed grid of waves takes deferred ranks from a device counter.
+template <int K>
+__global__ __launch_bounds__(256) void thin_check_deferred(ThinArgs a) {
+  constexpr int NM = 2 * K * K;
+  const int lane = threadIdx.x & 63;
+  u64 n_emit = 0, n_tested = 0;
+  thin::Cost cost;
+  const u32 nd = __hip_atomic_load(a.defer_n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (nd == 0) return;
+  for (;;) {
+    u32 i = 0;
+    if (lane == 0) i = atomicAdd(a.dwork, 1u);
+    i = (u32)wave_bcast_u64(i, 0);
+    if (i >= nd) break;
+    if (__hip_atomic_load(a.flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & FLAG_LIMIT) break;
+    const u64 r = a.defer[i];
+    u32 ch[NM], ctsz[NM];
+    u32 nch = 0;
+    const thin::Family* f = unrank_rank<K>(a, r, ch, ctsz, nch, cost);
+    thin::CandSegs cs;
+    thin::candidate_segments<NM>(a.g, f->j, f->nfree, f->fpick, f->nfpick, ch, ctsz, nch, a.k, cs, cost);
+    u64 total = 0;
+    for (u32 q = 0; q < cs.n; ++q) total += cs.len[q];
+    bool joins = false;
+    for (u64 base = a.budget; base < total; base += 64) {
+      const u64 idx = base + lane;
+      if (idx < total) {
+        u32 q = 0;
+        u64 o = idx;
+        while (o >= cs.len[q]) { o -= cs.len[q]; ++q; }
+        cost.bytes += 4;
+        joins = thin::candidate_joins<NM>(a.g, f->L, f->j, f->freep, f->nfree, ch, ctsz, nch, a.k, cs, q,
+                                          cs.lst[q][o], cost);
+      }
+      if (__ballot(joins)) break;
+    }
+    const bool maximal = cs.decided >= 0 ? cs.decided == 1 : !__ballot(joins);
+    if (lane == 0) {
+      ++n_tested;
+      if (maximal && emit_rank<K>(a, f, ch, nch, blockIdx.x * 4 + (threadIdx.x >> 6), cost)) ++n_emit;
+    }
+  }
+  flush_check_stats<K>(a, cost, n_emit, n_tested);
 }
 
 // One lane per rank over the whole grid.
@@ -389,6 +454,17 @@ hipError_t launch_thin_build(const ThinArgs& a, hipStream_t s) {
 
 hipError_t launch_thin_scan(const thin::Family* fams, u32 nfam, u64* prefix, hipStream_t s) {
   hipLaunchKernelGGL(thin_scan_totals, dim3(1), dim3(1024), 0, s, fams, nfam, prefix);
+  return hipGetLastError();
+}
+
+hipError_t launch_thin_check_deferred(const ThinArgs& a, hipStream_t s, u32 blocks) {
+  const dim3 g(blocks), b(256);
+  switch (a.k) {
+    case 1: hipLaunchKernelGGL(thin_check_deferred<1>, g, b, 0, s, a); break;
+    case 2: hipLaunchKernelGGL(thin_check_deferred<2>, g, b, 0, s, a); break;
+    case 3: hipLaunchKernelGGL(thin_check_deferred<3>, g, b, 0, s, a); break;
+    default: return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 

@@ -65,7 +65,7 @@ extern "C" long long thin_model(unsigned n, unsigned n_left, const unsigned* off
                 ++nch;
               }
             }
-            if (!side_a_maximal<MAXJ * 3>(g, f.L, f.j, f.freep, f.nfree, f.fpick, f.nfpick, ch, ct, nch, k, cost))
+            if (side_a_maximal<MAXJ * 3>(g, f.L, f.j, f.freep, f.nfree, f.fpick, f.nfpick, ch, ct, nch, k, cost) != 1)
               continue;
             std::vector<u32> R(f.freep, f.freep + f.nfree);
             R.insert(R.end(), ch, ch + nch);

@@ -1,0 +1,47 @@
+"""The thin checks' deferred pass gives the same result sets.
+
+A lane tests a combination's first candidates itself; one still open after
+GMK_CHECK_BUDGET of them goes to a second kernel in which a whole wave tests
+the rest.  Budgets 0 (every open combination deferred, the wave tests all its
+candidates), 1 and 3 push most combinations of the golden graphs through that
+second kernel: results must still equal the reference's canonical sets."""
+import os
+
+import pytest
+
+import golden_cases as GC
+import oracle_lib as O
+from gmkbe_amd.engine import Engine
+
+pytestmark = pytest.mark.gpu
+
+# (graph, k, theta): cases with results of at most 2k vertices on one side
+CASES = [("er12x10", 3, 1), ("er12x10", 2, 2), ("southern_women", 2, 3), ("er40x50", 1, 1),
+         ("moreno_crime", 1, 2), ("er200x300", 1, 1)]
+
+
+@pytest.fixture
+def budget():
+    old = os.environ.get("GMK_CHECK_BUDGET")
+
+    def set_budget(v):
+        os.environ["GMK_CHECK_BUDGET"] = str(v)
+
+    yield set_budget
+    if old is None:
+        os.environ.pop("GMK_CHECK_BUDGET", None)
+    else:
+        os.environ["GMK_CHECK_BUDGET"] = old
+
+
+@pytest.mark.parametrize("b", [0, 1, 3])
+def test_deferred_checks_are_exact(budget, b):
+    budget(b)
+    with Engine(0) as e:
+        for name, k, theta in CASES:
+            c = [c for c in GC.cases() if c["graph"] == name and c["k"] == k and c["theta"] == theta][0]
+            print(f"[gpu] {name} k={k} theta={theta} seeded, check budget {b}", flush=True)
+            e.set_graph(GC.graph(name))
+            for mode in ("plain", "batched"):
+                res = e.enumerate(k=k, theta=theta, mode=mode, partition="seeded")
+                assert GC.digest_matches(c, O.digest(res.words)), (name, k, theta, mode, b)
