@@ -71,7 +71,7 @@ def bench_batch(args, cfg) -> int:
     train = make(args.train_graphs, gen.SEED_C3 + 1000)
     sample = graphs[:args.batch_sample]
     log(f"[bench] generated ({time.time() - t0:.1f} s), {sum(g.n_edges for g, _, _ in graphs)} edges")
-    eng = Engine(0)
+    eng = Engine(0, tuning=args.tune)
     cap = cfg["limit"]
 
     # cost model: device time of each training graph run alone (batched engine)
@@ -222,6 +222,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="cl50kx30k", choices=sorted(WORKLOADS))
     ap.add_argument("--partition", default="seeded", choices=["auto", "global", "seeded"])
+    ap.add_argument("--tune", default=os.path.join(ROOT, "gmkbe_amd", "tune", "mi355x.json"), help="tuning table (written by python -m gmkbe_amd.autotune)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-mode run")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-mode run (profiling the batched kernels)")
@@ -261,7 +262,7 @@ def main() -> int:
     except Exception:
         sync = lambda: None  # noqa: E731
 
-    eng = Engine(0)
+    eng = Engine(0, tuning=args.tune)
     eng.set_graph(g)
     # plain: per-partition (per-seed) launches, the reference's way of working;
     # batched (the default mode): full launches, persistent expansion kernel

@@ -15,8 +15,10 @@ the largest side in the bucket.  Every candidate setting runs the batch in
 batched mode; the fastest is kept for the bucket's widths.  Every candidate
 must return the same per-graph counts, and the same fingerprints for graphs
 below the result cap: the tuning changes launches, not results.  The thin
-checks' launch form is tuned per k on graphs shaped like the workloads of
-that k, and the dedup table's load factor on the 50k x 30k workload.
+checks' launch form (one lane per rank, or waves taking chunks of ranks) is
+tuned per k on the workload of that k (50k x 30k Chung-Lu for k = 1,
+2M x 1M Chung-Lu for k = 2, 500k x 500k with planted near-bicliques for
+k = 3), and the dedup table's load factor on the 50k x 30k workload.
 
     python -m gmkbe_amd.autotune [--out gmkbe_amd/tune/mi355x.json]
 
@@ -41,6 +43,9 @@ BUCKETS = [(20, 63, (0,)), (64, 255, (1, 2)), (256, 1023, (3, 4)), (1024, 2000, 
 WAVES = (0, 16)
 BATCH = (16384, 65536)
 DEDUP_LOADS = (300, 500, 700)
+CHECK_FORMS = {0: "one rank per lane", 1: "chunked"}
+# the thin-result workload of each k and its result limit
+THIN_WORKLOADS = {1: ("cl50kx30k", 1_000_000), 2: ("cl2Mx1M", 10_000_000), 3: ("planted500k", 10_000_000)}
 CAL_SEED = gen.SEED_C3 + 2000
 
 
@@ -100,37 +105,32 @@ def tune(graphs_per_bucket: int = 24, cap: int = 1000, reps: int = 1) -> dict:
              f"batch={best[2]}")
 
     # thin checks per k: one lane per rank over the grid, or waves taking chunks
-    # of ranks; seeded, batched, r = 10^6, on a graph shaped like the workload
-    # each k is measured on: k=1 the 50k x 30k Chung-Lu graph, k=2 a 400k x 200k
-    # Chung-Lu graph (exponent 2.0, uncapped hubs), k=3 a 100k x 100k graph with
-    # planted near-bicliques
-    proxies = {1: gen.workload("cl50kx30k"), 2: gen.chung_lu(400_000, 200_000, 4_000_000, 2.0, gen.SEED_C4 + 7),
-               3: gen.planted(100_000, 100_000, 1_000_000, 2.2, gen.SEED_C5 + 7)}
-    chunked = list(base.get("check_chunked", [0, 0, 0, 1]))
-    for k in (1, 2, 3):
-        eng.set_graph(proxies[k])
+    # of ranks; seeded, batched, on the workload of that k at its result limit
+    forms = list(base.get("check_form", [0, 0, 0, 1]))
+    for k, (name, limit) in THIN_WORKLOADS.items():
+        eng.set_graph(gen.workload(name))
         best_c = None
-        for form in (0, 1):
+        for form in CHECK_FORMS:
             t = dict(base)
             t["waves_per_cu"], t["batch_states"] = chosen_w, chosen_b
-            t["check_chunked"] = [form] * 4
+            t["check_form"] = [form] * 4
             eng.set_tuning(t)
             times = []
             for _ in range(reps + 1):
                 t0 = time.perf_counter()
-                r = eng.enumerate(k=k, theta=1, limit=1_000_000, mode="batched", partition="seeded", keep=False)
+                r = eng.enumerate(k=k, theta=1, limit=limit, mode="batched", partition="seeded", keep=False)
                 times.append(time.perf_counter() - t0)
             s = min(times[1:])
-            trials.append({"k": k, "check_chunked": form, "s": s, "results": r.count})
-            _log(f"[autotune] thin checks k={k} {'chunked' if form else 'one rank per lane'}: {s * 1e3:.2f} ms")
+            trials.append({"k": k, "workload": name, "check_form": form, "s": s, "results": r.count})
+            _log(f"[autotune] thin checks k={k} ({name}) {CHECK_FORMS[form]}: {s * 1e3:.2f} ms")
             if best_c is None or s < best_c[0]:
                 best_c = (s, form)
-        chunked[k] = best_c[1]
+        forms[k] = best_c[1]
 
     # dedup table load factor on the 50k x 30k workload
-    eng.set_graph(proxies[1])
+    eng.set_graph(gen.workload("cl50kx30k"))
     t = dict(base)
-    t["waves_per_cu"], t["batch_states"], t["check_chunked"] = chosen_w, chosen_b, chunked
+    t["waves_per_cu"], t["batch_states"], t["check_form"] = chosen_w, chosen_b, forms
     eng.set_tuning(t)
     best_load = None
     for load in DEDUP_LOADS:
@@ -154,7 +154,7 @@ def tune(graphs_per_bucket: int = 24, cap: int = 1000, reps: int = 1) -> dict:
         "generated": datetime.datetime.now(datetime.timezone.utc).isoformat(timespec="seconds"),
         "buckets": [{"sides": [lo, hi], "W": [1 << q for q in lanes]} for lo, hi, lanes in BUCKETS],
         "tuning": {"version": TUNING_VERSION, "waves_per_cu": chosen_w, "batch_states": chosen_b,
-                   "dedup_load_permille": best_load[1], "check_chunked": chunked},
+                   "dedup_load_permille": best_load[1], "check_form": forms},
         "trials": trials,
     }
 

@@ -53,7 +53,7 @@ extern "C" int gmk_set_tuning(gmk_ctx* c, const gmk_tuning* t) {
     return GMK_EINVAL;
   }
   for (int q = 0; q < 4; ++q)
-    if (t->check_chunked[q] > 1) { c->err = "tuning value out of range"; return GMK_EINVAL; }
+    if (t->check_form[q] > 1) { c->err = "tuning value out of range"; return GMK_EINVAL; }
   c->tune = *t;
   return GMK_OK;
 }
@@ -69,8 +69,8 @@ extern "C" void gmk_destroy(gmk_ctx* c) {
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
   for (DeviceVec* v : {&c->rows, &c->maps, &c->parts, &c->res, &c->counters, &c->tuples, &c->arena, &c->fams,
-                        &c->prefix, &c->tcnt, &c->tdefer, &c->dedup, &c->gcnt, &c->bcsr_off, &c->bcsr_adj,
-                        &c->bcsr_base, &c->rowpids})
+                        &c->prefix, &c->tcnt, &c->tdefer, &c->nh_off, &c->nh, &c->nh_verts, &c->dedup, &c->gcnt,
+                        &c->bcsr_off, &c->bcsr_adj, &c->bcsr_base, &c->rowpids})
     if (v->p) (void)hipFree(v->p);
   for (int l = 0; l < 7; ++l)
     for (DeviceVec* v : {&c->lane_stack[l], &c->lane_next[l], &c->lane_retry[l], &c->lane_spill[l]})
@@ -99,7 +99,7 @@ static int upload_graph(gmk_ctx* c) {
   if (!c->adj.empty())
     HIPCHK(c, hipMemcpyAsync(c->d_adj, c->adj.data(), sizeof(u32) * c->adj.size(), hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  return GMK_OK;
+  return gmk::build_neighbour_hashes(c);
 }
 
 static int check_csr(gmk_ctx* c) {

@@ -100,6 +100,9 @@ struct gmk_ctx {
   DeviceVec lane_spill[7];  // shallow states moved out of a nearly full stack
   DeviceVec rows, maps, parts, res, counters;
   DeviceVec tuples, arena, fams, prefix, tcnt, tdefer;  // thin-result kernels
+  DeviceVec nh_off, nh, nh_verts;  // neighbour hashes of the vertices of degree >= nh_min
+  u32 nh_min = 0xFFFFFFFFu;        // as built for the current graph (none: all ones)
+  u64 nh_slots = 0;
   DeviceVec dedup;                              // result fingerprint table
   DeviceVec gcnt;                               // per-graph counters of a batch
   std::vector<u32> order;                        // degeneracy order (seeded runs)
@@ -209,3 +212,10 @@ int gmk_run_bb(gmk_ctx* c, const gmk::PartitionSet& ps, const gmk_params* prm, g
 int gmk_run_seeded(gmk_ctx* c, const gmk_params* prm, gmk_stats* st, gmk::RunState& rs);
 // Thin results with the tuple on side_a (0 left, 1 right); generated side >= min_b.
 int gmk_run_thin(gmk_ctx* c, const gmk_params* prm, gmk_stats* st, gmk::RunState& rs, int side_a, uint32_t min_b);
+namespace gmk {
+// Neighbour hashes of the current graph's vertices of degree >= NH_MIN_DEG
+// (the environment variable GMK_NHASH_MIN overrides it; 0: no hashes), used
+// by the thin checks' adjacency tests.  Called when a graph is uploaded.
+constexpr u32 NH_MIN_DEG = 128;
+int build_neighbour_hashes(gmk_ctx* c);
+}  // namespace gmk

@@ -115,7 +115,7 @@ fs::path record_path(const fs::path& graph, const std::string& suffix) {
 }
 
 // Reads the "tuning" object of an autotuner table: the arrays waves_per_cu and
-// batch_states (7 numbers each), dedup_load_permille and check_chunked (4
+// batch_states (7 numbers each), dedup_load_permille and check_form (4
 // numbers; the built-in choice when absent).
 bool read_tuning(const std::string& path, gmk_tuning& t) {
   std::ifstream in(path);
@@ -142,8 +142,8 @@ bool read_tuning(const std::string& path, gmk_tuning& t) {
     return true;
   };
   t.version = GMK_TUNING_VERSION;
-  const uint32_t chunked_default[4] = {0, 0, 0, 1};
-  if (!numbers("check_chunked", t.check_chunked, 4)) memcpy(t.check_chunked, chunked_default, sizeof chunked_default);
+  const uint32_t form_default[4] = {0, 0, 0, 1};
+  if (!numbers("check_form", t.check_form, 4)) memcpy(t.check_form, form_default, sizeof form_default);
   return numbers("waves_per_cu", t.waves_per_cu, 7) && numbers("batch_states", t.batch_states, 7) &&
          numbers("dedup_load_permille", &t.dedup_load_permille, 1);
 }

@@ -172,12 +172,48 @@ GMK_HD u32 pool_member(const Family& f, u32 t, u64 i, u32 b_lo, Cost& cost) {
   return ((const u32*)f.pool[t])[i];
 }
 
+// Neighbour hashes: every vertex v of degree d >= nh_min has an open-addressing
+// set of its neighbours, 2^nh_log2(d) >= 2d slots (load <= 1/2) from
+// nh[nh_off[v]], empty slots NH_EMPTY, linear probing from nh_slot.  A
+// membership test then reads about 1.5 slots instead of log2(d) binary-search
+// probes of a long list.
+constexpr u32 NH_EMPTY = 0xFFFFFFFFu;
+GMK_HD u32 nh_log2(u32 d) { return 32u - (u32)__builtin_clz(2u * d - 1u); }
+GMK_HD u32 nh_slot(u32 x, u32 lg) { return (u32)(((u64)(x * 0x9E3779B1u) << lg) >> 32); }
+
 struct Graph {
   const u32* off;
   const u32* adj;
   u32 a_lo, a_hi;  // side A (tuple side), global IDs
   u32 b_lo, b_hi;  // side B
+  const u64* nh_off = nullptr;  // neighbour hashes (none when nh is null)
+  const u32* nh = nullptr;
+  u32 nh_min = 0xFFFFFFFFu;
 };
+
+// The neighbour hash of v (degree d), or null when v has none.
+GMK_HD const u32* nh_table(const Graph& g, u32 v, u32 d, Cost& cost) {
+  if (!g.nh || d < g.nh_min) return nullptr;
+  cost.bytes += 8;
+  return g.nh + g.nh_off[v];
+}
+
+// Is x in the neighbour hash t of a vertex of degree d?
+GMK_HD bool nh_find(const u32* t, u32 d, u32 x, Cost& cost) {
+  const u32 lg = nh_log2(d), mask = (1u << lg) - 1u;  // lg <= 31: d < 2^30
+  for (u32 s = nh_slot(x, lg);; s = (s + 1) & mask) {
+    const u32 y = t[s];
+    cost.bytes += 4;
+    if (y == x) return true;
+    if (y == NH_EMPTY) return false;
+  }
+}
+
+// Is x a neighbour of a vertex with sorted list nv of size d and hash t
+// (null: none)?
+GMK_HD bool adjacent(const u32* t, const u32* nv, u32 d, u32 x, Cost& cost) {
+  return t ? nh_find(t, d, x, cost) : contains(nv, d, x, cost);
+}
 
 // Threat test: can some w in side A outside L be added to (L, R)?
 //   R = free (sorted, nfree) + ch (nch chosen, each with its type size tsz).
@@ -189,7 +225,8 @@ GMK_HD bool in_tuple(const u32* L, u32 j, u32 w) {
 
 // Does w miss at most k vertices of R, and is w adjacent to every saturated one?
 // `known` is a member of R already known to be adjacent to w (the vertex whose
-// list produced w as a candidate), or NONE.  Members are searched in N(w).
+// list produced w as a candidate), or NONE.  Members are looked up in N(w)
+// (its hash, when w has one).
 // NMAX bounds nch at compile time so the chosen vertices stay in registers.
 template <int NMAX>
 GMK_HD bool addable(const Graph& g, u32 w, const u32* freel, u32 nfree, const u32* ch, const u32* ctsz, u32 nch,
@@ -199,18 +236,19 @@ GMK_HD bool addable(const Graph& g, u32 w, const u32* freel, u32 nfree, const u3
   cost.bytes += 8;
   const u32 rs = nfree + nch;
   if (dw + k < rs) return false;
+  const u32* hw = nh_table(g, w, dw, cost);
   u32 miss = 0;
   bool out = false;  // a saturated member is missed, or more than k members
 #pragma unroll
   for (int i = 0; i < NMAX; ++i) {
-    if ((u32)i < nch && !out && ch[i] != known && !contains(nw, dw, ch[i], cost))
+    if ((u32)i < nch && !out && ch[i] != known && !adjacent(hw, nw, dw, ch[i], cost))
       out = ctsz[i] >= k || ++miss > k;
   }
   if (out) return false;
   // free vertices have type 0 (< k for k >= 1): only their number matters
   if (nfree <= dw) {
     for (u32 i = 0; i < nfree; ++i)
-      if (!contains(nw, dw, freel[i], cost) && ++miss > k) return false;
+      if (!adjacent(hw, nw, dw, freel[i], cost) && ++miss > k) return false;
     cost.bytes += 4ull * nfree;
   } else {
     // count free vertices adjacent to w by scanning N(w); stop once the rest of
@@ -239,6 +277,7 @@ struct CandSegs {
   u32 n;
   u32 must;          // candidates must be adjacent to this vertex (NONE: no filter)
   const u32* must_nb;
+  const u32* must_nh;  // its neighbour hash, or null
   u32 must_deg;
   int decided;
 };
@@ -259,6 +298,7 @@ GMK_HD void candidate_segments(const Graph& g, u32 j, u32 nfree, const u32* fpic
   cs.n = 0;
   cs.must = 0xFFFFFFFFu;
   cs.must_nb = nullptr;
+  cs.must_nh = nullptr;
   cs.must_deg = 0;
   cs.decided = -1;
   const u32 rs = nfree + nch;
@@ -311,6 +351,7 @@ GMK_HD void candidate_segments(const Graph& g, u32 j, u32 nfree, const u32* fpic
   if (best != 0xFFFFFFFFu) {
     cs.must = best;
     cs.must_nb = g.adj + g.off[best];
+    cs.must_nh = nh_table(g, best, bestdeg, cost);
     cs.must_deg = bestdeg;
   }
 }
@@ -320,7 +361,7 @@ template <int NMAX>
 GMK_HD bool candidate_joins(const Graph& g, const u32* L, u32 j, const u32* freel, u32 nfree, const u32* ch,
                             const u32* ctsz, u32 nch, u32 k, const CandSegs& cs, u32 seg, u32 w, Cost& cost) {
   if (in_tuple(L, j, w)) return false;
-  if (cs.must != 0xFFFFFFFFu && !contains(cs.must_nb, cs.must_deg, w, cost)) return false;
+  if (cs.must != 0xFFFFFFFFu && !adjacent(cs.must_nh, cs.must_nb, cs.must_deg, w, cost)) return false;
   return addable<NMAX>(g, w, freel, nfree, ch, ctsz, nch, k, cs.known[seg], cost);
 }
 

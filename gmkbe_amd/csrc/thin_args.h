@@ -42,8 +42,14 @@ struct ThinArgs {
 };
 
 hipError_t launch_thin_build(const ThinArgs& a, hipStream_t s);
-hipError_t launch_thin_check(const ThinArgs& a, hipStream_t s, u32 max_blocks, bool chunked);
+// Launch forms of the thin check (gmk_tuning.check_form): one lane per rank over
+// the whole grid, or waves of a fixed grid taking 64-rank chunks.
+enum : u32 { CHECK_GRID = 0, CHECK_CHUNKED = 1 };
+hipError_t launch_thin_check(const ThinArgs& a, hipStream_t s, u32 max_blocks, u32 form);
 hipError_t launch_thin_check_deferred(const ThinArgs& a, hipStream_t s, u32 blocks);
+// Neighbour hashes of the nv vertices in verts (slots already set to NH_EMPTY).
+hipError_t launch_build_neighbour_hash(const u32* off, const u32* adj, const u32* verts, u32 nv, const u64* nh_off,
+                                       u32* nh, hipStream_t s);
 hipError_t launch_thin_scan(const thin::Family* fams, u32 nfam, u64* prefix, hipStream_t s);
 
 }  // namespace gmk

@@ -80,7 +80,7 @@ int run_chunk(ThinRun& T, const u32* tuples, u32 count, u32 j) {
     // work, defer_n and dwork are consecutive
     HIPCHK(c, hipMemsetAsync(&((ThinCounters*)c->tcnt.p)->work, 0, 3 * sizeof(u32), s));
     HIPCHK(c, hipEventRecord(c->ev0, s));
-    HIPCHK(c, launch_thin_check(a, s, c->n_cu * 8, c->tune.check_chunked[a.k < 4 ? a.k : 3] != 0));
+    HIPCHK(c, launch_thin_check(a, s, c->n_cu * 8, c->tune.check_form[a.k < 4 ? a.k : 3]));
     HIPCHK(c, launch_thin_check_deferred(a, s, c->n_cu * 4));
     HIPCHK(c, hipEventRecord(c->ev1, s));
     HIPCHK(c, hipMemcpyAsync(c->h_cnt, dc, offsetof(RunCounters, fp), hipMemcpyDeviceToHost, s));
@@ -120,6 +120,9 @@ int gmk_run_thin(gmk_ctx* c, const gmk_params* prm, gmk_stats* st, RunState& rs,
   ThinArgs& a = T.a;
   a.g.off = c->d_off;
   a.g.adj = c->d_adj;
+  a.g.nh_off = (const u64*)c->nh_off.p;
+  a.g.nh = c->nh_slots ? (const u32*)c->nh.p : nullptr;
+  a.g.nh_min = c->nh_min;
   if (side_a == 0) { a.g.a_lo = 0; a.g.a_hi = c->n_left; a.g.b_lo = c->n_left; a.g.b_hi = c->n; }
   else { a.g.a_lo = c->n_left; a.g.a_hi = c->n; a.g.b_lo = 0; a.g.b_hi = c->n_left; }
   a.k = k;
@@ -205,5 +208,39 @@ int gmk_run_thin(gmk_ctx* c, const gmk_params* prm, gmk_stats* st, RunState& rs,
       if (!T.stop && (r = flush())) return r;
     }
   }
+  return GMK_OK;
+}
+
+int gmk::build_neighbour_hashes(gmk_ctx* c) {
+  c->nh_min = 0xFFFFFFFFu;
+  c->nh_slots = 0;
+  u32 nh_min = NH_MIN_DEG;
+  if (const char* e = getenv("GMK_NHASH_MIN")) nh_min = (u32)std::min<unsigned long>(strtoul(e, nullptr, 10), ~0u);
+  if (nh_min == 0) return GMK_OK;
+  std::vector<u64> noff(c->n, 0);
+  std::vector<u32> verts;
+  u64 slots = 0;
+  for (u32 v = 0; v < c->n; ++v) {
+    const u32 d = c->off[v + 1] - c->off[v];
+    if (d < nh_min) continue;
+    if (d >= (1u << 30)) { c->err = "a vertex has 2^30 or more neighbours"; return GMK_EUNSUPPORTED; }
+    noff[v] = slots;
+    slots += 1ull << thin::nh_log2(d);
+    verts.push_back(v);
+  }
+  if (verts.empty()) return GMK_OK;
+  int r;
+  if ((r = ensure(c, c->nh_off, (size_t)c->n * 8))) return r;
+  if ((r = ensure(c, c->nh, (size_t)slots * 4))) return r;
+  if ((r = ensure(c, c->nh_verts, verts.size() * 4))) return r;
+  hipStream_t s = c->stream;
+  HIPCHK(c, hipMemcpyAsync(c->nh_off.p, noff.data(), (size_t)c->n * 8, hipMemcpyHostToDevice, s));
+  HIPCHK(c, hipMemcpyAsync(c->nh_verts.p, verts.data(), verts.size() * 4, hipMemcpyHostToDevice, s));
+  HIPCHK(c, hipMemsetAsync(c->nh.p, 0xFF, (size_t)slots * 4, s));
+  HIPCHK(c, launch_build_neighbour_hash(c->d_off, c->d_adj, (const u32*)c->nh_verts.p, (u32)verts.size(),
+                                        (const u64*)c->nh_off.p, (u32*)c->nh.p, s));
+  HIPCHK(c, hipStreamSynchronize(s));
+  c->nh_min = nh_min;
+  c->nh_slots = slots;
   return GMK_OK;
 }

@@ -457,6 +457,30 @@ hipError_t launch_thin_scan(const thin::Family* fams, u32 nfam, u64* prefix, hip
   return hipGetLastError();
 }
 
+// One wave per hashed vertex: lanes insert its neighbours by compare-and-swap
+// with linear probing (neighbours are distinct, the set has >= 2d slots).
+__global__ __launch_bounds__(256) void build_neighbour_hash(const u32* off, const u32* adj, const u32* verts, u32 nv,
+                                                            const u64* nh_off, u32* nh) {
+  const u32 wv = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (wv >= nv) return;
+  const u32 v = verts[wv];
+  const u32 o = off[v], d = off[v + 1] - o;
+  const u32 lg = thin::nh_log2(d), mask = (1u << lg) - 1u;
+  u32* t = nh + nh_off[v];
+  for (u32 i = lane; i < d; i += 64) {
+    const u32 x = adj[o + i];
+    u32 sl = thin::nh_slot(x, lg);
+    while (atomicCAS(&t[sl], thin::NH_EMPTY, x) != thin::NH_EMPTY) sl = (sl + 1) & mask;
+  }
+}
+
+hipError_t launch_build_neighbour_hash(const u32* off, const u32* adj, const u32* verts, u32 nv, const u64* nh_off,
+                                       u32* nh, hipStream_t s) {
+  if (nv == 0) return hipSuccess;
+  hipLaunchKernelGGL(build_neighbour_hash, dim3((nv + 3) / 4), dim3(256), 0, s, off, adj, verts, nv, nh_off, nh);
+  return hipGetLastError();
+}
+
 hipError_t launch_thin_check_deferred(const ThinArgs& a, hipStream_t s, u32 blocks) {
   const dim3 g(blocks), b(256);
   switch (a.k) {
@@ -468,12 +492,12 @@ hipError_t launch_thin_check_deferred(const ThinArgs& a, hipStream_t s, u32 bloc
   return hipGetLastError();
 }
 
-hipError_t launch_thin_check(const ThinArgs& a, hipStream_t s, u32 max_blocks, bool chunked) {
+hipError_t launch_thin_check(const ThinArgs& a, hipStream_t s, u32 max_blocks, u32 form) {
   if (a.r1 <= a.r0) return hipSuccess;
   const u64 n = a.r1 - a.r0;
   const u64 need = (n + 255) / 256;
   const dim3 b(256);
-  if (chunked) {
+  if (form == CHECK_CHUNKED) {
     const dim3 g((u32)(need < max_blocks ? need : max_blocks));
     switch (a.k) {
       case 1: hipLaunchKernelGGL(thin_check_chunked<1>, g, b, 0, s, a); break;

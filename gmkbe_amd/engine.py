@@ -71,12 +71,12 @@ class Tuning(ctypes.Structure):
     """gmk_tuning: expansion launch parameters per bitset width (2^l words)."""
     _fields_ = [("version", ctypes.c_uint32), ("waves_per_cu", ctypes.c_uint32 * 7),
                 ("batch_states", ctypes.c_uint32 * 7), ("dedup_load_permille", ctypes.c_uint32),
-                ("check_chunked", ctypes.c_uint32 * 4)]
+                ("check_form", ctypes.c_uint32 * 4)]
 
     def as_dict(self) -> dict:
         return {"version": self.version, "waves_per_cu": list(self.waves_per_cu),
                 "batch_states": list(self.batch_states), "dedup_load_permille": self.dedup_load_permille,
-                "check_chunked": list(self.check_chunked)}
+                "check_form": list(self.check_form)}
 
     @classmethod
     def from_dict(cls, d: dict) -> "Tuning":
@@ -87,8 +87,8 @@ class Tuning(ctypes.Structure):
         for i, v in enumerate(d["batch_states"]):
             t.batch_states[i] = int(v)
         t.dedup_load_permille = int(d.get("dedup_load_permille", 500))
-        for i, v in enumerate(d.get("check_chunked", [0, 0, 0, 1])):
-            t.check_chunked[i] = int(v)
+        for i, v in enumerate(d.get("check_form", [0, 0, 0, 1])):
+            t.check_form[i] = int(v)
         return t
 
 

@@ -195,7 +195,7 @@ typedef struct {
   uint32_t waves_per_cu[7];         /* batched modes: persistent waves per CU; 0 = one workgroup per state */
   uint32_t batch_states[7];         /* states per expansion launch, 64 .. 65536 */
   uint32_t dedup_load_permille;     /* dedup table load when gmk_params leaves it 0 (100 .. 900) */
-  uint32_t check_chunked[4];        /* per k: thin checks as waves taking 64-rank chunks (1) or one lane per rank (0) */
+  uint32_t check_form[4];           /* per k, thin checks: 0 one lane per rank, 1 waves taking 64-rank chunks */
 } gmk_tuning;
 int gmk_set_tuning(gmk_ctx* ctx, const gmk_tuning* tuning);
 int gmk_get_tuning(gmk_ctx* ctx, gmk_tuning* tuning);

@@ -23,7 +23,57 @@ struct HostFams {
   std::vector<Family> f;
   Family* next() { f.emplace_back(); return &f.back(); }
 };
+
+// Neighbour hashes of the vertices of degree >= nh_min, in the layout the
+// device builds (thin.h): sequential inserts with the same slot function.
+struct HostHashes {
+  std::vector<u64> off;
+  std::vector<u32> slots;
+  void build(unsigned n, const unsigned* goff, const unsigned* adj, u32 nh_min) {
+    off.assign(n, 0);
+    u64 total = 0;
+    for (u32 v = 0; v < n; ++v) {
+      const u32 d = goff[v + 1] - goff[v];
+      if (d >= nh_min) { off[v] = total; total += 1ull << nh_log2(d); }
+    }
+    slots.assign(total ? total : 1, NH_EMPTY);
+    for (u32 v = 0; v < n; ++v) {
+      const u32 d = goff[v + 1] - goff[v];
+      if (d < nh_min) continue;
+      const u32 lg = nh_log2(d), mask = (1u << lg) - 1u;
+      u32* t = slots.data() + off[v];
+      for (u32 e = goff[v]; e < goff[v + 1]; ++e) {
+        u32 s = nh_slot(adj[e], lg);
+        while (t[s] != NH_EMPTY) s = (s + 1) & mask;
+        t[s] = adj[e];
+      }
+    }
+  }
+};
+u32 g_hash_min = 0;  // 0: the model uses sorted-list searches only
 }  // namespace
+
+// Later thin_model calls use neighbour hashes for vertices of degree >= v (0: none).
+extern "C" void thin_model_set_hash_min(unsigned v) { g_hash_min = v; }
+
+// Membership through the neighbour hashes against the sorted lists, for every
+// hashed vertex v and every vertex x: the number of disagreements.
+extern "C" long long thin_model_hash_check(unsigned n, const unsigned* off, const unsigned* adj, unsigned nh_min) {
+  HostHashes h;
+  h.build(n, off, adj, nh_min);
+  Graph g;
+  g.off = off; g.adj = adj; g.nh_off = h.off.data(); g.nh = h.slots.data(); g.nh_min = nh_min;
+  long long bad = 0;
+  Cost cost;
+  for (u32 v = 0; v < n; ++v) {
+    const u32 d = off[v + 1] - off[v];
+    const u32* t = nh_table(g, v, d, cost);
+    if (!t) continue;
+    for (u32 x = 0; x < n; ++x)
+      bad += nh_find(t, d, x, cost) != contains(adj + off[v], d, x, cost);
+  }
+  return bad;
+}
 
 extern "C" long long thin_model(unsigned n, unsigned n_left, const unsigned* off, const unsigned* adj, unsigned k,
                                 unsigned theta, int side_a, unsigned min_b, unsigned** out_words,
@@ -32,6 +82,11 @@ extern "C" long long thin_model(unsigned n, unsigned n_left, const unsigned* off
   g.off = off; g.adj = adj;
   if (side_a == 0) { g.a_lo = 0; g.a_hi = n_left; g.b_lo = n_left; g.b_hi = n; }
   else { g.a_lo = n_left; g.a_hi = n; g.b_lo = 0; g.b_hi = n_left; }
+  HostHashes hh;
+  if (g_hash_min) {
+    hh.build(n, off, adj, g_hash_min);
+    g.nh_off = hh.off.data(); g.nh = hh.slots.data(); g.nh_min = g_hash_min;
+  }
   std::vector<u32> words;
   long long count = 0;
   const u32 jmin = theta > 1 ? theta : 1;

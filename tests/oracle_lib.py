@@ -161,6 +161,19 @@ def _model_init():
                                       ctypes.c_int, ctypes.c_uint32, P(u32p), P(ctypes.c_uint64)]
         _model.thin_dominated.restype = ctypes.c_longlong
         _model.thin_dominated.argtypes = [ctypes.c_uint32, ctypes.c_uint32, u32p, u32p, ctypes.c_uint32, ctypes.c_int]
+        _model.thin_model_set_hash_min.restype = None
+        _model.thin_model_set_hash_min.argtypes = [ctypes.c_uint32]
+        _model.thin_model_hash_check.restype = ctypes.c_longlong
+        _model.thin_model_hash_check.argtypes = [ctypes.c_uint32, u32p, u32p, ctypes.c_uint32]
+
+
+def hash_check(g, nh_min):
+    """Neighbour-hash membership against the sorted lists, all (hashed vertex,
+    vertex) pairs: the number of disagreements (tests/model/thin_model.cpp)."""
+    _model_init()
+    off = np.ascontiguousarray(g.off, dtype=np.uint32)
+    adj = np.ascontiguousarray(g.adj, dtype=np.uint32)
+    return int(_model.thin_model_hash_check(g.n, _p(off), _p(adj), nh_min))
 
 
 def thin_dominated(g, k, side_a):
@@ -171,12 +184,19 @@ def thin_dominated(g, k, side_a):
     return int(_model.thin_dominated(g.n, g.n_left, _p(off), _p(adj), k, side_a))
 
 
-def thin_model(g, k, theta, side_a, min_b):
-    """CPU model of the thin-result kernels (tests/model/thin_model.cpp)."""
+def thin_model(g, k, theta, side_a, min_b, hash_min=0):
+    """CPU model of the thin-result kernels (tests/model/thin_model.cpp);
+    hash_min > 0: adjacency tests through neighbour hashes of the vertices of
+    at least that degree."""
     _model_init()
     off = np.ascontiguousarray(g.off, dtype=np.uint32)
     adj = np.ascontiguousarray(g.adj, dtype=np.uint32)
     wp = ctypes.POINTER(ctypes.c_uint32)()
     nw = ctypes.c_uint64(0)
-    _model.thin_model(g.n, g.n_left, _p(off), _p(adj), k, theta, side_a, min_b, ctypes.byref(wp), ctypes.byref(nw))
+    _model.thin_model_set_hash_min(hash_min)
+    try:
+        _model.thin_model(g.n, g.n_left, _p(off), _p(adj), k, theta, side_a, min_b, ctypes.byref(wp),
+                          ctypes.byref(nw))
+    finally:
+        _model.thin_model_set_hash_min(0)
     return _take(wp, nw)

@@ -41,3 +41,28 @@ def test_dominated_families_skipped_exactly(k, seed):
     a, b = O.digest(words), O.digest(ref)
     assert (a.count, a.canon_lo, a.canon_hi) == (b.count, b.canon_lo, b.canon_hi)
     assert O.validate(g, k, 1, words) == 0
+
+
+@pytest.mark.parametrize("name", ["er12x10", "southern_women", "er40x50", "moreno_crime"])
+def test_neighbour_hash_membership(name):
+    """Every hashed vertex's set answers membership like its sorted list, for
+    every vertex of the graph (hashes for degrees >= 1 and >= 4)."""
+    g = GC.graph(name)
+    assert O.hash_check(g, 1) == 0
+    assert O.hash_check(g, 4) == 0
+
+
+@pytest.mark.parametrize("c", [c for c in GC.cases(max_count=600_000, min_k=1) if c["theta"] <= 2 * c["k"]],
+                         ids=GC.case_id)
+def test_thin_model_with_neighbour_hashes(c):
+    """Adjacency tests through neighbour hashes (every vertex with a neighbour,
+    then degree >= 3) give the same thin results, in the same order, as sorted
+    lists."""
+    g = GC.graph(c["graph"])
+    k, th = c["k"], c["theta"]
+    lb = max(th, 2 * k + 1)
+    for side_a, min_b in ((0, th), (1, lb)):
+        base = O.thin_model(g, k, th, side_a, min_b)
+        for hm in (1, 3):
+            assert np.array_equal(O.thin_model(g, k, th, side_a, min_b, hash_min=hm), base), (side_a, hm)
+

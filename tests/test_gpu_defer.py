@@ -1,10 +1,11 @@
-"""The thin checks' deferred pass gives the same result sets.
+"""The thin checks' launch forms and deferred pass give the same result sets.
 
-A lane tests a combination's first candidates itself; one still open after
-GMK_CHECK_BUDGET of them goes to a second kernel in which a whole wave tests
-the rest.  Budgets 0 (every open combination deferred, the wave tests all its
-candidates), 1 and 3 push most combinations of the golden graphs through that
-second kernel: results must still equal the reference's canonical sets."""
+The first check kernel (either launch form) tests a combination's first
+candidates in one lane; one still open after GMK_CHECK_BUDGET of them goes to
+a second kernel in which a whole wave tests the rest.  Budgets 0 (every open combination deferred, the wave tests
+all its candidates), 1 and 3 push most combinations of the golden graphs
+through that second kernel, 64 (the default) few: with every launch form,
+results must still equal the reference's canonical sets."""
 import os
 
 import pytest
@@ -34,13 +35,15 @@ def budget():
         os.environ["GMK_CHECK_BUDGET"] = old
 
 
-@pytest.mark.parametrize("b", [0, 1, 3])
-def test_deferred_checks_are_exact(budget, b):
+@pytest.mark.parametrize("form", [0, 1], ids=["lane", "chunked"])
+@pytest.mark.parametrize("b", [0, 1, 3, 64])
+def test_deferred_checks_are_exact(budget, b, form):
     budget(b)
     with Engine(0) as e:
+        e.set_tuning(dict(e.get_tuning(), check_form=[form] * 4))
         for name, k, theta in CASES:
             c = [c for c in GC.cases() if c["graph"] == name and c["k"] == k and c["theta"] == theta][0]
-            print(f"[gpu] {name} k={k} theta={theta} seeded, check budget {b}", flush=True)
+            print(f"[gpu] {name} k={k} theta={theta} seeded, check form {form}, budget {b}", flush=True)
             e.set_graph(GC.graph(name))
             for mode in ("plain", "batched"):
                 res = e.enumerate(k=k, theta=theta, mode=mode, partition="seeded")
