@@ -27,6 +27,7 @@ struct ThinArgs {
   u32* flags;           // run flags (limit, result overflow)
   // T2
   const u64* prefix;    // nfam + 1 prefix sums of family totals
+  const u32* ridx;      // per 2^RIDX_LOG ranks of the launch's range: the family of its first rank
   u32 nfam;
   u64 r0, r1;           // rank range of this launch
   int keep;
@@ -51,5 +52,9 @@ hipError_t launch_thin_check_deferred(const ThinArgs& a, hipStream_t s, u32 bloc
 hipError_t launch_build_neighbour_hash(const u32* off, const u32* adj, const u32* verts, u32 nv, const u64* nh_off,
                                        u32* nh, hipStream_t s);
 hipError_t launch_thin_scan(const thin::Family* fams, u32 nfam, u64* prefix, hipStream_t s);
+// Rank index of a check launch: ridx[b] = family of rank min(r0 + b 2^RIDX_LOG,
+// r1 - 1), b = 0 .. ceil((r1 - r0) / 2^RIDX_LOG).
+constexpr u32 RIDX_LOG = 8;
+hipError_t launch_thin_rank_index(const u64* prefix, u32 nfam, u64 r0, u64 r1, u32* ridx, hipStream_t s);
 
 }  // namespace gmk

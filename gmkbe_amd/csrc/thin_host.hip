@@ -80,6 +80,7 @@ int run_chunk(ThinRun& T, const u32* tuples, u32 count, u32 j) {
     // work, defer_n and dwork are consecutive
     HIPCHK(c, hipMemsetAsync(&((ThinCounters*)c->tcnt.p)->work, 0, 3 * sizeof(u32), s));
     HIPCHK(c, hipEventRecord(c->ev0, s));
+    HIPCHK(c, launch_thin_rank_index((const u64*)c->prefix.p, nfam, a.r0, a.r1, (u32*)c->ridx.p, s));
     HIPCHK(c, launch_thin_check(a, s, c->n_cu * 8, c->tune.check_form[a.k < 4 ? a.k : 3]));
     HIPCHK(c, launch_thin_check_deferred(a, s, c->n_cu * 4));
     HIPCHK(c, hipEventRecord(c->ev1, s));
@@ -115,6 +116,7 @@ int gmk_run_thin(gmk_ctx* c, const gmk_params* prm, gmk_stats* st, RunState& rs,
   if ((r = ensure(c, c->prefix, (size_t)(FCAP + 1) * 8))) return r;
   if ((r = ensure(c, c->tcnt, sizeof(ThinCounters)))) return r;
   if ((r = ensure(c, c->tdefer, RCH * 8))) return r;
+  if ((r = ensure(c, c->ridx, ((RCH >> RIDX_LOG) + 2) * 4))) return r;
   ThinRun T;
   T.c = c; T.prm = prm; T.rs = &rs;
   ThinArgs& a = T.a;
@@ -143,6 +145,7 @@ int gmk_run_thin(gmk_ctx* c, const gmk_params* prm, gmk_stats* st, RunState& rs,
   // small graphs
   if (const char* e = getenv("GMK_CHECK_BUDGET")) a.budget = (u32)std::min<unsigned long>(strtoul(e, nullptr, 10), 1u << 20);
   a.defer = (u64*)c->tdefer.p;
+  a.ridx = (const u32*)c->ridx.p;
   a.defer_n = &tc->defer_n;
   a.dwork = &tc->dwork;
   RunCounters* dc = (RunCounters*)c->counters.p;

@@ -237,7 +237,10 @@ __device__ __forceinline__ const thin::Family* unrank_rank(const ThinArgs& a, u6
 #pragma unroll
   for (int q = 0; q < NM; ++q) { ch[q] = 0; ctsz[q] = 0; }
   nch = 0;
-  u32 lo = 0, hi = a.nfam;
+  // the families of the first ranks of r's block and of the next block bound r's
+  const u64 b = (r - a.r0) >> RIDX_LOG;
+  u32 lo = a.ridx[b], hi = a.ridx[b + 1] + 1;
+  cost.bytes += 8;
   while (hi - lo > 1) {
     const u32 mid = (lo + hi) >> 1;
     if (a.prefix[mid] <= r) lo = mid; else hi = mid;
@@ -263,30 +266,84 @@ __device__ __forceinline__ const thin::Family* unrank_rank(const ThinArgs& a, u6
   return f;
 }
 
-// Emits the maximal (L, R) of family f with chosen vertices ch (one lane):
-// fingerprint, dedup, result slot under the limit, record.  Returns whether it
-// was emitted.
-template <int K>
-__device__ __forceinline__ bool emit_rank(const ThinArgs& a, const thin::Family* f, u32 (&ch)[2 * K * K], u32 nch,
-                                          u32 stripe, thin::Cost& cost) {
-  constexpr int NM = 2 * K * K;
-  const u32 nr0 = f->nfree + nch;
-  const u32* fr0 = f->freep;
-  u64 h = 0;
-  for (u32 i = 0; i < f->j; ++i) h += mix64(f->L[i]);
-  for (u32 i = 0; i < f->nfree; ++i) h += mix64(fr0[i]);
+// Exclusive prefix sum over the wave.
+__device__ __forceinline__ u64 wave_excl_scan_u64(u64 v) {
+  const int lane = threadIdx.x & 63;
+  u64 inc = v;
 #pragma unroll
-  for (int i = 0; i < NM; ++i)
-    if ((u32)i < nch) h += mix64(ch[i]);
-  cost.bytes += 4ull * f->nfree;
-  const u64 fpv = fmix64(h + (u64)(a.swap_sides ? nr0 : f->j) * FP_A + (u64)(a.swap_sides ? f->j : nr0) * FP_B);
-  if (!dedup_insert(a.dedup, fpv, a.flags, stripe)) return false;
-  const u64 slot = atomicAdd((unsigned long long*)a.res_count, 1ull);
-  if (slot >= a.limit) {
-    atomicOr(a.flags, FLAG_LIMIT);
-    return false;
+  for (int d = 1; d < 64; d <<= 1) {
+    const u64 o = wave_bcast_u64(inc, lane >= d ? lane - d : lane);
+    if (lane >= d) inc += o;
   }
-  if (slot + 1 == a.limit) atomicOr(a.flags, FLAG_LIMIT);
+  return inc - v;
+}
+
+// Has the result limit been reached?  The same answer in every lane.
+__device__ __forceinline__ bool limit_reached(const ThinArgs& a) {
+  return __ballot((__hip_atomic_load(a.flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & FLAG_LIMIT) != 0) != 0;
+}
+
+// Emits the maximal (L, R) of family f with chosen vertices ch for every lane
+// with `want` set: fingerprint, dedup, result slot under the limit, record.
+// Every lane of the wave calls it together: the wave takes its result slots,
+// record space and fingerprint sums with one atomic each instead of one per
+// result, since results from every wave meet on the same counters.  Returns
+// whether this lane's result was emitted.
+template <int K>
+__device__ __forceinline__ bool emit_wave(const ThinArgs& a, bool want, const thin::Family* f, u32 (&ch)[2 * K * K],
+                                          u32 nch, u32 stripe, thin::Cost& cost) {
+  constexpr int NM = 2 * K * K;
+  const int lane = threadIdx.x & 63;
+  u64 fpv = 0;
+  if (want) {
+    const u32 nr0 = f->nfree + nch;
+    const u32* fr0 = f->freep;
+    u64 h = 0;
+    for (u32 i = 0; i < f->j; ++i) h += mix64(f->L[i]);
+    for (u32 i = 0; i < f->nfree; ++i) h += mix64(fr0[i]);
+#pragma unroll
+    for (int i = 0; i < NM; ++i)
+      if ((u32)i < nch) h += mix64(ch[i]);
+    cost.bytes += 4ull * f->nfree;
+    fpv = fmix64(h + (u64)(a.swap_sides ? nr0 : f->j) * FP_A + (u64)(a.swap_sides ? f->j : nr0) * FP_B);
+    want = dedup_insert(a.dedup, fpv, a.flags, stripe);
+  }
+  const u64 mask = __ballot(want);
+  if (!mask) return false;
+  const int leader = __ffsll((unsigned long long)mask) - 1;
+  const u32 nwant = (u32)__popcll(mask);
+  u64 base = 0;
+  if (lane == leader) {
+    base = atomicAdd((unsigned long long*)a.res_count, (unsigned long long)nwant);
+    if (base + nwant >= a.limit) atomicOr(a.flags, FLAG_LIMIT);
+  }
+  base = wave_bcast_u64(base, leader);
+  const u64 slot = base + (u64)__popcll(mask & ((1ull << lane) - 1));
+  if (want && slot >= a.limit) want = false;
+  // fingerprint sums of the emitted results
+  const u64 fsum = wave_sum_u64(want ? fpv : 0);
+  u64 fxor = want ? fpv : 0;
+  for (int m = 32; m > 0; m >>= 1) fxor ^= shfl_xor_u64(fxor, m);
+  if (lane == leader && fsum | fxor) {
+    u64* fs = a.fp + (stripe % NSTRIPE) * 2;
+    atomicAdd((unsigned long long*)&fs[0], (unsigned long long)fsum);
+    atomicXor((unsigned long long*)&fs[1], (unsigned long long)fxor);
+  }
+  if (!a.keep) return want;
+  const u32 nl = f ? f->j : 0, nr = want ? f->nfree + nch : 0;
+  const u64 len = want ? 2ull + nl + nr : 0;
+  const u64 off = wave_excl_scan_u64(len);
+  const u64 tot = wave_bcast_u64(off + len, 63);
+  if (!tot) return want;
+  u64 pos = 0;
+  if (lane == leader) pos = atomicAdd((unsigned long long*)a.res_words, (unsigned long long)tot);
+  pos = wave_bcast_u64(pos, leader) + off;
+  if (!want) return false;
+  cost.bytes += 4ull * len;
+  if (pos + len > a.res_cap) {
+    atomicOr(a.flags, FLAG_RES_OVERFLOW);
+    return true;
+  }
   // sort the chosen vertices (few) for the merge with the free list:
   // odd-even transposition over the first nch slots
 #pragma unroll
@@ -294,60 +351,47 @@ __device__ __forceinline__ bool emit_rank(const ThinArgs& a, const thin::Family*
 #pragma unroll
     for (int q = pass & 1; q + 1 < NM; q += 2)
       if ((u32)(q + 1) < nch && ch[q] > ch[q + 1]) { const u32 tmp = ch[q]; ch[q] = ch[q + 1]; ch[q + 1] = tmp; }
-  const u32 nr = f->nfree + nch;
   const u32* fr = f->freep;
-  const u32 nl = f->j;
-  u64* fs = a.fp + (stripe % NSTRIPE) * 2;
-  atomicAdd((unsigned long long*)&fs[0], (unsigned long long)fpv);
-  atomicXor((unsigned long long*)&fs[1], (unsigned long long)fpv);
-  if (a.keep) {
-    const u64 len = 2ull + nl + nr;
-    cost.bytes += 4ull * len;
-    const u64 pos = atomicAdd((unsigned long long*)a.res_words, (unsigned long long)len);
-    if (pos + len > a.res_cap) {
-      atomicOr(a.flags, FLAG_RES_OVERFLOW);
-    } else {
-      // tuple side and generated side, left side first
-      u64 at_t = a.swap_sides ? pos + 1 + nr + 1 : pos + 1;
-      u64 at_r = a.swap_sides ? pos + 1 : pos + 1 + nl + 1;
-      a.res[a.swap_sides ? pos + 1 + nr : pos] = nl;
-      a.res[a.swap_sides ? pos : pos + 1 + nl] = nr;
-      for (u32 i = 0; i < nl; ++i) a.res[at_t + i] = f->L[i];
-      u32 p = 0, q = 0;
-      for (u32 i = 0; i < nr; ++i) {
-        u32 cq = 0xFFFFFFFFu;  // ch[q], or none
+  // tuple side and generated side, left side first
+  const u64 at_t = a.swap_sides ? pos + 1 + nr + 1 : pos + 1;
+  const u64 at_r = a.swap_sides ? pos + 1 : pos + 1 + nl + 1;
+  a.res[a.swap_sides ? pos + 1 + nr : pos] = nl;
+  a.res[a.swap_sides ? pos : pos + 1 + nl] = nr;
+  for (u32 i = 0; i < nl; ++i) a.res[at_t + i] = f->L[i];
+  u32 p = 0, q = 0;
+  for (u32 i = 0; i < nr; ++i) {
+    u32 cq = 0xFFFFFFFFu;  // ch[q], or none
 #pragma unroll
-        for (int s2 = 0; s2 < NM; ++s2)
-          if ((u32)s2 == q && q < nch) cq = ch[s2];
-        u32 v;
-        if (q >= nch || (p < f->nfree && fr[p] < cq)) v = fr[p++];
-        else { v = cq; ++q; }
-        a.res[at_r + i] = v;
-      }
-    }
+    for (int s2 = 0; s2 < NM; ++s2)
+      if ((u32)s2 == q && q < nch) cq = ch[s2];
+    u32 v;
+    if (q >= nch || (p < f->nfree && fr[p] < cq)) v = fr[p++];
+    else { v = cq; ++q; }
+    a.res[at_r + i] = v;
   }
   return true;
 }
 
-// Checks rank r (one lane): unrank its combination, test maximality, emit.  A
-// rank still open after its first a.budget candidates goes to the deferred
-// list (thin_check_deferred tests the rest, a wave per rank).
+// Checks rank r (one lane): unrank its combination into f and ch, test
+// maximality.  Returns whether it is maximal (the caller emits it).  A rank
+// still open after its first a.budget candidates goes to the deferred list
+// (thin_check_deferred tests the rest, a wave per rank).
 template <int K>
-__device__ __forceinline__ void check_rank(const ThinArgs& a, u64 r, thin::Cost& cost, u64& n_emit, u64& n_tested) {
+__device__ __forceinline__ bool check_rank(const ThinArgs& a, u64 r, const thin::Family*& f, u32 (&ch)[2 * K * K],
+                                           u32& nch, thin::Cost& cost, u64& n_tested) {
   constexpr int NM = 2 * K * K;
-  if (r >= a.r1) return;
-  u32 ch[NM], ctsz[NM];
-  u32 nch = 0;
-  const thin::Family* f = unrank_rank<K>(a, r, ch, ctsz, nch, cost);
+  if (r >= a.r1) return false;
+  u32 ctsz[NM];
+  f = unrank_rank<K>(a, r, ch, ctsz, nch, cost);
   const int m = thin::side_a_maximal<NM>(a.g, f->L, f->j, f->freep, f->nfree, f->fpick, f->nfpick, ch, ctsz, nch,
                                          a.k, cost, a.budget);
   if (m < 0) {
     const u32 i = atomicAdd(a.defer_n, 1u);
     a.defer[i] = r;  // the list holds a launch's ranks
-    return;
+    return false;
   }
   ++n_tested;
-  if (m == 1 && emit_rank<K>(a, f, ch, nch, blockIdx.x * 4 + (threadIdx.x >> 6), cost)) ++n_emit;
+  return m == 1;
 }
 
 // Deferred ranks, one wave each: every lane unranks the combination, the lanes
@@ -367,7 +411,7 @@ __global__ __launch_bounds__(256) void thin_check_deferred(ThinArgs a) {
     if (lane == 0) i = atomicAdd(a.dwork, 1u);
     i = (u32)wave_bcast_u64(i, 0);
     if (i >= nd) break;
-    if (__hip_atomic_load(a.flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & FLAG_LIMIT) break;
+    if (limit_reached(a)) break;
     const u64 r = a.defer[i];
     u32 ch[NM], ctsz[NM];
     u32 nch = 0;
@@ -390,10 +434,8 @@ __global__ __launch_bounds__(256) void thin_check_deferred(ThinArgs a) {
       if (__ballot(joins)) break;
     }
     const bool maximal = cs.decided >= 0 ? cs.decided == 1 : !__ballot(joins);
-    if (lane == 0) {
-      ++n_tested;
-      if (maximal && emit_rank<K>(a, f, ch, nch, blockIdx.x * 4 + (threadIdx.x >> 6), cost)) ++n_emit;
-    }
+    if (lane == 0) ++n_tested;
+    if (emit_wave<K>(a, lane == 0 && maximal, f, ch, nch, blockIdx.x * 4 + (threadIdx.x >> 6), cost)) ++n_emit;
   }
   flush_check_stats<K>(a, cost, n_emit, n_tested);
 }
@@ -401,11 +443,16 @@ __global__ __launch_bounds__(256) void thin_check_deferred(ThinArgs a) {
 // One lane per rank over the whole grid.
 template <int K>
 __global__ __launch_bounds__(256) void thin_check(ThinArgs a) {
+  constexpr int NM = 2 * K * K;
   const u64 r = a.r0 + (u64)blockIdx.x * blockDim.x + threadIdx.x;
   u64 n_emit = 0, n_tested = 0;
   thin::Cost cost;
-  if (!(__hip_atomic_load(a.flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & FLAG_LIMIT))
-    check_rank<K>(a, r, cost, n_emit, n_tested);
+  if (!limit_reached(a)) {
+    const thin::Family* f = nullptr;
+    u32 ch[NM], nch = 0;
+    const bool want = check_rank<K>(a, r, f, ch, nch, cost, n_tested);
+    if (emit_wave<K>(a, want, f, ch, nch, blockIdx.x * 4 + (threadIdx.x >> 6), cost)) ++n_emit;
+  }
   flush_check_stats<K>(a, cost, n_emit, n_tested);
 }
 
@@ -414,6 +461,7 @@ __global__ __launch_bounds__(256) void thin_check(ThinArgs a) {
 // expensive rank does not hold back the chunks after it.
 template <int K>
 __global__ __launch_bounds__(256) void thin_check_chunked(ThinArgs a) {
+  constexpr int NM = 2 * K * K;
   const int lane = threadIdx.x & 63;
   u64 n_emit = 0, n_tested = 0;
   thin::Cost cost;
@@ -423,8 +471,11 @@ __global__ __launch_bounds__(256) void thin_check_chunked(ThinArgs a) {
     chunk = (u32)wave_bcast_u64(chunk, 0);
     const u64 c0 = a.r0 + (u64)chunk * 64;
     if (c0 >= a.r1) break;
-    if (__hip_atomic_load(a.flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & FLAG_LIMIT) break;
-    check_rank<K>(a, c0 + lane, cost, n_emit, n_tested);
+    if (limit_reached(a)) break;
+    const thin::Family* f = nullptr;
+    u32 ch[NM], nch = 0;
+    const bool want = check_rank<K>(a, c0 + lane, f, ch, nch, cost, n_tested);
+    if (emit_wave<K>(a, want, f, ch, nch, blockIdx.x * 4 + (threadIdx.x >> 6), cost)) ++n_emit;
   }
   flush_check_stats<K>(a, cost, n_emit, n_tested);
 }
@@ -449,6 +500,27 @@ hipError_t launch_thin_build(const ThinArgs& a, hipStream_t s) {
       default: return hipErrorInvalidValue;  // j <= k <= 3 here
     }
   }
+  return hipGetLastError();
+}
+
+// One thread per index entry: binary search of the family prefix sums.
+__global__ __launch_bounds__(256) void thin_rank_index(const u64* prefix, u32 nfam, u64 r0, u64 r1, u32 nb,
+                                                       u32* ridx) {
+  const u32 b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b > nb) return;
+  const u64 rb = r0 + ((u64)b << RIDX_LOG);
+  const u64 r = rb < r1 ? rb : r1 - 1;
+  u32 lo = 0, hi = nfam;
+  while (hi - lo > 1) {
+    const u32 mid = (lo + hi) >> 1;
+    if (prefix[mid] <= r) lo = mid; else hi = mid;
+  }
+  ridx[b] = lo;
+}
+
+hipError_t launch_thin_rank_index(const u64* prefix, u32 nfam, u64 r0, u64 r1, u32* ridx, hipStream_t s) {
+  const u32 nb = (u32)((r1 - r0 + (1ull << RIDX_LOG) - 1) >> RIDX_LOG);
+  hipLaunchKernelGGL(thin_rank_index, dim3((nb + 1 + 255) / 256), dim3(256), 0, s, prefix, nfam, r0, r1, nb, ridx);
   return hipGetLastError();
 }
 
