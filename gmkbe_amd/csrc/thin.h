@@ -84,7 +84,10 @@ GMK_HD u64 mul_sat(u64 a, u64 b) {
 
 // Colex unrank: rank r of C(n, c) -> idx[0] < ... < idx[c-1].
 GMK_HD void unrank(u64 r, u32 c, u64 n, u32* idx) {
-  for (u32 t = c; t >= 1; --t) {
+  // c <= 3; constant indices keep idx in registers
+#pragma unroll
+  for (int t = 3; t >= 1; --t) {
+    if ((u32)t > c) continue;
     // largest x in [t-1, n-1] with C(x, t) <= r
     u64 lo = t - 1, hi = n - 1;
     while (lo < hi) {
@@ -318,23 +321,37 @@ GMK_HD void candidate_segments(const Graph& g, u32 j, u32 nfree, const u32* fpic
     cs.decided = (u64)(g.a_hi - g.a_lo) <= j ? 1 : 0;
     return;
   }
-  // the k + 1 members of least degree (excluding `best`)
-  u32 pk[4], pd[4];
+  // the k + 1 members of least degree (excluding `best`); k + 1 <= 4 slots
+  // addressed by constant indices so they stay in registers
+  u32 pk[4] = {0, 0, 0, 0}, pd[4] = {0, 0, 0, 0};
   u32 np = 0;
   auto take = [&](u32 y) {
     const u32 d = g.off[y + 1] - g.off[y];
     cost.bytes += 8;
-    if (np < k + 1) { pk[np] = y; pd[np] = d; ++np; return; }
-    u32 worst = 0;
-    for (u32 q = 1; q < np; ++q) if (pd[q] > pd[worst]) worst = q;
-    if (d < pd[worst]) { pk[worst] = y; pd[worst] = d; }
+    if (np < k + 1) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        if ((u32)q == np) { pk[q] = y; pd[q] = d; }
+      ++np;
+      return;
+    }
+    u32 worst = 0, wd = pd[0];
+#pragma unroll
+    for (int q = 1; q < 4; ++q)
+      if ((u32)q < np && pd[q] > wd) { worst = q; wd = pd[q]; }
+    if (d < wd) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        if ((u32)q == worst) { pk[q] = y; pd[q] = d; }
+    }
   };
 #pragma unroll
   for (int i = 0; i < NMAX; ++i)
     if ((u32)i < nch && ch[i] != best) take(ch[i]);
   for (u32 i = 0; i < nfpick; ++i) take(fpick[i]);
   u64 psum = 0;
-  for (u32 q = 0; q < np; ++q) psum += pd[q];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) psum += (u32)q < np ? pd[q] : 0u;
   if (best != 0xFFFFFFFFu && !(np == k + 1 && psum < bestdeg)) {
     cs.lst[0] = g.adj + g.off[best];
     cs.len[0] = bestdeg;
@@ -342,10 +359,12 @@ GMK_HD void candidate_segments(const Graph& g, u32 j, u32 nfree, const u32* fpic
     cs.n = 1;
     return;
   }
-  for (u32 q = 0; q < np; ++q) {
-    cs.lst[q] = g.adj + g.off[pk[q]];
-    cs.len[q] = pd[q];
-    cs.known[q] = pk[q];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const bool on = (u32)q < np;
+    cs.lst[q] = on ? g.adj + g.off[pk[q]] : g.adj;
+    cs.len[q] = on ? pd[q] : 0u;
+    cs.known[q] = on ? pk[q] : 0xFFFFFFFFu;
   }
   cs.n = np;
   if (best != 0xFFFFFFFFu) {
@@ -356,13 +375,13 @@ GMK_HD void candidate_segments(const Graph& g, u32 j, u32 nfree, const u32* fpic
   }
 }
 
-// Can candidate w (from list `seg`) be added?
+// Can candidate w be added?  `known`: the member whose list produced w.
 template <int NMAX>
 GMK_HD bool candidate_joins(const Graph& g, const u32* L, u32 j, const u32* freel, u32 nfree, const u32* ch,
-                            const u32* ctsz, u32 nch, u32 k, const CandSegs& cs, u32 seg, u32 w, Cost& cost) {
+                            const u32* ctsz, u32 nch, u32 k, const CandSegs& cs, u32 known, u32 w, Cost& cost) {
   if (in_tuple(L, j, w)) return false;
   if (cs.must != 0xFFFFFFFFu && !adjacent(cs.must_nh, cs.must_nb, cs.must_deg, w, cost)) return false;
-  return addable<NMAX>(g, w, freel, nfree, ch, ctsz, nch, k, cs.known[seg], cost);
+  return addable<NMAX>(g, w, freel, nfree, ch, ctsz, nch, k, known, cost);
 }
 
 // 1 if (L, R) is maximal on side A, 0 if not; -1 when the first `budget`
@@ -376,12 +395,18 @@ GMK_HD int side_a_maximal(const Graph& g, const u32* L, u32 j, const u32* freel,
   candidate_segments<NMAX>(g, j, nfree, fpick, nfpick, ch, ctsz, nch, k, cs, cost);
   if (cs.decided >= 0) return cs.decided;
   u32 tested = 0;
-  for (u32 q = 0; q < cs.n; ++q)
-    for (u32 e = 0; e < cs.len[q]; ++e) {
+  // at most 4 lists, visited by constant index (cs stays in registers)
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    if ((u32)q >= cs.n) break;
+    const u32* lst = cs.lst[q];
+    const u32 len = cs.len[q], known = cs.known[q];
+    for (u32 e = 0; e < len; ++e) {
       if (tested++ == budget) return -1;
       cost.bytes += 4;
-      if (candidate_joins<NMAX>(g, L, j, freel, nfree, ch, ctsz, nch, k, cs, q, cs.lst[q][e], cost)) return 0;
+      if (candidate_joins<NMAX>(g, L, j, freel, nfree, ch, ctsz, nch, k, cs, known, lst[e], cost)) return 0;
     }
+  }
   return 1;
 }
 

@@ -48,8 +48,10 @@ hipError_t launch_thin_build(const ThinArgs& a, hipStream_t s);
 enum : u32 { CHECK_GRID = 0, CHECK_CHUNKED = 1 };
 hipError_t launch_thin_check(const ThinArgs& a, hipStream_t s, u32 max_blocks, u32 form);
 hipError_t launch_thin_check_deferred(const ThinArgs& a, hipStream_t s, u32 blocks);
-// Neighbour hashes of the nv vertices in verts (slots already set to NH_EMPTY).
-hipError_t launch_build_neighbour_hash(const u32* off, const u32* adj, const u32* verts, u32 nv, const u64* nh_off,
+// Neighbour hashes from nv work items (vertex, first neighbour index), each
+// inserting up to NH_CHUNK neighbours of its vertex (slots already NH_EMPTY).
+constexpr u32 NH_CHUNK = 2048;
+hipError_t launch_build_neighbour_hash(const u32* off, const u32* adj, const u32* items, u32 nv, const u64* nh_off,
                                        u32* nh, hipStream_t s);
 hipError_t launch_thin_scan(const thin::Family* fams, u32 nfam, u64* prefix, hipStream_t s);
 // Rank index of a check launch: ridx[b] = family of rank min(r0 + b 2^RIDX_LOG,

@@ -221,7 +221,7 @@ int gmk::build_neighbour_hashes(gmk_ctx* c) {
   if (const char* e = getenv("GMK_NHASH_MIN")) nh_min = (u32)std::min<unsigned long>(strtoul(e, nullptr, 10), ~0u);
   if (nh_min == 0) return GMK_OK;
   std::vector<u64> noff(c->n, 0);
-  std::vector<u32> verts;
+  std::vector<u32> verts;  // work items: (vertex, first neighbour of its chunk)
   u64 slots = 0;
   for (u32 v = 0; v < c->n; ++v) {
     const u32 d = c->off[v + 1] - c->off[v];
@@ -229,7 +229,7 @@ int gmk::build_neighbour_hashes(gmk_ctx* c) {
     if (d >= (1u << 30)) { c->err = "a vertex has 2^30 or more neighbours"; return GMK_EUNSUPPORTED; }
     noff[v] = slots;
     slots += 1ull << thin::nh_log2(d);
-    verts.push_back(v);
+    for (u32 c0 = 0; c0 < d; c0 += NH_CHUNK) { verts.push_back(v); verts.push_back(c0); }
   }
   if (verts.empty()) return GMK_OK;
   int r;
@@ -240,7 +240,7 @@ int gmk::build_neighbour_hashes(gmk_ctx* c) {
   HIPCHK(c, hipMemcpyAsync(c->nh_off.p, noff.data(), (size_t)c->n * 8, hipMemcpyHostToDevice, s));
   HIPCHK(c, hipMemcpyAsync(c->nh_verts.p, verts.data(), verts.size() * 4, hipMemcpyHostToDevice, s));
   HIPCHK(c, hipMemsetAsync(c->nh.p, 0xFF, (size_t)slots * 4, s));
-  HIPCHK(c, launch_build_neighbour_hash(c->d_off, c->d_adj, (const u32*)c->nh_verts.p, (u32)verts.size(),
+  HIPCHK(c, launch_build_neighbour_hash(c->d_off, c->d_adj, (const u32*)c->nh_verts.p, (u32)(verts.size() / 2),
                                         (const u64*)c->nh_off.p, (u32*)c->nh.p, s));
   HIPCHK(c, hipStreamSynchronize(s));
   c->nh_min = nh_min;

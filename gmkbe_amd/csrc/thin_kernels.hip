@@ -251,10 +251,13 @@ __device__ __forceinline__ const thin::Family* unrank_rank(const ThinArgs& a, u6
     const u64 radix = thin::binom(f->psize[t], f->cnt[t]);
     const u64 rt = rr % radix;
     rr /= radix;
-    u32 idx[3];
-    thin::unrank(rt, f->cnt[t], f->psize[t], idx);
+    u32 idx[3] = {0, 0, 0};
+    const u32 cnt = f->cnt[t];
+    thin::unrank(rt, cnt, f->psize[t], idx);
     const u32 tsz = thin::popc(f->mask[t]);
-    for (u32 q = 0; q < f->cnt[t]; ++q) {
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      if ((u32)q >= cnt) break;
       const u32 y = thin::pool_member(*f, t, idx[q], a.g.b_lo, cost);
 #pragma unroll
       for (int s2 = 0; s2 < NM; ++s2)
@@ -419,17 +422,27 @@ __global__ __launch_bounds__(256) void thin_check_deferred(ThinArgs a) {
     thin::CandSegs cs;
     thin::candidate_segments<NM>(a.g, f->j, f->nfree, f->fpick, f->nfpick, ch, ctsz, nch, a.k, cs, cost);
     u64 total = 0;
-    for (u32 q = 0; q < cs.n; ++q) total += cs.len[q];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) total += (u32)q < cs.n ? cs.len[q] : 0u;
     bool joins = false;
     for (u64 base = a.budget; base < total; base += 64) {
       const u64 idx = base + lane;
       if (idx < total) {
-        u32 q = 0;
+        // list and position of flattened candidate idx (constant indices into cs)
         u64 o = idx;
-        while (o >= cs.len[q]) { o -= cs.len[q]; ++q; }
+        const u32* lst = cs.lst[0];
+        u32 known = cs.known[0];
+        bool found = false;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          if (!found && (u32)q < cs.n) {
+            if (o < cs.len[q]) { lst = cs.lst[q]; known = cs.known[q]; found = true; }
+            else o -= cs.len[q];
+          }
+        }
         cost.bytes += 4;
-        joins = thin::candidate_joins<NM>(a.g, f->L, f->j, f->freep, f->nfree, ch, ctsz, nch, a.k, cs, q,
-                                          cs.lst[q][o], cost);
+        joins = thin::candidate_joins<NM>(a.g, f->L, f->j, f->freep, f->nfree, ch, ctsz, nch, a.k, cs, known,
+                                          lst[o], cost);
       }
       if (__ballot(joins)) break;
     }
@@ -529,27 +542,30 @@ hipError_t launch_thin_scan(const thin::Family* fams, u32 nfam, u64* prefix, hip
   return hipGetLastError();
 }
 
-// One wave per hashed vertex: lanes insert its neighbours by compare-and-swap
-// with linear probing (neighbours are distinct, the set has >= 2d slots).
-__global__ __launch_bounds__(256) void build_neighbour_hash(const u32* off, const u32* adj, const u32* verts, u32 nv,
+// One wave per work item (a hashed vertex and a chunk of NH_CHUNK of its
+// neighbours, so hubs spread over many waves): lanes insert the neighbours by
+// compare-and-swap with linear probing (neighbours are distinct, the set has
+// >= 2d slots).
+__global__ __launch_bounds__(256) void build_neighbour_hash(const u32* off, const u32* adj, const u32* items, u32 nv,
                                                             const u64* nh_off, u32* nh) {
   const u32 wv = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
   if (wv >= nv) return;
-  const u32 v = verts[wv];
+  const u32 v = items[2 * wv], c0 = items[2 * wv + 1];
   const u32 o = off[v], d = off[v + 1] - o;
   const u32 lg = thin::nh_log2(d), mask = (1u << lg) - 1u;
   u32* t = nh + nh_off[v];
-  for (u32 i = lane; i < d; i += 64) {
+  const u32 c1 = d - c0 < NH_CHUNK ? d : c0 + NH_CHUNK;
+  for (u32 i = c0 + lane; i < c1; i += 64) {
     const u32 x = adj[o + i];
     u32 sl = thin::nh_slot(x, lg);
     while (atomicCAS(&t[sl], thin::NH_EMPTY, x) != thin::NH_EMPTY) sl = (sl + 1) & mask;
   }
 }
 
-hipError_t launch_build_neighbour_hash(const u32* off, const u32* adj, const u32* verts, u32 nv, const u64* nh_off,
+hipError_t launch_build_neighbour_hash(const u32* off, const u32* adj, const u32* items, u32 nv, const u64* nh_off,
                                        u32* nh, hipStream_t s) {
   if (nv == 0) return hipSuccess;
-  hipLaunchKernelGGL(build_neighbour_hash, dim3((nv + 3) / 4), dim3(256), 0, s, off, adj, verts, nv, nh_off, nh);
+  hipLaunchKernelGGL(build_neighbour_hash, dim3((nv + 3) / 4), dim3(256), 0, s, off, adj, items, nv, nh_off, nh);
   return hipGetLastError();
 }
 
