@@ -69,7 +69,7 @@ extern "C" void gmk_destroy(gmk_ctx* c) {
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
   for (DeviceVec* v : {&c->rows, &c->maps, &c->parts, &c->res, &c->counters, &c->tuples, &c->arena, &c->fams,
-                        &c->prefix, &c->tcnt, &c->tdefer, &c->ridx, &c->nh_off, &c->nh, &c->nh_verts, &c->dedup, &c->gcnt,
+                        &c->prefix, &c->tcnt, &c->tdefer, &c->ridx, &c->tcount, &c->nh_off, &c->nh, &c->nh_verts, &c->dedup, &c->gcnt,
                         &c->bcsr_off, &c->bcsr_adj, &c->bcsr_base, &c->rowpids})
     if (v->p) (void)hipFree(v->p);
   for (int l = 0; l < 7; ++l)

@@ -99,7 +99,7 @@ struct gmk_ctx {
   DeviceVec lane_stack[7], lane_next[7], lane_retry[7];  // state stacks per bitset width
   DeviceVec lane_spill[7];  // shallow states moved out of a nearly full stack
   DeviceVec rows, maps, parts, res, counters;
-  DeviceVec tuples, arena, fams, prefix, tcnt, tdefer, ridx;  // thin-result kernels
+  DeviceVec tuples, arena, fams, prefix, tcnt, tdefer, ridx, tcount;  // thin-result kernels
   DeviceVec nh_off, nh, nh_verts;  // neighbour hashes of the vertices of degree >= nh_min
   u32 nh_min = 0xFFFFFFFFu;        // as built for the current graph (none: all ones)
   u64 nh_slots = 0;

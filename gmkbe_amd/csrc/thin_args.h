@@ -21,6 +21,8 @@ struct ThinArgs {
   u32* tflags;          // family-construction overflow flags
   u32* work;            // chunk counter of a check launch (zeroed before it)
   u32 budget;           // candidates a lane tests before deferring the rest of a rank
+  u32 wave1_max;        // one-vertex tuples per launch below which a wave builds each tuple
+  u32* tcount;          // that build: families per tuple, then their offsets (2 x tuples per launch)
   u64* defer;           // deferred ranks of a check launch (capacity: the launch's ranks)
   u32* defer_n;         // their number (zeroed before the launch)
   u32* dwork;           // next deferred rank to take (zeroed before the launch)
@@ -53,6 +55,9 @@ hipError_t launch_thin_check_deferred(const ThinArgs& a, hipStream_t s, u32 bloc
 constexpr u32 NH_CHUNK = 2048;
 hipError_t launch_build_neighbour_hash(const u32* off, const u32* adj, const u32* items, u32 nv, const u64* nh_off,
                                        u32* nh, hipStream_t s);
+// Launches of fewer one-vertex tuples than this build them a wave per tuple (a
+// lane per tuple would not fill the device); ThinArgs.wave1_max holds it.
+constexpr u32 WAVE1_MAX_TUPLES = 16384;
 hipError_t launch_thin_scan(const thin::Family* fams, u32 nfam, u64* prefix, hipStream_t s);
 // Rank index of a check launch: ridx[b] = family of rank min(r0 + b 2^RIDX_LOG,
 // r1 - 1), b = 0 .. ceil((r1 - r0) / 2^RIDX_LOG).

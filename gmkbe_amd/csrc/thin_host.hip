@@ -117,6 +117,7 @@ int gmk_run_thin(gmk_ctx* c, const gmk_params* prm, gmk_stats* st, RunState& rs,
   if ((r = ensure(c, c->tcnt, sizeof(ThinCounters)))) return r;
   if ((r = ensure(c, c->tdefer, RCH * 8))) return r;
   if ((r = ensure(c, c->ridx, ((RCH >> RIDX_LOG) + 2) * 4))) return r;
+  if ((r = ensure(c, c->tcount, (size_t)TCH * 2 * 4))) return r;
   ThinRun T;
   T.c = c; T.prm = prm; T.rs = &rs;
   ThinArgs& a = T.a;
@@ -144,8 +145,13 @@ int gmk_run_thin(gmk_ctx* c, const gmk_params* prm, gmk_stats* st, RunState& rs,
   // defers every open rank): small budgets exercise the deferred pass on
   // small graphs
   if (const char* e = getenv("GMK_CHECK_BUDGET")) a.budget = (u32)std::min<unsigned long>(strtoul(e, nullptr, 10), 1u << 20);
+  // GMK_WAVE1_TUPLES overrides the launch size below which one-vertex tuples
+  // are built a wave each (0: always a lane each), so tests cover both forms
+  a.wave1_max = WAVE1_MAX_TUPLES;
+  if (const char* e = getenv("GMK_WAVE1_TUPLES")) a.wave1_max = (u32)std::min<unsigned long>(strtoul(e, nullptr, 10), ~0u);
   a.defer = (u64*)c->tdefer.p;
   a.ridx = (const u32*)c->ridx.p;
+  a.tcount = (u32*)c->tcount.p;
   a.defer_n = &tc->defer_n;
   a.dwork = &tc->dwork;
   RunCounters* dc = (RunCounters*)c->counters.p;

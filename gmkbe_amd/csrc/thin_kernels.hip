@@ -51,6 +51,94 @@ __global__ __launch_bounds__(256) void thin_build_families(ThinArgs a) {
     atomicAdd((unsigned long long*)&a.stat[((t >> 6) % NSTRIPE) * NSTAT + 4], (unsigned long long)b);
 }
 
+// The k+1 free vertices of least degree (ties: first in the list), as
+// free_picks, by a whole wave: each lane keeps its best k+1 of a strided share,
+// then k+1 rounds of a wave minimum take the overall best.  Returns their
+// number.
+__device__ __forceinline__ u32 wave_free_picks(const ThinArgs& a, const u32* freel, u32 nfree, u32 (&fpick)[4],
+                                               thin::Cost& cost) {
+  const int lane = threadIdx.x & 63;
+  const u32 npk = a.k + 1 < 4 ? a.k + 1 : 4;
+  u64 best[4] = {~0ull, ~0ull, ~0ull, ~0ull};
+  for (u32 i = lane; i < nfree; i += 64) {
+    const u32 y = freel[i];
+    u64 key = ((u64)(a.g.off[y + 1] - a.g.off[y]) << 32) | i;
+    cost.bytes += 12;
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      if ((u32)q < npk && key < best[q]) { const u64 tmp = best[q]; best[q] = key; key = tmp; }
+  }
+  u32 nfp = 0, head = 0;
+  for (u32 q = 0; q < npk; ++q) {
+    u64 mine = ~0ull;
+#pragma unroll
+    for (int h = 0; h < 4; ++h)
+      if ((u32)h == head) mine = best[h];
+    const u64 m = wave_min_u64(mine);
+    if (m == ~0ull) break;
+    if (mine == m) ++head;
+    const u32 y = freel[(u32)(m & 0xFFFFFFFFu)];
+#pragma unroll
+    for (int h = 0; h < 4; ++h)
+      if ((u32)h == nfp) fpick[h] = y;
+    ++nfp;
+  }
+  return nfp;
+}
+
+// Family slots of the ordered build: counting only, or consecutive slots from
+// a tuple's offset.
+struct CountFamilies {
+  u32 n = 0;
+  __device__ thin::Family* next() { ++n; return nullptr; }
+};
+struct SlotFamilies {
+  thin::Family* at;
+  __device__ thin::Family* next() { return at++; }
+};
+
+// One wave per one-vertex tuple (launches too small to fill the device one
+// lane per tuple): as build_families_t for j = 1 (free list = the vertex's CSR
+// row, far pool = everything else), with the lanes splitting the witness scan
+// and the choice of the k+1 lightest free vertices.  Families are written in
+// tuple order, as the lane-per-tuple form of a small launch does, since the
+// order decides which ranks a result limit reaches first: COUNT counts each
+// tuple's families, thin_scan_counts turns the counts into offsets, and the
+// second pass writes them there.
+template <bool COUNT>
+__global__ __launch_bounds__(256) void thin_build_wave1(ThinArgs a) {
+  const int lane = threadIdx.x & 63;
+  const u32 t = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (t >= a.ntuples) return;  // uniform per wave
+  if (!COUNT && __hip_atomic_load(a.fam_n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) > a.fam_cap) return;
+  const u32* L = a.tuples + t;
+  thin::Cost cost;
+  const u32 s = L[0];
+  const u32* ns = a.g.adj + a.g.off[s];
+  const u32 d = a.g.off[s + 1] - a.g.off[s];
+  cost.bytes += 8;
+  const u32 dom = (u32)wave_min_u64((u64)thin::dominance_bound(a.g, L, 1, a.k, ns, d, (u32)lane, 64u, cost));
+  u32 fpick[4];
+  const u32 nfp = wave_free_picks(a, ns, d, fpick, cost);
+  if (lane == 0) {
+    u32 mcnt[2] = {d, 0};
+    u64 poolp[2] = {(u64)ns, 0};
+    const u64 far = (u64)(a.g.b_hi - a.g.b_lo) - d;
+    if (COUNT) {
+      CountFamilies fo;
+      thin::Cost none;
+      thin::emit_families<1>(L, 1, a.k, a.min_b, t, mcnt, far, poolp, ns, d, dom, fpick, nfp, fo, none);
+      a.tcount[t] = fo.n;
+    } else {
+      SlotFamilies fo{a.fams + a.tcount[a.ntuples + t]};
+      thin::emit_families<1>(L, 1, a.k, a.min_b, t, mcnt, far, poolp, ns, d, dom, fpick, nfp, fo, cost);
+    }
+  }
+  if (COUNT) return;  // the writing pass counts the bytes
+  const u64 b = wave_sum_u64(cost.bytes);
+  if (lane == 0 && b) atomicAdd((unsigned long long*)&a.stat[(t % NSTRIPE) * NSTAT + 4], (unsigned long long)b);
+}
+
 // One wave per tuple of j >= 2 vertices without a far pool (j > k): lanes sweep
 // the members' neighbour lists in coalesced chunks; an element is counted once,
 // by the first list that holds it, with its miss type found by binary search in
@@ -144,31 +232,8 @@ __global__ __launch_bounds__(256) void thin_build_wave(ThinArgs a) {
     const u32* freel = a.arena + s_base[wv] + s_off[wv][0];
     u32 dom = thin::dominance_bound(a.g, L, j, a.k, freel, mcnt[0], (u32)lane, 64u, cost);
     dom = (u32)wave_min_u64((u64)dom);
-    // the k+1 free vertices of least degree (ties: first in the list): each
-    // lane keeps its best k+1 of a strided share, then k+1 rounds of a wave
-    // minimum take the overall best
-    const u32 npk = a.k + 1 < 4 ? a.k + 1 : 4;
-    u64 best[4] = {~0ull, ~0ull, ~0ull, ~0ull};
-    for (u32 i = lane; i < mcnt[0]; i += 64) {
-      const u32 y = freel[i];
-      u64 key = ((u64)(a.g.off[y + 1] - a.g.off[y]) << 32) | i;
-      cost.bytes += 12;
-#pragma unroll
-      for (int q = 0; q < 4; ++q)
-        if ((u32)q < npk && key < best[q]) { const u64 tmp = best[q]; best[q] = key; key = tmp; }
-    }
     u32 fpick[4];
-    u32 nfp = 0, head = 0;
-    for (u32 q = 0; q < npk; ++q) {
-      u64 mine = ~0ull;
-#pragma unroll
-      for (int h = 0; h < 4; ++h)
-        if ((u32)h == head) mine = best[h];
-      const u64 m = wave_min_u64(mine);
-      if (m == ~0ull) break;
-      if (mine == m) ++head;
-      fpick[nfp++] = freel[(u32)(m & 0xFFFFFFFFu)];
-    }
+    const u32 nfp = wave_free_picks(a, freel, mcnt[0], fpick, cost);
     if (lane == 0) {
       u64 poolp[1 << J];
       for (u32 mm = 0; mm <= full; ++mm) poolp[mm] = (u64)(a.arena + s_base[wv] + s_off[wv][mm]);
@@ -179,6 +244,37 @@ __global__ __launch_bounds__(256) void thin_build_wave(ThinArgs a) {
   }
   const u64 b = wave_sum_u64(cost.bytes);
   if (lane == 0 && b) atomicAdd((unsigned long long*)&a.stat[(t % NSTRIPE) * NSTAT + 4], (unsigned long long)b);
+}
+
+// Exclusive prefix sums of the per-tuple family counts cnt[0..n) into
+// cnt[n..2n), one block of 1024 threads; sets the launch's family count and
+// flags a full table.
+__global__ __launch_bounds__(1024) void thin_scan_counts(u32* cnt, u32 n, u32* fam_n, u32 cap, u32* tflags) {
+  __shared__ u32 wsum[16];
+  const u32 t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  const u32 seg = (n + 1023) / 1024;
+  const u32 b = t * seg < n ? t * seg : n;
+  const u32 e = b + seg < n ? b + seg : n;
+  u32 s = 0;
+  for (u32 i = b; i < e; ++i) s += cnt[i];
+  u32 inc = s;
+  for (int d = 1; d < 64; d <<= 1) {
+    const u32 o = __shfl_up(inc, d, 64);
+    if ((int)lane >= d) inc += o;
+  }
+  if (lane == 63) wsum[wv] = inc;
+  __syncthreads();
+  if (t == 0) {
+    u32 run = 0;
+    for (int w = 0; w < 16; ++w) { const u32 v = wsum[w]; wsum[w] = run; run += v; }
+  }
+  __syncthreads();
+  u32 run = wsum[wv] + inc - s;
+  for (u32 i = b; i < e; ++i) { cnt[n + i] = run; run += cnt[i]; }
+  if (t == 1023) {
+    *fam_n = run;
+    if (run > cap) atomicOr(tflags, FLAG_TABLE_FULL);
+  }
 }
 
 // Exclusive prefix sums of the family totals (saturating), one block of 1024
@@ -504,6 +600,12 @@ hipError_t launch_thin_build(const ThinArgs& a, hipStream_t s) {
       case 5: hipLaunchKernelGGL(thin_build_wave<5>, g, b, 0, s, a); break;
       default: hipLaunchKernelGGL(thin_build_wave<6>, g, b, 0, s, a); break;
     }
+  } else if (a.j == 1 && a.ntuples < a.wave1_max) {
+    const dim3 g((a.ntuples + 3) / 4), b(256);
+    hipLaunchKernelGGL(thin_build_wave1<true>, g, b, 0, s, a);
+    hipLaunchKernelGGL(thin_scan_counts, dim3(1), dim3(1024), 0, s, a.tcount, a.ntuples, a.fam_n, a.fam_cap,
+                       a.tflags);
+    hipLaunchKernelGGL(thin_build_wave1<false>, g, b, 0, s, a);
   } else {
     const dim3 g((a.ntuples + 255) / 256), b(256);
     switch (a.j) {

@@ -48,3 +48,28 @@ def test_deferred_checks_are_exact(budget, b, form):
             for mode in ("plain", "batched"):
                 res = e.enumerate(k=k, theta=theta, mode=mode, partition="seeded")
                 assert GC.digest_matches(c, O.digest(res.words)), (name, k, theta, mode, b)
+
+
+@pytest.fixture
+def wave1():
+    old = os.environ.get("GMK_WAVE1_TUPLES")
+    yield lambda v: os.environ.__setitem__("GMK_WAVE1_TUPLES", str(v))
+    if old is None:
+        os.environ.pop("GMK_WAVE1_TUPLES", None)
+    else:
+        os.environ["GMK_WAVE1_TUPLES"] = old
+
+
+@pytest.mark.parametrize("limit", [0, 1 << 30], ids=["lane", "wave"])
+def test_one_vertex_tuple_build_forms_are_exact(wave1, limit):
+    """One-vertex tuples are built a lane each (GMK_WAVE1_TUPLES=0) or a wave
+    each (every launch below the limit): the same canonical sets."""
+    wave1(limit)
+    with Engine(0) as e:
+        for name, k, theta in CASES:
+            c = [c for c in GC.cases() if c["graph"] == name and c["k"] == k and c["theta"] == theta][0]
+            print(f"[gpu] {name} k={k} theta={theta} seeded, wave build below {limit} tuples", flush=True)
+            e.set_graph(GC.graph(name))
+            res = e.enumerate(k=k, theta=theta, mode="batched", partition="seeded")
+            assert GC.digest_matches(c, O.digest(res.words)), (name, k, theta, limit)
+
