@@ -106,7 +106,7 @@ def tune(graphs_per_bucket: int = 24, cap: int = 1000, reps: int = 1) -> dict:
 
     # thin checks per k: one lane per rank over the grid, or waves taking chunks
     # of ranks; seeded, batched, on the workload of that k at its result limit
-    forms = list(base.get("check_form", [0, 0, 0, 1]))
+    forms = list(base.get("check_form", [0, 0, 0, 0]))
     for k, (name, limit) in THIN_WORKLOADS.items():
         eng.set_graph(gen.workload(name))
         best_c = None

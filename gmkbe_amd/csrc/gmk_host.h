@@ -87,7 +87,7 @@ struct gmk_ctx {
   u32 n_cu = 256;                   // compute units of the device
   // launch table measured on MI355X by gmkbe_amd/autotune.py (gmkbe_amd/tune/mi355x.json)
   gmk_tuning tune{GMK_TUNING_VERSION, {0, 0, 0, 0, 0, 0, 0},
-                  {16384, 16384, 16384, 16384, 16384, 16384, 16384}, 500, {0, 0, 0, 1}};
+                  {16384, 16384, 16384, 16384, 16384, 16384, 16384}, 500, {0, 0, 0, 0}};
   hipStream_t stream = nullptr;
   std::string err;
   // graph (host copy kept for partition construction)

@@ -142,7 +142,7 @@ bool read_tuning(const std::string& path, gmk_tuning& t) {
     return true;
   };
   t.version = GMK_TUNING_VERSION;
-  const uint32_t form_default[4] = {0, 0, 0, 1};
+  const uint32_t form_default[4] = {0, 0, 0, 0};
   if (!numbers("check_form", t.check_form, 4)) memcpy(t.check_form, form_default, sizeof form_default);
   return numbers("waves_per_cu", t.waves_per_cu, 7) && numbers("batch_states", t.batch_states, 7) &&
          numbers("dedup_load_permille", &t.dedup_load_permille, 1);

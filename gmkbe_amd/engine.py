@@ -87,7 +87,7 @@ class Tuning(ctypes.Structure):
         for i, v in enumerate(d["batch_states"]):
             t.batch_states[i] = int(v)
         t.dedup_load_permille = int(d.get("dedup_load_permille", 500))
-        for i, v in enumerate(d.get("check_form", [0, 0, 0, 1])):
+        for i, v in enumerate(d.get("check_form", [0, 0, 0, 0])):
             t.check_form[i] = int(v)
         return t
 
