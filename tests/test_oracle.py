@@ -61,6 +61,22 @@ def test_brute_matching_k1(n):
         assert len(L) == 2 and tuple(v - n for v in R) == L
 
 
+@pytest.mark.parametrize("a,b", [(4, 5), (3, 3), (2, 5)])
+@pytest.mark.parametrize("k", [0, 1, 2, 3])
+def test_brute_empty_graph(a, b, k):
+    # No edges: every vertex misses the whole other side, so a k-biplex has at
+    # most k vertices per side and the maximal ones take min(k, side) of each:
+    # C(a, min(k, a)) * C(b, min(k, b)) results when theta <= min(k, a, b).
+    from math import comb
+    g = graphio.from_edges(a, b, np.array([], dtype=np.int64), np.array([], dtype=np.int64))
+    for theta in (1, 2, 3):
+        wb, _ = O.brute(g, k, theta)
+        recs = list(O.records(wb))
+        expect = comb(a, min(k, a)) * comb(b, min(k, b)) if k >= 1 and theta <= min(k, a, b) else 0
+        assert len(recs) == expect
+        assert all(len(L) == min(k, a) and len(R) == min(k, b) for L, R in recs)
+
+
 def test_brute_star():
     # Star K_{1,5} plus a pendant edge on a second centre: k = 1.
     g = graphio.from_edges(2, 6, np.array([0, 0, 0, 0, 0, 1]), np.array([0, 1, 2, 3, 4, 5]))
