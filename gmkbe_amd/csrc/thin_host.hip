@@ -1,7 +1,8 @@
 // Host orchestration of the thin-result kernels: tuples are generated seed by
 // seed (ascending IDs on the tuple side), families built on the device in
 // chunks, then every concrete combination is checked on the device, rank range
-// by rank range, until the result limit is reached.
+// by rank range (each range: rank index, first pass, deferred pass), until the
+// result limit is reached.  Also builds the graph's neighbour hashes at upload.
 #include <algorithm>
 #include <cstddef>
 #include <vector>

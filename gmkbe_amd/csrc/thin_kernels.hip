@@ -1,13 +1,15 @@
-// Thin-result kernels (see thin.h): family construction (one lane per tuple)
-// and concrete-combination checking with emission (one lane per rank).
+// Thin-result kernels (see thin.h): family construction (a lane per tuple, or
+// a wave per tuple for one-vertex tuples in small launches and for tuples of
+// more than k vertices), the per-launch rank index, and concrete-combination
+// checking (a lane per rank, then a wave per rank for those with long
+// candidate lists) with emission a wave at a time; also the neighbour-hash
+// build used by the checks' adjacency tests.
 #include "gmk_device.h"
 #include "thin.h"
 #include "thin_build.h"
 #include "thin_args.h"
 
 namespace gmk {
-
-
 
 struct DevArena {
   u32* base;
