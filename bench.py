@@ -37,7 +37,7 @@ HBM_PEAK_GBS = 8000.0
 WORKLOADS = {
     "cl50kx30k": dict(k=1, theta=1, limit=1_000_000, cpu_sample=4000,
                       desc="Chung-Lu bipartite |U|=50000 |V|=30000 m=500000 alpha=2.1 cap=5000 (seeded), k=1, theta=1, r=1000000"),
-    "er200x300": dict(k=1, theta=1, limit=0, cpu_sample=0,
+    "er200x300": dict(k=1, theta=1, limit=0, cpu_sample=4_397_076,  # the whole set: about 7 s on one core
                       desc="Erdos-Renyi bipartite 200x300 p=0.05 (seeded), k=1, theta=1, full enumeration"),
     "cl2Mx1M": dict(k=2, theta=1, limit=10_000_000, cpu_sample=200,
                     desc="Chung-Lu bipartite |U|=2000000 |V|=1000000 m=20000000 alpha=2.0 (seeded), k=2, theta=1, r=10000000"),
