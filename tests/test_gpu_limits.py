@@ -34,3 +34,21 @@ def test_cl50kx30k_other_k_valid(c2, k, theta, limit):
         res = e.enumerate(k=k, theta=theta, limit=limit, mode="batched", partition="seeded")
     assert res.count == limit
     assert O.validate(c2, k, theta, res.words) == 0
+
+
+@pytest.mark.parametrize("partition,mode", [("global", "batched"), ("seeded", "plain"), ("seeded", "batched")])
+def test_result_buffer_overflow_is_reported(partition, mode):
+    """A record buffer too small for the results (1000 words for 21,087
+    results) ends the call with an overflow error from the expansion (global)
+    or the thin checks (seeded); the next call on the same context returns the
+    reference's set."""
+    import golden_cases as GC
+    from gmkbe_amd.engine import GmkError
+    c = [c for c in GC.cases() if c["graph"] == "er40x50" and c["k"] == 1 and c["theta"] == 1][0]
+    with Engine(0) as e:
+        e.set_graph(GC.graph("er40x50"))
+        print(f"[gpu] er40x50 k=1 theta=1 {partition} {mode} with a 1000-word record buffer", flush=True)
+        with pytest.raises(GmkError, match="result buffer overflow"):
+            e.enumerate(k=1, theta=1, mode=mode, partition=partition, capacity_words=1000)
+        res = e.enumerate(k=1, theta=1, mode=mode, partition=partition)
+        assert GC.digest_matches(c, O.digest(res.words))
