@@ -37,3 +37,18 @@ def test_params_and_stats_layout_match_header():
     assert ctypes.sizeof(engine.Params) == 48
     assert ctypes.sizeof(engine.Tuning) == 4 + 4 * 7 + 4 * 7 + 4 + 4 * 4
     assert ctypes.sizeof(engine.Stats) == 8 * 13 + 8 + 32 * 3
+
+
+def test_committed_tuning_table_is_valid():
+    """The table every Engine loads holds the current version and values the
+    library accepts (gmk_set_tuning's ranges), and survives the ctypes round
+    trip unchanged."""
+    import json
+    with open(engine.TUNE_TABLE) as f:
+        t = json.load(f)["tuning"]
+    assert t["version"] == engine.TUNING_VERSION
+    assert len(t["waves_per_cu"]) == 7 and all(0 <= w <= 64 for w in t["waves_per_cu"])
+    assert len(t["batch_states"]) == 7 and all(64 <= b <= 65536 for b in t["batch_states"])
+    assert 100 <= t["dedup_load_permille"] <= 900
+    assert len(t["check_form"]) == 4 and all(f in (0, 1) for f in t["check_form"])
+    assert engine.Tuning.from_dict(t).as_dict() == t
