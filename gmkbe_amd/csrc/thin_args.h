@@ -22,7 +22,7 @@ struct ThinArgs {
   u32* work;            // chunk counter of a check launch (zeroed before it)
   u32 budget;           // candidates a lane tests before deferring the rest of a rank
   u32 wave1_max;        // one-vertex tuples per launch below which a wave builds each tuple
-  u32* tcount;          // that build: families per tuple, then their offsets (2 x tuples per launch)
+  u32* tcount;          // that build: families per tuple, their offsets, 6 words kept per tuple (8 x tuples)
   u64* defer;           // deferred ranks of a check launch (capacity: the launch's ranks)
   u32* defer_n;         // their number (zeroed before the launch)
   u32* dwork;           // next deferred rank to take (zeroed before the launch)

@@ -118,7 +118,7 @@ int gmk_run_thin(gmk_ctx* c, const gmk_params* prm, gmk_stats* st, RunState& rs,
   if ((r = ensure(c, c->tcnt, sizeof(ThinCounters)))) return r;
   if ((r = ensure(c, c->tdefer, RCH * 8))) return r;
   if ((r = ensure(c, c->ridx, ((RCH >> RIDX_LOG) + 2) * 4))) return r;
-  if ((r = ensure(c, c->tcount, (size_t)TCH * 2 * 4))) return r;
+  if ((r = ensure(c, c->tcount, (size_t)TCH * 8 * 4))) return r;
   ThinRun T;
   T.c = c; T.prm = prm; T.rs = &rs;
   ThinArgs& a = T.a;

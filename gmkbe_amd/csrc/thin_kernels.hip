@@ -104,15 +104,16 @@ struct SlotFamilies {
 // row, far pool = everything else), with the lanes splitting the witness scan
 // and the choice of the k+1 lightest free vertices.  Families are written in
 // tuple order, as the lane-per-tuple form of a small launch does, since the
-// order decides which ranks a result limit reaches first: COUNT counts each
-// tuple's families, thin_scan_counts turns the counts into offsets, and the
-// second pass writes them there.
-template <bool COUNT>
+// order decides which ranks a result limit reaches first: this kernel counts
+// each tuple's families and keeps its witness bound and lightest vertices,
+// thin_scan_counts turns the counts into offsets, and thin_emit_wave1 writes
+// the families there.
+//   tcount[t]: families of tuple t; tcount[n + t]: their offset;
+//   tcount[2n + 6t ..]: witness bound, number of picks, up to 4 picks.
 __global__ __launch_bounds__(256) void thin_build_wave1(ThinArgs a) {
   const int lane = threadIdx.x & 63;
   const u32 t = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (t >= a.ntuples) return;  // uniform per wave
-  if (!COUNT && __hip_atomic_load(a.fam_n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) > a.fam_cap) return;
   const u32* L = a.tuples + t;
   thin::Cost cost;
   const u32 s = L[0];
@@ -120,25 +121,48 @@ __global__ __launch_bounds__(256) void thin_build_wave1(ThinArgs a) {
   const u32 d = a.g.off[s + 1] - a.g.off[s];
   cost.bytes += 8;
   const u32 dom = (u32)wave_min_u64((u64)thin::dominance_bound(a.g, L, 1, a.k, ns, d, (u32)lane, 64u, cost));
-  u32 fpick[4];
+  u32 fpick[4] = {0, 0, 0, 0};
   const u32 nfp = wave_free_picks(a, ns, d, fpick, cost);
   if (lane == 0) {
     u32 mcnt[2] = {d, 0};
     u64 poolp[2] = {(u64)ns, 0};
-    const u64 far = (u64)(a.g.b_hi - a.g.b_lo) - d;
-    if (COUNT) {
-      CountFamilies fo;
-      thin::Cost none;
-      thin::emit_families<1>(L, 1, a.k, a.min_b, t, mcnt, far, poolp, ns, d, dom, fpick, nfp, fo, none);
-      a.tcount[t] = fo.n;
-    } else {
-      SlotFamilies fo{a.fams + a.tcount[a.ntuples + t]};
-      thin::emit_families<1>(L, 1, a.k, a.min_b, t, mcnt, far, poolp, ns, d, dom, fpick, nfp, fo, cost);
-    }
+    CountFamilies fo;
+    thin::Cost none;
+    thin::emit_families<1>(L, 1, a.k, a.min_b, t, mcnt, (u64)(a.g.b_hi - a.g.b_lo) - d, poolp, ns, d, dom, fpick,
+                           nfp, fo, none);
+    a.tcount[t] = fo.n;
+    u32* keep = a.tcount + 2ull * a.ntuples + 6ull * t;
+    keep[0] = dom;
+    keep[1] = nfp;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) keep[2 + q] = fpick[q];
   }
-  if (COUNT) return;  // the writing pass counts the bytes
   const u64 b = wave_sum_u64(cost.bytes);
   if (lane == 0 && b) atomicAdd((unsigned long long*)&a.stat[(t % NSTRIPE) * NSTAT + 4], (unsigned long long)b);
+}
+
+// One lane per one-vertex tuple: its families, from what thin_build_wave1
+// kept, at the offset thin_scan_counts gave it.
+__global__ __launch_bounds__(256) void thin_emit_wave1(ThinArgs a) {
+  const u32 t = blockIdx.x * blockDim.x + threadIdx.x;
+  thin::Cost cost;
+  if (t < a.ntuples && __hip_atomic_load(a.fam_n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <= a.fam_cap) {
+    const u32* L = a.tuples + t;
+    const u32 s = L[0];
+    const u32* ns = a.g.adj + a.g.off[s];
+    const u32 d = a.g.off[s + 1] - a.g.off[s];
+    const u32* keep = a.tcount + 2ull * a.ntuples + 6ull * t;
+    const u32 fpick[4] = {keep[2], keep[3], keep[4], keep[5]};
+    u32 mcnt[2] = {d, 0};
+    u64 poolp[2] = {(u64)ns, 0};
+    SlotFamilies fo{a.fams + a.tcount[a.ntuples + t]};
+    cost.bytes += 8 + 24;
+    thin::emit_families<1>(L, 1, a.k, a.min_b, t, mcnt, (u64)(a.g.b_hi - a.g.b_lo) - d, poolp, ns, d, keep[0], fpick,
+                           keep[1], fo, cost);
+  }
+  const u64 b = wave_sum_u64(cost.bytes);
+  if ((threadIdx.x & 63) == 0 && b)
+    atomicAdd((unsigned long long*)&a.stat[((t >> 6) % NSTRIPE) * NSTAT + 4], (unsigned long long)b);
 }
 
 // One wave per tuple of j >= 2 vertices without a far pool (j > k): lanes sweep
@@ -604,10 +628,10 @@ hipError_t launch_thin_build(const ThinArgs& a, hipStream_t s) {
     }
   } else if (a.j == 1 && a.ntuples < a.wave1_max) {
     const dim3 g((a.ntuples + 3) / 4), b(256);
-    hipLaunchKernelGGL(thin_build_wave1<true>, g, b, 0, s, a);
+    hipLaunchKernelGGL(thin_build_wave1, g, b, 0, s, a);
     hipLaunchKernelGGL(thin_scan_counts, dim3(1), dim3(1024), 0, s, a.tcount, a.ntuples, a.fam_n, a.fam_cap,
                        a.tflags);
-    hipLaunchKernelGGL(thin_build_wave1<false>, g, b, 0, s, a);
+    hipLaunchKernelGGL(thin_emit_wave1, dim3((a.ntuples + 255) / 256), b, 0, s, a);
   } else {
     const dim3 g((a.ntuples + 255) / 256), b(256);
     switch (a.j) {
