@@ -243,7 +243,7 @@ def main() -> int:
         return 0
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the enumerator is single-device and "
-              "its multi-GPU seed sharding is a design (DESIGN.md), not a measured path.", file=sys.stderr)
+              "its multi-GPU seed sharding is designed, not built or measured.", file=sys.stderr)
         return 2
 
     from gmkbe_amd import gen

@@ -14,7 +14,7 @@ namespace thin {
 // (among the first 16), and each candidate's misses are counted by binary
 // search until they reach the best seen.  Returns k + 1 when no w with at most
 // k misses was seen.  Any value returned is attained by a real w, so a family
-// judged dominated with it is non-maximal (a witness exists); the bound only
+// found dominated with it is non-maximal (a witness exists); the bound only
 // decides how many families are recognised.  cand0/cstep split the candidates
 // between cooperating lanes (0/1 for one lane).
 GMK_HD u32 dominance_bound(const Graph& g, const u32* L, u32 j, u32 k, const u32* freel, u32 nfree, u32 cand0,

@@ -17,7 +17,7 @@
 // nL..N-1 the right side, neighbour lists sorted ascending.  A result is recorded as
 // u32 words  [|L|, L ascending..., |R|, R ascending...]  with global IDs.
 //
-// Parity status of each function is stated in DESIGN.md and pinned by tests:
+// Each function is pinned by tests:
 //   or_bppivot  vs the compiled reference binary (oracle/_ref/bppivot) golden
 //               vectors and vs or_brute;  or_brute vs closed forms (K_{a,b},
 //               matchings, stars);  or_convert byte-exact vs oracle/_ref/graph_convert.
