@@ -154,19 +154,24 @@ __device__ void emit_state(Smem<W>& sm, const PartDesc& P, const u32* maps, cons
   const u64* SR = setp(sm.set, 1, 0, W);
   const u32 nl = popc_set<W>(SL, P.wl), nr = popc_set<W>(SR, P.wr);
   // fingerprint first: it decides whether the record is a duplicate
-  u64 h = 0;
+  u64 h = 0, h2 = 0;
   for (int side = 0; side < 2; ++side) {
     const u64* Sx = side ? SR : SL;
     const u32 wx = side ? P.wr : P.wl;
     const u32* map = maps + (side ? P.map_r : P.map_l);
     for (u32 i = 0; i < wx; ++i)
-      if ((Sx[i] >> lane) & 1ull) h += mix64(map[i * 64 + lane]);
+      if ((Sx[i] >> lane) & 1ull) {
+        const u32 id = map[i * 64 + lane];
+        h += mix64(id);
+        if (a.dedup.on) h2 += mix64b(id);  // the dedup table's check word
+      }
   }
   h = wave_sum_u64(h);
   const u64 fp = fmix64(h + (u64)nl * FP_A + (u64)nr * FP_B);
+  if (a.dedup.on) h2 = fmix64(wave_sum_u64(h2) + (u64)nl * FP_B + (u64)nr * FP_A);
   u64 slot = 0;
   if (lane == 0) {
-    bool take = dedup_insert(a.dedup, fp, a.flags, blockIdx.x);
+    bool take = dedup_insert(a.dedup, fp, h2, a.flags, blockIdx.x);
     if (take && a.gcount) {
       // per-graph cap and digest
       const unsigned long long gs = atomicAdd(&a.gcount[P.graph], 1ull);

@@ -297,6 +297,8 @@ int BBEngine::step() {
   a.fp = dc->fp;
   a.stat = dc->stat;
   a.flags = &dc->flags;
+  // each state emits at most one result
+  if ((r = dedup_reserve(c_, rs_, c_->h_cnt->res_count + (u64)NLANE * kBatchMax))) return r;
   a.dedup = rs_.dedup;
   a.gcount = rs_.gcount;
   a.gfp = rs_.gfp;

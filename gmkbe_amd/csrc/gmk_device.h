@@ -47,36 +47,63 @@ struct PartDesc {
   u32 graph;    // graph of a multi-graph batch (0 otherwise)
 };
 
-// Result dedup table: open addressing over 64-bit fingerprints (0 = empty),
-// linear probing from a slot taken from a second mix of the key, insertion by
-// compare-and-swap.  A fingerprint found already present is a duplicate.
+// Second, independent per-vertex mix for the dedup table's check word.
+__host__ __device__ __forceinline__ u64 mix64b(u64 x) { return fmix64(x * 0xD1B54A32D192ED03ull + 0x8CB92BA72F3D8DD7ull); }
+
+// Result dedup table: open addressing over two independent 64-bit hashes of a
+// result, kept in one 16-byte slot: word 0 the fingerprint (0 = empty slot),
+// word 1 a check hash.  Linear probing from a slot taken from a second mix of
+// the fingerprint; a slot is claimed by compare-and-swap on word 0 and its check
+// word is written right after.  A result is a duplicate only when both words
+// match, so two distinct results whose fingerprints collide are both kept.
 struct DedupTable {
-  unsigned long long* slots;  // capacity = mask + 1 (power of two)
+  unsigned long long* slots;  // 2 (mask + 1) words; capacity mask + 1, a power of two
   u64 mask;
-  u64* stats;                 // [NSTRIPE][4]: inserts, probes, duplicates
+  u64* stats;                 // [NSTRIPE][4]: inserts, probes, duplicates, undecided
   int on;
 };
+constexpr u32 DEDUP_SPIN = 1u << 14;  // bounded wait for a just-claimed slot's check word
 
-// Returns false when fp is already in the table (the result is a duplicate).
-__device__ __forceinline__ bool dedup_insert(const DedupTable& t, u64 fp, u32* flags, u32 stripe) {
+// Returns false when (fp, fp2) is already in the table (the result is a
+// duplicate).  A slot with the same fingerprint whose check word is not yet
+// visible after DEDUP_SPIN polls leaves the result undecided: it is kept
+// (never dropped on a guess) and counted in stats[3].
+__device__ __forceinline__ bool dedup_insert(const DedupTable& t, u64 fp, u64 fp2, u32* flags, u32 stripe) {
   if (!t.on) return true;
-  const unsigned long long key = fp ? fp : 1ull;
+  const unsigned long long key = fp ? fp : 1ull, key2 = fp2 ? fp2 : 1ull;
   u64 h = mix64(key ^ 0xD6E8FEB86659FD93ull) & t.mask;
   u64 probes = 0;
-  bool fresh = true;
-  for (;;) {
+  int state = 0;  // 1 new, 2 duplicate, 3 undecided, 4 table full
+  while (state == 0) {
     ++probes;
-    const unsigned long long old = atomicCAS(&t.slots[h], 0ull, key);
-    if (old == 0ull) break;
-    if (old == key) { fresh = false; break; }
-    h = (h + 1) & t.mask;
-    if (probes > t.mask) { atomicOr(flags, 4u /* FLAG_TABLE_FULL */); break; }
+    unsigned long long* sl = t.slots + 2 * h;
+    const unsigned long long old = atomicCAS(&sl[0], 0ull, key);
+    if (old == 0ull) {
+      __hip_atomic_store(&sl[1], key2, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      state = 1;
+    }
+    if (state == 0 && old == key) {
+      unsigned long long k2 = 0;
+      for (u32 spin = 0; spin < DEDUP_SPIN; ++spin) {
+        k2 = __hip_atomic_load(&sl[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+        if (k2) break;
+        __builtin_amdgcn_s_sleep(1);
+      }
+      if (k2 == key2) state = 2;
+      else if (k2 == 0) state = 3;
+    }
+    if (state == 0) {
+      h = (h + 1) & t.mask;
+      if (probes > t.mask) state = 4;
+    }
   }
+  if (state == 4) atomicOr(flags, 4u /* FLAG_TABLE_FULL */);
   u64* st = t.stats + (stripe % NSTRIPE) * 4;
   atomicAdd((unsigned long long*)&st[0], 1ull);
   atomicAdd((unsigned long long*)&st[1], (unsigned long long)probes);
-  if (!fresh) atomicAdd((unsigned long long*)&st[2], 1ull);
-  return fresh;
+  if (state == 2) atomicAdd((unsigned long long*)&st[2], 1ull);
+  if (state == 3) atomicAdd((unsigned long long*)&st[3], 1ull);
+  return state != 2;
 }
 
 struct ExpandArgs {

@@ -88,6 +88,14 @@ extern "C" void gmk_destroy(gmk_ctx* c) {
 
 extern "C" const char* gmk_last_error(gmk_ctx* c) { return c ? c->err.c_str() : "null context"; }
 
+extern "C" int gmk_device_name(gmk_ctx* c, char* out, uint32_t cap) {
+  if (!c || !out || cap == 0) return GMK_EINVAL;
+  hipDeviceProp_t prop;
+  HIPCHK(c, hipGetDeviceProperties(&prop, c->device));
+  snprintf(out, cap, "%s", prop.name);
+  return GMK_OK;
+}
+
 static int upload_graph(gmk_ctx* c) {
   HIPCHK(c, hipSetDevice(c->device));
   if (c->d_off) (void)hipFree(c->d_off);
@@ -120,6 +128,25 @@ static int check_csr(gmk_ctx* c) {
       }
     }
   }
+  // Symmetry: visiting the vertices in ascending order, the vertices that list
+  // w arrive in ascending order, so they must be exactly N(w), in order.  One
+  // cursor per vertex walks N(w) as its listers arrive.
+  std::vector<u32> cur(c->off.begin(), c->off.end() - 1);
+  for (u32 v = 0; v < c->n; ++v)
+    for (u32 e = c->off[v]; e < c->off[v + 1]; ++e) {
+      const u32 w = c->adj[e];
+      if (cur[w] >= c->off[w + 1] || c->adj[cur[w]] != v) {
+        c->err = "edge " + std::to_string(v) + " -> " + std::to_string(w) + " has no reverse edge";
+        return GMK_EINVAL;
+      }
+      ++cur[w];
+    }
+  for (u32 w = 0; w < c->n; ++w)
+    if (cur[w] != c->off[w + 1]) {
+      c->err = "vertex " + std::to_string(w) + " lists " + std::to_string(c->adj[cur[w]]) +
+               ", which does not list it";
+      return GMK_EINVAL;
+    }
   return GMK_OK;
 }
 
@@ -219,6 +246,47 @@ extern "C" int gmk_fetch_results(gmk_ctx* c, uint32_t* words, uint64_t cap, uint
 }
 
 // ---------------------------------------------------------------------------
+// Result dedup table growth.
+// ---------------------------------------------------------------------------
+// One thread per slot of the old table: a filled slot (fingerprint, check word)
+// is claimed in the new table at the first empty slot of its probe sequence.
+// The old table holds no two slots with both words equal, so no key test is
+// needed; every slot's check word was written by the launch that claimed it.
+__global__ __launch_bounds__(256) void dedup_rehash(const unsigned long long* old, u64 old_cap,
+                                                    unsigned long long* slots, u64 mask) {
+  const u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= old_cap) return;
+  const unsigned long long key = old[2 * i];
+  if (!key) return;
+  u64 h = mix64(key ^ 0xD6E8FEB86659FD93ull) & mask;
+  while (atomicCAS(&slots[2 * h], 0ull, key) != 0ull) h = (h + 1) & mask;  // the new table has room: load < 1/2
+  slots[2 * h + 1] = old[2 * i + 1];
+}
+
+int gmk::dedup_reserve(gmk_ctx* c, RunState& rs, u64 bound) {
+  if (!rs.dedup.on) return GMK_OK;
+  u64 cap = rs.dedup.mask + 1;
+  if ((double)bound <= (double)cap * rs.dedup_load) return GMK_OK;
+  u64 ncap = cap;
+  while ((double)ncap * rs.dedup_load < (double)bound) ncap <<= 1;
+  if (ncap > (1ull << 34)) { c->err = "dedup table would exceed 2^34 slots"; return GMK_EOVERFLOW; }
+  DeviceVec nv;
+  int r;
+  if ((r = ensure(c, nv, ncap * 16))) return r;
+  HIPCHK(c, hipMemsetAsync(nv.p, 0, ncap * 16, c->stream));
+  const u64 blocks = (cap + 255) / 256;
+  hipLaunchKernelGGL(dedup_rehash, dim3((u32)blocks), dim3(256), 0, c->stream, (const unsigned long long*)c->dedup.p,
+                     cap, (unsigned long long*)nv.p, ncap - 1);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  (void)hipFree(c->dedup.p);
+  c->dedup = nv;
+  rs.dedup.slots = (unsigned long long*)nv.p;
+  rs.dedup.mask = ncap - 1;
+  return GMK_OK;
+}
+
+// ---------------------------------------------------------------------------
 // Branch-and-bound over a set of partitions.
 // ---------------------------------------------------------------------------
 int gmk_pick_w(u32 words) {
@@ -253,18 +321,26 @@ extern "C" int gmk_enumerate(gmk_ctx* c, const gmk_params* prm, gmk_stats* st) {
     rs.res_cap = 0;
   }
   c->res_words = 0;
+  c->h_cnt->res_count = 0;  // read back after every launch; the dedup table is sized from it
   if (prm->dedup) {
-    const u64 expect = prm->limit ? prm->limit : (1ull << 25);
-    const double lf = (prm->dedup_load_permille ? prm->dedup_load_permille : c->tune.dedup_load_permille) / 1000.0;
+    // A limited run sizes the table for the limit; an unlimited one starts at
+    // 2^20 slots and doubles (rehashing on the device) whenever the results so
+    // far plus the most a launch can add would pass the load factor.
+    rs.dedup_load = (prm->dedup_load_permille ? prm->dedup_load_permille : c->tune.dedup_load_permille) / 1000.0;
+    const u64 expect = prm->limit ? prm->limit : (1ull << 19);
     u64 cap = 1024;
-    while ((double)cap * lf < (double)expect) cap <<= 1;
-    if ((r = ensure(c, c->dedup, cap * 8))) return r;
-    HIPCHK(c, hipMemsetAsync(c->dedup.p, 0, cap * 8, c->stream));
+    while ((double)cap * rs.dedup_load < (double)expect) cap <<= 1;
+    if ((r = ensure(c, c->dedup, cap * 16))) return r;
+    HIPCHK(c, hipMemsetAsync(c->dedup.p, 0, cap * 16, c->stream));
     rs.dedup.slots = (unsigned long long*)c->dedup.p;
     rs.dedup.mask = cap - 1;
     rs.dedup.stats = ((RunCounters*)c->counters.p)->dstat;
     rs.dedup.on = 1;
   }
+  // GMK_DEDUP_REPLAY=1 runs every thin check launch twice and pushes every
+  // global root state twice, so each result is produced twice: tests use it to
+  // show that the dedup table drops exactly the repeats.
+  if (const char* e = getenv("GMK_DEDUP_REPLAY")) rs.replay = atoi(e) != 0;
 
   const u32 nr = c->n - c->n_left;
   const u32 wl = (c->n_left + 63) / 64, wr = (nr + 63) / 64;
@@ -275,6 +351,7 @@ extern "C" int gmk_enumerate(gmk_ctx* c, const gmk_params* prm, gmk_stats* st) {
     if (!Wg) { c->err = "graph too large for a global partition"; return GMK_EUNSUPPORTED; }
     PartitionSet ps;
     build_global_partition(c->n, c->n_left, c->off.data(), c->adj.data(), prm->k, prm->theta, Wg, ps);
+    if (rs.replay) ps.roots.insert(ps.roots.end(), ps.roots.begin(), ps.roots.end());
     st->partitions = 1;
     if ((r = gmk_run_bb(c, ps, prm, st, rs))) return r;
   } else {
@@ -302,6 +379,7 @@ extern "C" int gmk_enumerate(gmk_ctx* c, const gmk_params* prm, gmk_stats* st) {
     st->dedup_inserts += h.dstat[4 * i];
     st->dedup_probes += h.dstat[4 * i + 1];
     st->dedup_dups += h.dstat[4 * i + 2];
+    st->dedup_undecided += h.dstat[4 * i + 3];
   }
   if (h.flags & FLAG_TABLE_FULL) { c->err = "dedup table full (raise dedup_load_permille headroom)"; return GMK_EOVERFLOW; }
   st->fp_sum += rs.fp_sum_extra;

@@ -21,6 +21,8 @@ struct PartitionSet {
 
 struct RunState {
   DedupTable dedup{};
+  double dedup_load = 0.5;  // load factor the dedup table is kept under
+  bool replay = false;      // produce every result twice (dedup tests)
   unsigned long long* gcount = nullptr;  // per-graph caps (multi-graph batches)
   u64* gfp = nullptr;
   u64 gcap = ~0ull;
@@ -44,6 +46,11 @@ struct RunState {
   double last_print_ms = -1e30;
   void report(const char* phase, u64 count, u64 live);
 };
+
+// Grows the run's dedup table (doubling, rehashed on the device) so that
+// `bound` results fit under its load factor.  Callers pass the results counted
+// so far plus the most the next launch can add.
+int dedup_reserve(gmk_ctx* c, RunState& rs, u64 bound);
 
 // Whole graph as one partition (local IDs = global IDs, right side shifted).
 void build_global_partition(u32 n, u32 n_left, const u32* off, const u32* adj, u32 k, u32 theta, int W, PartitionSet& ps,

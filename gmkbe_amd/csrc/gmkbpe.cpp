@@ -10,7 +10,9 @@
 //   -g <n>               accepted and ignored, as in the reference
 // Additions: -t <theta> (size lower bound, default 1), --mode plain|batched|learned,
 // --partition auto|global|seeded, --tune <table.json> (launch table written by
-// gmkbe_amd/autotune.py; the library's built-in MI355X table otherwise).
+// gmkbe_amd/autotune.py; the library's built-in MI355X table otherwise),
+// --format gmkbpe|bppivot|itraversal (the -q 1 result lines of the reference,
+// of its BPPivot baseline or iTraversal's result keys, for diffs across tools).
 //
 // Records, as written by the reference:
 //   "Output n, cost Time: t ms" at the milestones {10^i < r} and r
@@ -148,10 +150,15 @@ bool read_tuning(const std::string& path, gmk_tuning& t) {
          numbers("dedup_load_permille", &t.dedup_load_permille, 1);
 }
 
+// Result line formats of -q 1: the reference's own, and those of its two CPU
+// enumerators, so outputs diff line by line across tools.
+enum { FMT_GMKBPE = 0, FMT_BPPIVOT = 1, FMT_ITRAVERSAL = 2 };
+
 void usage() {
   fprintf(stderr,
           "gmkbpe -f <graph.bin> [-k 1] [-r 10000000] [-d 0] [-q 0] [-m 0] [-t 1] "
-          "[--mode plain|batched|learned] [--partition auto|global|seeded] [--tune table.json]\n");
+          "[--mode plain|batched|learned] [--partition auto|global|seeded] [--tune table.json] "
+          "[--format gmkbpe|bppivot|itraversal]\n");
 }
 
 }  // namespace
@@ -163,6 +170,7 @@ int main(int argc, char** argv) {
   unsigned k = 1, theta = 1;
   int mode = GMK_MODE_BATCHED, part = GMK_PART_AUTO;
   std::string tune_path;
+  int format = FMT_GMKBPE;
   for (int i = 1; i < argc; ++i) {
     std::string a = argv[i];
     auto val = [&]() -> const char* {
@@ -185,6 +193,12 @@ int main(int argc, char** argv) {
       part = v == "global" ? GMK_PART_GLOBAL : v == "seeded" ? GMK_PART_SEEDED : GMK_PART_AUTO;
     } else if (a == "--tune") {
       tune_path = val();
+    } else if (a == "--format") {
+      std::string v = val();
+      if (v == "gmkbpe") format = FMT_GMKBPE;
+      else if (v == "bppivot") format = FMT_BPPIVOT;
+      else if (v == "itraversal") format = FMT_ITRAVERSAL;
+      else { fprintf(stderr, "unknown format %s\n", v.c_str()); usage(); return 1; }
     } else if (a == "-h" || a == "--help") { usage(); return 0; }
     else { fprintf(stderr, "unknown option %s\n", a.c_str()); usage(); return 1; }
   }
@@ -207,7 +221,11 @@ int main(int argc, char** argv) {
       return 1;
     }
   }
-  printf("Using GPU id: %d\n", dev);
+  {
+    char name[256] = "";
+    gmk_device_name(ctx, name, sizeof name);
+    printf("Using GPU: %s, GPU id: %d\n", name, dev);  // src/main.cu:1768-1770
+  }
   uint32_t n = 0, nl = 0;
   uint64_t nnz = 0;
   if (gmk_load_graph_bin(ctx, file.c_str(), &n, &nl, &nnz) != GMK_OK) {
@@ -269,10 +287,30 @@ int main(int argc, char** argv) {
     std::string line;
     while (i < nw) {
       const uint32_t a = w[i], b = w[i + 1 + a];
-      line = "Result " + std::to_string(++c) + ": X: ";
-      for (uint32_t t = 0; t < a; ++t) { line += std::to_string(w[i + 1 + t]); line += t + 1 < a ? " " : ""; }
-      line += " | Y: ";
-      for (uint32_t t = 0; t < b; ++t) { line += std::to_string(w[i + 2 + a + t]); line += ' '; }
+      const uint32_t* X = w.data() + i + 1;
+      const uint32_t* Y = w.data() + i + 2 + a;
+      ++c;
+      if (format == FMT_BPPIVOT) {
+        // "Biplex No.n:" then the sorted sides, right labels local to the right
+        // side (baselines/MaximalBiPlex/src/biplex/biplex.cpp:812-813, 605-618)
+        line = "Biplex No." + std::to_string(c) + ":\n  SL: ";
+        for (uint32_t t = 0; t < a; ++t) line += std::to_string(X[t]) + ", ";
+        line += "\n  SR: ";
+        for (uint32_t t = 0; t < b; ++t) line += std::to_string(Y[t] - nl) + ", ";
+      } else if (format == FMT_ITRAVERSAL) {
+        // the result key "x&...y&..." of iTraversal's dedup tree: left then
+        // right, each sorted, global IDs (SGMD22-MBPE/include/iTraversal.h:586-625)
+        line.clear();
+        for (uint32_t t = 0; t < a; ++t) line += std::to_string(X[t]) + "&";
+        for (uint32_t t = 0; t < b; ++t) line += std::to_string(Y[t]) + "&";
+      } else {
+        // include/output.cuh:102-111, 136-144: X joined by spaces, each Y
+        // followed by a space
+        line = "Result " + std::to_string(c) + ": X: ";
+        for (uint32_t t = 0; t < a; ++t) { line += std::to_string(X[t]); line += t + 1 < a ? " " : ""; }
+        line += " | Y: ";
+        for (uint32_t t = 0; t < b; ++t) { line += std::to_string(Y[t]); line += ' '; }
+      }
       puts(line.c_str());
       i += 2 + a + b;
     }

@@ -82,6 +82,47 @@ GMK_HD u64 mul_sat(u64 a, u64 b) {
   return a * b;
 }
 
+// min(a + b, SAT) for a, b <= SAT (associative, so any scan order gives the
+// same prefix sums).
+GMK_HD u64 add_sat(u64 a, u64 b) {
+  const u64 z = a + b;
+  return (z < a || z > SAT) ? SAT : z;
+}
+
+// Exclusive prefix sums of family totals, saturating at SAT, as computed by one
+// block of TSCAN threads (thin_scan_totals): thread t owns a contiguous
+// segment of families, the segment sums are scanned across the block, and each
+// thread writes its segment's prefixes from the sum of the segments before it.
+// The phases are shared with the CPU model that tests them.
+constexpr unsigned TSCAN = 1024;
+GMK_HD void scan_segment(u32 n, u32 t, u32& b, u32& e) {
+  const u32 seg = (n + TSCAN - 1) / TSCAN;
+  b = (u64)t * seg < n ? t * seg : n;
+  e = b + seg < n ? b + seg : n;
+}
+// Phase 1: the sum of thread t's segment.
+template <class TotalAt>
+GMK_HD u64 scan_phase_sum(u32 n, u32 t, TotalAt total) {
+  u32 b, e;
+  scan_segment(n, t, b, e);
+  u64 s = 0;
+  for (u32 i = b; i < e; ++i) s = add_sat(s, total(i));
+  return s;
+}
+// Phase 2 (one step of an inclusive Hillis-Steele scan over the segment sums,
+// d = 1, 2, 4, ..., TSCAN / 2): thread t's new value from the old values.
+GMK_HD u64 scan_phase_step(const u64* part, u32 t, u32 d) { return t >= d ? add_sat(part[t], part[t - d]) : part[t]; }
+// Phase 3: thread t writes its segment's exclusive prefixes, given the
+// inclusive scan `incl` of the segment sums; the last thread writes the total.
+template <class TotalAt>
+GMK_HD void scan_phase_write(u32 n, u32 t, const u64* incl, TotalAt total, u64* prefix) {
+  u32 b, e;
+  scan_segment(n, t, b, e);
+  u64 run = t ? incl[t - 1] : 0;
+  for (u32 i = b; i < e; ++i) { prefix[i] = run; run = add_sat(run, total(i)); }
+  if (t == TSCAN - 1) prefix[n] = incl[TSCAN - 1];
+}
+
 // Colex unrank: rank r of C(n, c) -> idx[0] < ... < idx[c-1].
 GMK_HD void unrank(u64 r, u32 c, u64 n, u32* idx) {
   // c <= 3; constant indices keep idx in registers

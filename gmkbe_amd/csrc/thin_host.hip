@@ -75,9 +75,14 @@ int run_chunk(ThinRun& T, const u32* tuples, u32 count, u32 j) {
   HIPCHK(c, hipStreamSynchronize(s));
   a.nfam = nfam;
   RunCounters* dc = (RunCounters*)c->counters.p;
-  for (u64 r0 = 0; r0 < total; r0 += RCH) {
+  // GMK_DEDUP_REPLAY: every rank range a second time (its results repeat)
+  for (u64 q = 0, nq = T.rs->replay ? 2 : 1; q < nq * ((total + RCH - 1) / RCH); ++q) {
+    const u64 r0 = (q % ((total + RCH - 1) / RCH)) * RCH;
     a.r0 = r0;
     a.r1 = std::min(total, r0 + RCH);
+    int rr;
+    if ((rr = dedup_reserve(c, *T.rs, c->h_cnt->res_count + (a.r1 - a.r0)))) return rr;
+    a.dedup = T.rs->dedup;
     // work, defer_n and dwork are consecutive
     HIPCHK(c, hipMemsetAsync(&((ThinCounters*)c->tcnt.p)->work, 0, 3 * sizeof(u32), s));
     HIPCHK(c, hipEventRecord(c->ev0, s));

@@ -303,36 +303,24 @@ __global__ __launch_bounds__(1024) void thin_scan_counts(u32* cnt, u32 n, u32* f
   }
 }
 
-// Exclusive prefix sums of the family totals (saturating), one block of 1024
-// threads: per-thread segment sums, a wave-level scan of the segment sums, then
-// a scan over the 16 wave totals.
-__global__ __launch_bounds__(1024) void thin_scan_totals(const thin::Family* fams, u32 nfam, u64* prefix) {
-  __shared__ u64 wsum[16];
-  const u32 t = threadIdx.x, lane = t & 63, wv = t >> 6;
-  const u32 seg = (nfam + 1023) / 1024;
-  const u32 b = t * seg < nfam ? t * seg : nfam;
-  const u32 e = b + seg < nfam ? b + seg : nfam;
-  auto add = [](u64 x, u64 y) -> u64 { const u64 z = x + y; return (z < x || z > thin::SAT) ? thin::SAT : z; };
-  u64 s = 0;
-  for (u32 i = b; i < e; ++i) s = add(s, fams[i].total);
-  // inclusive scan within the wave
-  u64 inc = s;
-  for (int d = 1; d < 64; d <<= 1) {
-    const u64 o = wave_bcast_u64(inc, (int)lane >= d ? (int)lane - d : (int)lane);
-    if ((int)lane >= d) inc = add(inc, o);
-  }
-  if (lane == 63) wsum[wv] = inc;
+// Exclusive prefix sums of the family totals (saturating at SAT), one block of
+// TSCAN threads: per-thread segment sums, an inclusive scan of those in LDS,
+// then each thread's segment from the sum of the segments before it (the
+// phases are thin::scan_phase_*).  A family whose prefix is finite keeps it
+// even when the running sum saturates inside its segment.
+__global__ __launch_bounds__(thin::TSCAN) void thin_scan_totals(const thin::Family* fams, u32 nfam, u64* prefix) {
+  __shared__ u64 part[thin::TSCAN];
+  const u32 t = threadIdx.x;
+  auto total = [fams](u32 i) { return fams[i].total; };
+  part[t] = thin::scan_phase_sum(nfam, t, total);
   __syncthreads();
-  if (t == 0) {
-    u64 run = 0;
-    for (int w = 0; w < 16; ++w) { const u64 v = wsum[w]; wsum[w] = run; run = add(run, v); }
+  for (u32 d = 1; d < thin::TSCAN; d <<= 1) {
+    const u64 v = thin::scan_phase_step(part, t, d);
+    __syncthreads();
+    part[t] = v;
+    __syncthreads();
   }
-  __syncthreads();
-  u64 run = add(wsum[wv], inc - s > inc ? 0 : inc - s);  // exclusive within the wave
-  if (inc >= thin::SAT) run = thin::SAT;
-  for (u32 i = b; i < e; ++i) { prefix[i] = run; run = add(run, fams[i].total); }
-  if (b < e && e == nfam) prefix[nfam] = run;
-  if (nfam == 0 && t == 0) prefix[0] = 0;
+  thin::scan_phase_write(nfam, t, part, total, prefix);
 }
 
 template <int K>
@@ -351,7 +339,10 @@ __device__ __forceinline__ void flush_check_stats(const ThinArgs& a, const thin:
 // besides its free ones (each costs budget of at least one of <= 2K members),
 // so the chosen vertices live in registers: every access uses a constant index.
 
-// Family of rank r and its chosen vertices (with their type sizes).
+// Family of rank r and its chosen vertices (with their type sizes); null when r
+// lies past the end of the family the prefix sums give (ranks at or above SAT
+// when the sums saturate are never launched, so this does not happen; the
+// check keeps a wrong prefix from decoding an out-of-range offset).
 template <int K>
 __device__ __forceinline__ const thin::Family* unrank_rank(const ThinArgs& a, u64 r, u32 (&ch)[2 * K * K],
                                                           u32 (&ctsz)[2 * K * K], u32& nch, thin::Cost& cost) {
@@ -369,6 +360,8 @@ __device__ __forceinline__ const thin::Family* unrank_rank(const ThinArgs& a, u6
   }
   const thin::Family* f = a.fams + lo;
   u64 rr = r - a.prefix[lo];
+  cost.bytes += 8ull * 8 + 16ull * f->ntypes;  // prefix probes, family header and type fields
+  if (rr >= f->total) return nullptr;  // past the family's combinations: no rank
   for (u32 t = 0; t < f->ntypes; ++t) {
     const u64 radix = thin::binom(f->psize[t], f->cnt[t]);
     const u64 rt = rr % radix;
@@ -387,7 +380,6 @@ __device__ __forceinline__ const thin::Family* unrank_rank(const ThinArgs& a, u6
       ++nch;
     }
   }
-  cost.bytes += 8ull * 8 + 16ull * f->ntypes;  // prefix probes, family header and type fields
   return f;
 }
 
@@ -430,8 +422,19 @@ __device__ __forceinline__ bool emit_wave(const ThinArgs& a, bool want, const th
     for (int i = 0; i < NM; ++i)
       if ((u32)i < nch) h += mix64(ch[i]);
     cost.bytes += 4ull * f->nfree;
-    fpv = fmix64(h + (u64)(a.swap_sides ? nr0 : f->j) * FP_A + (u64)(a.swap_sides ? f->j : nr0) * FP_B);
-    want = dedup_insert(a.dedup, fpv, a.flags, stripe);
+    const u64 nlr = a.swap_sides ? nr0 : f->j, nrr = a.swap_sides ? f->j : nr0;
+    fpv = fmix64(h + nlr * FP_A + nrr * FP_B);
+    if (a.dedup.on) {
+      // the table's check word: an independent hash of the same record
+      u64 h2 = 0;
+      for (u32 i = 0; i < f->j; ++i) h2 += mix64b(f->L[i]);
+      for (u32 i = 0; i < f->nfree; ++i) h2 += mix64b(fr0[i]);
+#pragma unroll
+      for (int i = 0; i < NM; ++i)
+        if ((u32)i < nch) h2 += mix64b(ch[i]);
+      cost.bytes += 4ull * f->nfree;
+      want = dedup_insert(a.dedup, fpv, fmix64(h2 + nlr * FP_B + nrr * FP_A), a.flags, stripe);
+    }
   }
   const u64 mask = __ballot(want);
   if (!mask) return false;
@@ -508,6 +511,7 @@ __device__ __forceinline__ bool check_rank(const ThinArgs& a, u64 r, const thin:
   if (r >= a.r1) return false;
   u32 ctsz[NM];
   f = unrank_rank<K>(a, r, ch, ctsz, nch, cost);
+  if (!f) return false;
   const int m = thin::side_a_maximal<NM>(a.g, f->L, f->j, f->freep, f->nfree, f->fpick, f->nfpick, ch, ctsz, nch,
                                          a.k, cost, a.budget);
   if (m < 0) {
@@ -541,6 +545,7 @@ __global__ __launch_bounds__(256) void thin_check_deferred(ThinArgs a) {
     u32 ch[NM], ctsz[NM];
     u32 nch = 0;
     const thin::Family* f = unrank_rank<K>(a, r, ch, ctsz, nch, cost);
+    if (!f) continue;  // the same rank in every lane
     thin::CandSegs cs;
     thin::candidate_segments<NM>(a.g, f->j, f->nfree, f->fpick, f->nfpick, ch, ctsz, nch, a.k, cs, cost);
     u64 total = 0;
@@ -666,7 +671,7 @@ hipError_t launch_thin_rank_index(const u64* prefix, u32 nfam, u64 r0, u64 r1, u
 }
 
 hipError_t launch_thin_scan(const thin::Family* fams, u32 nfam, u64* prefix, hipStream_t s) {
-  hipLaunchKernelGGL(thin_scan_totals, dim3(1), dim3(1024), 0, s, fams, nfam, prefix);
+  hipLaunchKernelGGL(thin_scan_totals, dim3(1), dim3(thin::TSCAN), 0, s, fams, nfam, prefix);
   return hipGetLastError();
 }
 

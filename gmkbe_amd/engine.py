@@ -40,7 +40,7 @@ class Stats(ctypes.Structure):
                 ("dedup_probes", ctypes.c_uint64), ("dedup_dups", ctypes.c_uint64), ("ms", ctypes.c_double),
                 ("ms_expand", ctypes.c_double), ("hit_limit", ctypes.c_int32), ("spills", ctypes.c_uint32),
                 ("ms_kernel", ctypes.c_double * 4), ("launches_kernel", ctypes.c_uint64 * 4),
-                ("bytes_kernel", ctypes.c_uint64 * 4)]
+                ("bytes_kernel", ctypes.c_uint64 * 4), ("dedup_undecided", ctypes.c_uint64)]
 
     KERNELS = ("bb_expand", "thin_build_families", "thin_check", "partition_build")
 
@@ -143,6 +143,8 @@ def load_library():
     L.gmk_destroy.restype = None
     L.gmk_last_error.argtypes = [ctypes.c_void_p]
     L.gmk_last_error.restype = ctypes.c_char_p
+    L.gmk_device_name.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_uint32]
+    L.gmk_device_name.restype = ctypes.c_int
     L.gmk_load_graph_bin.argtypes = [ctypes.c_void_p, ctypes.c_char_p, u32p, u32p, P(ctypes.c_uint64)]
     L.gmk_load_graph_bin.restype = ctypes.c_int
     L.gmk_set_graph_csr.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, u32p, u32p]
@@ -211,6 +213,11 @@ class Engine:
     def set_tuning(self, t) -> None:
         t = t if isinstance(t, Tuning) else Tuning.from_dict(t)
         self._check(self._L.gmk_set_tuning(self._h, ctypes.byref(t)), "gmk_set_tuning")
+
+    def device_name(self) -> str:
+        buf = ctypes.create_string_buffer(256)
+        self._check(self._L.gmk_device_name(self._h, buf, 256), "gmk_device_name")
+        return buf.value.decode()
 
     def get_tuning(self) -> dict:
         t = Tuning()

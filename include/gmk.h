@@ -57,10 +57,12 @@ typedef struct {
   int32_t keep_results;  /* 1: keep the record stream for gmk_fetch_results */
   int32_t verbose;       /* -m */
   uint64_t result_capacity_words; /* 0 = automatic */
-  int32_t dedup;         /* 1: insert every result's 64-bit fingerprint into a device
-                            hash table and drop repeats (a safety net: the
-                            decomposition emits no repeats) */
-  uint32_t dedup_load_permille;   /* table load factor at the expected result count, 0 = 500 */
+  int32_t dedup;         /* 1: insert every result into a device hash table (two
+                            independent 64-bit hashes per result) and drop
+                            repeats: a safety net, the decomposition emits none.
+                            Limited runs size the table for the limit, unlimited
+                            ones grow it as results arrive */
+  uint32_t dedup_load_permille;   /* table load factor, 0 = the tuning table's */
 } gmk_params;
 
 typedef struct {
@@ -81,12 +83,18 @@ typedef struct {
   double ms_kernel[4];       /* device time (HIP events around each launch) */
   uint64_t launches_kernel[4];
   uint64_t bytes_kernel[4];  /* algorithmic bytes read + written */
+  uint64_t dedup_undecided;  /* results kept because a just-claimed slot with the same
+                                fingerprint had not yet shown its check word (dedup) */
 } gmk_stats;
 
 /* Create a context on HIP device `device` (-d, src/main.cu:1765-1766). */
 int gmk_create(int device, gmk_ctx** out);
 void gmk_destroy(gmk_ctx* ctx);
 const char* gmk_last_error(gmk_ctx* ctx);
+/* Name of the context's device, NUL-terminated and cut to cap bytes (replaces
+ * cudaGetDeviceProperties for the "Using GPU: <name>, GPU id: <d>" line,
+ * src/main.cu:1768-1770). */
+int gmk_device_name(gmk_ctx* ctx, char* out, uint32_t cap);
 
 /* Load a .graph.bin file and upload its CSR (replaces load_graph_from_bin,
  * data/src/tmp_load.h:213-295, and the H2D copies at src/main.cu:1680-1690). */

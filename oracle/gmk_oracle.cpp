@@ -10,8 +10,8 @@
 //   * or_validate     - k-biplex / maximality / size / duplicate checker.
 //   * or_canon_digest - canonical sorted-set digest and the order-free fingerprint
 //                       digest also computed by the GPU path.
-//   * or_convert      - edge list -> .graph.bin CSR, restating
-//                       preprocess/graph_conver.cpp:500-817 (relabel order included).
+// (The .graph.bin converter is gmkbe_amd/convert.py, a product tool; the
+// reference converter built under oracle/_ref/ is its checker.)
 //
 // Conventions.  A graph is the .graph.bin CSR: vertices 0..nL-1 are the left side,
 // nL..N-1 the right side, neighbour lists sorted ascending.  A result is recorded as
@@ -20,7 +20,7 @@
 // Each function is pinned by tests:
 //   or_bppivot  vs the compiled reference binary (oracle/_ref/bppivot) golden
 //               vectors and vs or_brute;  or_brute vs closed forms (K_{a,b},
-//               matchings, stars);  or_convert byte-exact vs oracle/_ref/graph_convert.
+//               matchings, stars).
 #include <algorithm>
 #include <chrono>
 #include <cstdint>

@@ -168,3 +168,18 @@ extern "C" long long thin_dominated(unsigned n, unsigned n_left, const unsigned*
   }
   return c;
 }
+
+// Prefix sums of family totals by the phases of the device scan
+// (thin_scan_totals: thin::scan_phase_*), run thread by thread, each barrier of
+// the block becoming the end of a loop over the TSCAN threads.  prefix gets
+// n + 1 entries.
+extern "C" void thin_scan_totals_model(const unsigned long long* totals, unsigned n, unsigned long long* prefix) {
+  std::vector<u64> part(TSCAN), next(TSCAN);
+  auto total = [totals](u32 i) { return (u64)totals[i]; };
+  for (u32 t = 0; t < TSCAN; ++t) part[t] = scan_phase_sum(n, t, total);
+  for (u32 d = 1; d < TSCAN; d <<= 1) {
+    for (u32 t = 0; t < TSCAN; ++t) next[t] = scan_phase_step(part.data(), t, d);
+    part.swap(next);
+  }
+  for (u32 t = 0; t < TSCAN; ++t) scan_phase_write(n, t, part.data(), total, (u64*)prefix);
+}

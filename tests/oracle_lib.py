@@ -165,6 +165,20 @@ def _model_init():
         _model.thin_model_set_hash_min.argtypes = [ctypes.c_uint32]
         _model.thin_model_hash_check.restype = ctypes.c_longlong
         _model.thin_model_hash_check.argtypes = [ctypes.c_uint32, u32p, u32p, ctypes.c_uint32]
+        u64p = P(ctypes.c_uint64)
+        _model.thin_scan_totals_model.restype = None
+        _model.thin_scan_totals_model.argtypes = [u64p, ctypes.c_uint32, u64p]
+
+
+def scan_totals_model(totals):
+    """Exclusive saturating prefix sums of family totals by the device scan's
+    own phases, run on the CPU (tests/model/thin_model.cpp); n + 1 entries."""
+    _model_init()
+    t = np.ascontiguousarray(totals, dtype=np.uint64)
+    out = np.zeros(len(t) + 1, dtype=np.uint64)
+    P = ctypes.POINTER(ctypes.c_uint64)
+    _model.thin_scan_totals_model(t.ctypes.data_as(P), len(t), out.ctypes.data_as(P))
+    return out
 
 
 def hash_check(g, nh_min):

@@ -31,12 +31,30 @@ def test_engine_fails_loudly_without_gpu_or_library(monkeypatch):
         engine.Engine(0)
 
 
-def test_params_and_stats_layout_match_header():
-    # the ctypes mirrors must have the C structs' sizes (x86-64 SysV layout)
+def test_params_and_stats_layout_match_header(tmp_path):
+    """The ctypes mirrors have the C structs' sizes and field offsets, as the C
+    compiler lays out include/gmk.h."""
     import ctypes
-    assert ctypes.sizeof(engine.Params) == 48
-    assert ctypes.sizeof(engine.Tuning) == 4 + 4 * 7 + 4 * 7 + 4 + 4 * 4
-    assert ctypes.sizeof(engine.Stats) == 8 * 13 + 8 + 32 * 3
+    import subprocess
+    structs = {"gmk_params": engine.Params, "gmk_stats": engine.Stats, "gmk_tuning": engine.Tuning,
+               "gmk_graph_desc": engine.GraphDesc, "gmk_graph_stats": engine.GraphStats, "gmk_mlp": engine.Mlp}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{os.path.join(ROOT, "include", "gmk.h")}"',
+             "int main(void) {"]
+    for cname, py in structs.items():
+        lines.append(f'  printf("{cname} size %zu\\n", sizeof({cname}));')
+        for fname, _ in py._fields_:
+            lines.append(f'  printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines.append("  return 0;\n}")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {tuple(ln.split()[:2]): int(ln.split()[2]) for ln in out if ln}
+    for cname, py in structs.items():
+        assert got[(cname, "size")] == ctypes.sizeof(py), cname
+        for fname, _ in py._fields_:
+            assert got[(cname, fname)] == getattr(py, fname).offset, (cname, fname)
 
 
 def test_committed_tuning_table_is_valid():

@@ -79,3 +79,54 @@ def test_cli_tuning_table(tmp_path):
             assert GC.digest_matches(c, O.digest(_records_from_lines(p.stdout)))
         else:
             assert p.returncode != 0 and "tuning" in p.stderr
+
+
+def _records_bppivot(out: str, n_left: int):
+    """Records from BPPivot-style lines ("Biplex No.n:", "  SL: a, b, ",
+    "  SR: c, d, " with right labels local to the right side)."""
+    words, L = [], None
+    for ln in out.splitlines():
+        if ln.startswith("  SL: "):
+            L = [int(x) for x in ln[6:].split(",") if x.strip()]
+        elif ln.startswith("  SR: "):
+            R = [int(x) + n_left for x in ln[6:].split(",") if x.strip()]
+            words += [len(L), *L, len(R), *R]
+    return np.asarray(words, dtype=np.uint32)
+
+
+def _records_itraversal(out: str, n_left: int):
+    """Records from iTraversal result keys ("x&...y&...", global IDs, left
+    side first, each side ascending)."""
+    words = []
+    for ln in out.splitlines():
+        if re.fullmatch(r"(\d+&)+", ln):
+            ids = [int(x) for x in ln.split("&") if x]
+            L = [x for x in ids if x < n_left]
+            R = [x for x in ids if x >= n_left]
+            assert ids == sorted(L) + sorted(R)
+            words += [len(L), *L, len(R), *R]
+    return np.asarray(words, dtype=np.uint32)
+
+
+@pytest.mark.parametrize("fmt", ["gmkbpe", "bppivot", "itraversal"])
+def test_cli_result_formats(tmp_path, fmt):
+    """-q 1 result lines in the reference's format and in those of its two CPU
+    enumerators each give back the reference's set; the device line names the
+    GPU as the reference does."""
+    c = [c for c in GC.cases(min_k=1) if c["graph"] == "er40x50" and c["k"] == 1 and c["theta"] == 1][0]
+    src = os.path.join(GC.HERE, GC._G["graphs"]["er40x50"])
+    g = str(tmp_path / "er40x50.graph.bin")
+    shutil.copy(src, g)
+    print(f"[gpu] gmkbpe --format {fmt}", flush=True)
+    p = subprocess.run([CLI, "-f", g, "-k", "1", "-r", "0", "-q", "1", "--format", fmt], capture_output=True,
+                       text=True, timeout=120)
+    assert p.returncode == 0, p.stderr
+    assert re.search(r"^Using GPU: .+, GPU id: 0$", p.stdout, re.M)
+    n_left = GC.graph("er40x50").n_left
+    parse = {"gmkbpe": lambda o, n: _records_from_lines(o), "bppivot": _records_bppivot,
+             "itraversal": _records_itraversal}[fmt]
+    words = parse(p.stdout, n_left)
+    assert GC.digest_matches(c, O.digest(words))
+    if fmt == "bppivot":
+        nums = [int(m) for m in re.findall(r"^Biplex No\.(\d+):$", p.stdout, re.M)]
+        assert nums == list(range(1, len(nums) + 1)) and len(nums) == c["count"]

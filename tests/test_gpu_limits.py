@@ -52,3 +52,25 @@ def test_result_buffer_overflow_is_reported(partition, mode):
             e.enumerate(k=1, theta=1, mode=mode, partition=partition, capacity_words=1000)
         res = e.enumerate(k=1, theta=1, mode=mode, partition=partition)
         assert GC.digest_matches(c, O.digest(res.words))
+
+
+def test_asymmetric_csr_is_refused():
+    """An edge listed on one side only is refused (the thin checks read side A's
+    lists and the partitions side B's, so the two must agree)."""
+    import numpy as np
+    from gmkbe_amd.engine import GmkError
+    from gmkbe_amd.graphio import Graph, from_edges
+    g = from_edges(3, 3, np.array([0, 1, 2, 2]), np.array([0, 1, 1, 2]))
+    with Engine(0) as e:
+        e.set_graph(g)  # symmetric: accepted
+        adj = g.adj.copy()
+        # left vertex 2 drops right vertex 5 (local 2) from its list; 5 still lists 2
+        off = g.off.copy()
+        s, t = int(off[2]), int(off[3])
+        keep = [x for x in adj[s:t] if x != 5]
+        adj2 = np.concatenate([adj[:s], np.array(keep, dtype=np.uint32), adj[t:]])
+        off2 = off.copy()
+        off2[3:] -= 1
+        print("[gpu] asymmetric CSR", flush=True)
+        with pytest.raises(GmkError, match="invalid argument"):
+            e.set_graph(Graph(g.n_left, g.n, off2, adj2))
