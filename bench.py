@@ -1,15 +1,18 @@
 """Benchmark: maximal k-biplexes per second on one MI355X.
 
-Workload (default): Chung-Lu bipartite graph |U| = 50,000, |V| = 30,000,
-500,000 edges, Pareto degree exponent 2.1 capped at 5,000 (seeded, converted
-with the reference converter's relabel), k = 1, theta = 1, result limit
-r = 1,000,000.  One step = one enumeration to the limit through the C ABI,
-graph already resident on the device.  `value` = results per second over the
-timed steps in batched mode (full launches, persistent expansion kernel: the
-library's default); plain per-partition mode (one launch per seed, the
-reference's way of working) is timed on the same graph and reported beside it
-with the speedup.  Other workloads: --workload (C1/C4/C5 single graphs, and
-er_batch for the many-graphs batch with the learned schedule).
+Workload (default): Chung-Lu bipartite graph |U| = 2,000,000, |V| = 1,000,000,
+20,000,000 edges, Pareto degree exponent 2.0 (seeded, converted with the
+reference converter's relabel), k = 2, theta = 1, result limit r = 10,000,000.
+One step = one enumeration to the limit through the C ABI, graph already
+resident on the device.  `value` = results per second over the timed steps in
+learned mode: batched launches with the work items ordered by the item cost
+model (an FP32 MLP trained, before the timed steps, on the per-item yield and
+device time the method logs on the 50k x 30k graph and on this graph's first
+5% of items).  Plain mode (one launch sequence per work item, the reference's
+way of working) and batched mode in the heuristic (degree) order are timed on
+the same graph in the same run and reported beside it with the speedups.
+Other workloads: --workload (C1/C2/C5 single graphs, and er_batch for the
+many-graphs batch with the learned graph order).
 
 Also reported: the roofline of the dominant kernel (algorithmic bytes counted
 in-kernel / device time from HIP events on the kernel's stream) and the CPU
@@ -34,14 +37,16 @@ sys.path.insert(0, ROOT)
 METRIC = "maximal k-biplexes/sec, 1 GPU, exact set; batched+learned vs plain speedup"
 HBM_PEAK_GBS = 8000.0
 
+# train: (workload, fraction of its tuple-side items) probed to train the item
+# cost model of the learned schedule; None: no learned run (global partition)
 WORKLOADS = {
-    "cl50kx30k": dict(k=1, theta=1, limit=1_000_000, cpu_sample=4000,
+    "cl50kx30k": dict(k=1, theta=1, limit=1_000_000, cpu_sample=4000, train=[("cl50kx30k", 0.05)],
                       desc="Chung-Lu bipartite |U|=50000 |V|=30000 m=500000 alpha=2.1 cap=5000 (seeded), k=1, theta=1, r=1000000"),
-    "er200x300": dict(k=1, theta=1, limit=0, cpu_sample=4_397_076,  # the whole set: about 7 s on one core
+    "er200x300": dict(k=1, theta=1, limit=0, cpu_sample=4_397_076, train=None,  # the whole set: about 7 s on one core
                       desc="Erdos-Renyi bipartite 200x300 p=0.05 (seeded), k=1, theta=1, full enumeration"),
-    "cl2Mx1M": dict(k=2, theta=1, limit=10_000_000, cpu_sample=200,
+    "cl2Mx1M": dict(k=2, theta=1, limit=10_000_000, cpu_sample=200, train=[("cl50kx30k", 1.0), ("cl2Mx1M", 0.05)],
                     desc="Chung-Lu bipartite |U|=2000000 |V|=1000000 m=20000000 alpha=2.0 (seeded), k=2, theta=1, r=10000000"),
-    "planted500k": dict(k=3, theta=1, limit=10_000_000, cpu_sample=200,
+    "planted500k": dict(k=3, theta=1, limit=10_000_000, cpu_sample=200, train=[("cl50kx30k", 1.0), ("planted500k", 0.05)],
                         desc="Chung-Lu 500000x500000 m=5000000 alpha=2.2 + 200 planted near-bicliques (seeded), k=3, theta=1, r=10000000"),
     "er_batch": dict(k=0, theta=0, limit=1000, cpu_sample=0,
                      desc="2000 ER bipartite graphs, sides log-uniform in [20,2000], p log-uniform in [0.01,0.2], "
@@ -190,6 +195,30 @@ def cpu_baseline(g, k, theta, sample):
                 "sample": f"first {dg.count} results of the BPPivot restatement on the same graph ({dt:.1f} s)"}
 
 
+def train_item_model(eng, cfg, workload, g):
+    """Item cost model of the learned schedule: probes (gmk_probe_items, the
+    first 1024 combinations of every family of each item) on the training
+    workloads, then the library's own HIP training kernels.  Runs before the
+    timed steps."""
+    from gmkbe_amd import gen, schedule
+    samples, info = [], []
+    t0 = time.perf_counter()
+    for name, frac in cfg["train"]:
+        gt = g if name == workload else gen.workload(name)
+        k = WORKLOADS[name]["k"] if name != workload else cfg["k"]
+        eng.set_graph(gt)
+        n = max(1, int(gt.n_left * frac))
+        X, y, inf = schedule.probe_samples(eng, k, 0, n)
+        samples.append((X, y))
+        info.append(dict(inf, workload=name, k=k, fraction=frac, samples=int(len(y))))
+        log(f"[bench] probed {n} items of {name} (k={k}): {inf['working']} with work, {len(y)} samples")
+    model, loss = schedule.train_item_model(eng, samples)
+    dt = time.perf_counter() - t0
+    log(f"[bench] item model trained: loss {loss:.3f} ({dt:.1f} s with the probes)")
+    return model, {"train": info, "train_loss": loss, "seconds": dt, "features": "gmk_item_features",
+                   "target": "log((results + 1) / (lane ms + 10)) of a 1024-rank probe per family"}
+
+
 def run_steps(eng, cfg, mode, partition, steps, warmup, sync):
     """Timed steps write every result record to device memory; the records are
     not copied to the host inside the timed region."""
@@ -220,12 +249,13 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="cl50kx30k", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="cl2Mx1M", choices=sorted(WORKLOADS))
     ap.add_argument("--partition", default="seeded", choices=["auto", "global", "seeded"])
     ap.add_argument("--tune", default=os.path.join(ROOT, "gmkbe_amd", "tune", "mi355x.json"), help="tuning table (written by python -m gmkbe_amd.autotune)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-mode run")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-mode run (profiling the batched kernels)")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-schedule run")
     ap.add_argument("--batch-sample", type=int, default=100, help="er_batch: graphs in the speedup comparison")
     ap.add_argument("--train-graphs", type=int, default=200, help="er_batch: graphs that train the cost model")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -264,29 +294,37 @@ def main() -> int:
 
     eng = Engine(0, tuning=args.tune)
     eng.set_graph(g)
-    # plain: per-partition (per-seed) launches, the reference's way of working;
-    # batched (the default mode): full launches, persistent expansion kernel
-    plain, vp = None, None
-    mode, dt, last = None, None, None
-    if not args.no_plain or args.no_batched:
-        log("[bench] plain mode")
-        dtp, lastp = run_steps(eng, cfg, "plain", args.partition, args.steps, args.warmup, sync)
-        vp = args.steps * lastp.stats["count"] / dtp
-        plain = {"value": vp, "ms_per_step": dtp / args.steps * 1e3, "count": lastp.stats["count"]}
-        log(f"[bench] plain: {lastp.stats['count']} results/step, {dtp / args.steps * 1e3:.2f} ms/step, "
-            f"{vp:.3e} biplexes/s")
-        mode, dt, last = "plain", dtp, lastp
-    if not args.no_batched:
-        log("[bench] batched mode")
-        dt, last = run_steps(eng, cfg, "batched", args.partition, args.steps, args.warmup, sync)
-        mode = "batched"
+    runs = {}
+    for mode in ("plain", "batched"):
+        if (mode == "plain" and args.no_plain) or (mode == "batched" and args.no_batched):
+            continue
+        log(f"[bench] {mode} mode")
+        dt, last = run_steps(eng, cfg, mode, args.partition, args.steps, args.warmup, sync)
+        runs[mode] = (dt, last)
+        log(f"[bench] {mode}: {last.stats['count']} results/step, {dt / args.steps * 1e3:.2f} ms/step, "
+            f"{args.steps * last.stats['count'] / dt:.3e} biplexes/s")
+    cost_model = None
+    if cfg["train"] and args.partition != "global" and not args.no_learned:
+        model, cost_model = train_item_model(eng, cfg, args.workload, g)
+        eng.set_graph(g)
+        eng.set_item_model(model)
+        log("[bench] learned mode")
+        dt, last = run_steps(eng, cfg, "learned", args.partition, args.steps, args.warmup, sync)
+        runs["learned"] = (dt, last)
+        log(f"[bench] learned: {last.stats['count']} results/step, {dt / args.steps * 1e3:.2f} ms/step, "
+            f"{args.steps * last.stats['count'] / dt:.3e} biplexes/s, schedule {last.stats['ms_schedule']:.3f} ms")
+    mode = "learned" if "learned" in runs else "batched" if "batched" in runs else "plain"
+    dt, last = runs[mode]
     st = last.stats
     count = st["count"]
     value = args.steps * count / dt
-    if plain:
-        plain["speedup_of_batched"] = value / vp
-    log(f"[bench] {mode}: {count} results/step, {dt / args.steps * 1e3:.2f} ms/step, {value:.3e} biplexes/s"
-        + (f" ({value / vp:.2f}x plain)" if plain else ""))
+    rate = {m: args.steps * r[1].stats["count"] / r[0] for m, r in runs.items()}
+    others = {m: {"value": rate[m], "ms_per_step": r[0] / args.steps * 1e3, "count": r[1].stats["count"],
+                  "ranks_checked": r[1].stats["intersections"]} for m, r in runs.items() if m != mode}
+    speedups = {f"{a}_vs_{b}": rate[a] / rate[b] for a, b in (("learned", "plain"), ("learned", "batched"),
+                                                              ("batched", "plain")) if a in rate and b in rate}
+    for name, v in speedups.items():
+        log(f"[bench] speedup {name}: {v:.2f}x")
     pcie = with_fetch(eng, cfg, mode, args.partition)
     log(f"[bench] {mode} with the records copied to the host: {pcie['value']:.3e} biplexes/s")
     eng.close()
@@ -299,8 +337,18 @@ def main() -> int:
     bytes_per_launch = st["bytes_kernel"][i] / launches
     ms_per_launch = ms_k[i] / launches
     achieved = bytes_per_launch / (ms_per_launch * 1e-3) / 1e9 if ms_per_launch > 0 else 0.0
+    traffic = traffic_from_profiles(names[i], args.workload)
+    # The roof is HBM bandwidth (integer gathers, no MFMA work).  `achieved`
+    # counts algorithmic bytes: every array element the kernel reads or writes
+    # (list entries, offsets, search and hash probes, family and result
+    # records), tallied in-kernel; most of them hit L2, so the HBM traffic
+    # measured by rocprofv3 (`traffic`) is far lower.
     roofline = {"bound": "hbm", "kernel": names[i], "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profiles(names[i], args.workload),
+                "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                "basis": "algorithmic bytes counted in-kernel / HIP-event device time per launch",
+                "hbm_frac_measured": (traffic / (ms_per_launch * 1e-3) / 1e9 / HBM_PEAK_GBS
+                                      if traffic and ms_per_launch > 0 else None),
+                "limiter": "latency of dependent per-lane gathers (L2-resident), per the rocprofv3 counters",
                 "launches": launches, "algo_bytes_per_launch": bytes_per_launch, "ms_per_launch": ms_per_launch}
 
     cpu = None
@@ -325,11 +373,16 @@ def main() -> int:
                    "limit": cfg["limit"], "mode": mode, "partition": args.partition,
                    "graph": {"n_left": g.n_left, "n_right": g.n_right, "edges": g.n_edges}},
         "count_per_step": count,
+        "ranks_checked_per_step": st["intersections"],
+        "ms_schedule_per_step": st["ms_schedule"],
+        "speedup": speedups,
+        "target_speedup_learned_vs_plain": 1.5,
+        "other_modes": others,
+        "cost_model": cost_model,
         "hit_limit": bool(st["hit_limit"]),
         "digest": {"fp_sum": str(st["fp_sum"]), "fp_xor": str(st["fp_xor"])},
         "kernels": {names[q]: {"ms": ms_k[q], "launches": st["launches_kernel"][q], "algo_bytes": st["bytes_kernel"][q]}
                     for q in range(4)},
-        "plain": plain,
         "with_records_to_host": pcie,
         "roofline": roofline,
         "cpu_baseline": cpu,

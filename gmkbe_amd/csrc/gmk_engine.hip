@@ -70,7 +70,8 @@ extern "C" void gmk_destroy(gmk_ctx* c) {
   (void)hipStreamSynchronize(c->stream);
   for (DeviceVec* v : {&c->rows, &c->maps, &c->parts, &c->res, &c->counters, &c->tuples, &c->arena, &c->fams,
                         &c->prefix, &c->tcnt, &c->tdefer, &c->ridx, &c->tcount, &c->nh_off, &c->nh, &c->nh_verts, &c->dedup, &c->gcnt,
-                        &c->bcsr_off, &c->bcsr_adj, &c->bcsr_base, &c->rowpids})
+                        &c->bcsr_off, &c->bcsr_adj, &c->bcsr_base, &c->rowpids, &c->sched_p, &c->sched_hist,
+                        &c->sched_sel})
     if (v->p) (void)hipFree(v->p);
   for (int l = 0; l < 7; ++l)
     for (DeviceVec* v : {&c->lane_stack[l], &c->lane_next[l], &c->lane_retry[l], &c->lane_spill[l]})
@@ -92,7 +93,9 @@ extern "C" int gmk_device_name(gmk_ctx* c, char* out, uint32_t cap) {
   if (!c || !out || cap == 0) return GMK_EINVAL;
   hipDeviceProp_t prop;
   HIPCHK(c, hipGetDeviceProperties(&prop, c->device));
-  snprintf(out, cap, "%s", prop.name);
+  // the marketing name can be empty where the driver lacks its ID table; the
+  // architecture name is always set
+  snprintf(out, cap, "%s", prop.name[0] ? prop.name : prop.gcnArchName);
   return GMK_OK;
 }
 
@@ -388,6 +391,7 @@ extern "C" int gmk_enumerate(gmk_ctx* c, const gmk_params* prm, gmk_stats* st) {
   st->launches = rs.launches;
   st->spills = rs.spills;
   st->ms_expand = rs.ms_expand;
+  st->ms_schedule = rs.ms_schedule;
   st->hit_limit = rs.hit_limit || h.res_count >= rs.limit;
   st->ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   rs.progress(st->count);

@@ -23,6 +23,7 @@ struct RunState {
   DedupTable dedup{};
   double dedup_load = 0.5;  // load factor the dedup table is kept under
   bool replay = false;      // produce every result twice (dedup tests)
+  double ms_schedule = 0;   // host time spent ordering items (learned runs)
   unsigned long long* gcount = nullptr;  // per-graph caps (multi-graph batches)
   u64* gfp = nullptr;
   u64 gcap = ~0ull;
@@ -120,8 +121,11 @@ struct gmk_ctx {
   DeviceVec bcsr_off, bcsr_adj, bcsr_base, rowpids;  // CSRs of a batch's graphs; partitions whose rows are built
   std::vector<u64> prog_count;   // progress log of the last run
   std::vector<double> prog_ms;
-  bool has_model = false;        // cost model for GMK_MODE_LEARNED
+  bool has_model = false;        // cost model of GMK_MODE_LEARNED batches (per graph)
   gmk_mlp model{};
+  bool has_item_model = false;   // cost model of GMK_MODE_LEARNED single-graph runs (per work item)
+  gmk_mlp item_model{};
+  DeviceVec sched_p, sched_hist, sched_sel;  // learned schedule: model parameters, score histogram, selection
 };
 
 inline int ensure(gmk_ctx* c, DeviceVec& v, size_t bytes) {
@@ -140,6 +144,27 @@ inline int ensure(gmk_ctx* c, DeviceVec& v, size_t bytes) {
 
 
 int gmk_pick_w(uint32_t words);
+namespace gmk {
+// Flat parameter vector of the cost model: w1 [in][hid], b1, w2 [hid][hid],
+// b2, w3, b3 (the order of gmk_mlp without mu and sd).
+constexpr int MLP_NPARAM = GMK_MLP_IN * GMK_MLP_HID + GMK_MLP_HID + GMK_MLP_HID * GMK_MLP_HID + GMK_MLP_HID +
+                           GMK_MLP_HID + 1;
+void mlp_pack(const gmk_mlp& m, float* P);
+// Learned schedule of single-graph runs: about the 2 `want` one-vertex work
+// items of side A ([a_lo, a_hi)) with the highest predicted results per unit
+// of device time under the item cost model, unsorted, as (score key << 32 |
+// ~(item - a_lo)) entries (a sampled threshold decides which are gathered).
+int learned_front(gmk_ctx* c, u32 a_lo, u32 a_hi, u32 k, u32 want, std::vector<u64>& front);
+// Hands out gathered items best first (ties: lower ID first), sorting only as
+// far as they are taken.
+struct LearnedOrder {
+  static constexpr size_t STEP = 256;
+  std::vector<u64> cand;
+  size_t pos = 0, sorted = 0;
+  bool done() const { return pos >= cand.size(); }
+  u32 next(u32 a_lo);
+};
+}  // namespace gmk
 int gmk_predict_costs(gmk_ctx* c, const gmk_graph_desc* graphs, uint32_t n, double* cost);
 namespace gmk {
 // Seed partitions of one graph, consumed in degeneracy order.

@@ -14,7 +14,7 @@ using namespace gmk;
 namespace {
 
 constexpr int IN = GMK_MLP_IN, H = GMK_MLP_HID;
-constexpr int NPARAM = IN * H + H + H * H + H + H + 1;  // w1 b1 w2 b2 w3 b3
+constexpr int NPARAM = gmk::MLP_NPARAM;  // w1 b1 w2 b2 w3 b3
 // parameter offsets inside the flat vector (same order as gmk_mlp)
 constexpr int OW1 = 0, OB1 = OW1 + IN * H, OW2 = OB1 + H, OB2 = OW2 + H * H, OW3 = OB2 + H, OB3 = OW3 + H;
 
@@ -71,38 +71,52 @@ __global__ __launch_bounds__(256) void mlp_backward(const float* __restrict__ P,
   }
 }
 
-// Gradient of every parameter: one thread per parameter, sum over samples.
-__global__ __launch_bounds__(256) void mlp_wgrad(const float* __restrict__ X, uint32_t n, const float* __restrict__ Z1,
-                                                 const float* __restrict__ Z2, const float* __restrict__ DZ1,
-                                                 const float* __restrict__ DZ2, const float* __restrict__ DY,
-                                                 float* __restrict__ G) {
+// Gradient of every parameter: one thread per (parameter, slice of the
+// samples) writes that slice's sum; mlp_wgrad_reduce adds the WG_SLICES
+// partial sums in a fixed order (deterministic).
+constexpr int WG_SLICES = 64;
+__global__ __launch_bounds__(256) void mlp_wgrad(const float* __restrict__ X, uint32_t n_all,
+                                                 const float* __restrict__ Z1, const float* __restrict__ Z2,
+                                                 const float* __restrict__ DZ1, const float* __restrict__ DZ2,
+                                                 const float* __restrict__ DY, float* __restrict__ Gp) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= NPARAM) return;
+  const uint32_t s0 = (uint32_t)(((uint64_t)n_all * blockIdx.y) / WG_SLICES);
+  const uint32_t n = (uint32_t)(((uint64_t)n_all * (blockIdx.y + 1)) / WG_SLICES);
+  float* G = Gp + (size_t)blockIdx.y * NPARAM;
   float acc = 0.f;
   if (p < OB1) {  // w1[i][j] = sum x_i dz1_j
     const int i = p / H, j = p % H;
-    for (uint32_t s = 0; s < n; ++s) acc = fmaf(X[(size_t)s * IN + i], DZ1[(size_t)s * H + j], acc);
+    for (uint32_t s = s0; s < n; ++s) acc = fmaf(X[(size_t)s * IN + i], DZ1[(size_t)s * H + j], acc);
   } else if (p < OW2) {  // b1
     const int j = p - OB1;
-    for (uint32_t s = 0; s < n; ++s) acc += DZ1[(size_t)s * H + j];
+    for (uint32_t s = s0; s < n; ++s) acc += DZ1[(size_t)s * H + j];
   } else if (p < OB2) {  // w2[i][j] = sum relu(z1_i) dz2_j
     const int q = p - OW2, i = q / H, j = q % H;
-    for (uint32_t s = 0; s < n; ++s) {
+    for (uint32_t s = s0; s < n; ++s) {
       const float a = Z1[(size_t)s * H + i];
       if (a > 0.f) acc = fmaf(a, DZ2[(size_t)s * H + j], acc);
     }
   } else if (p < OW3) {  // b2
     const int j = p - OB2;
-    for (uint32_t s = 0; s < n; ++s) acc += DZ2[(size_t)s * H + j];
+    for (uint32_t s = s0; s < n; ++s) acc += DZ2[(size_t)s * H + j];
   } else if (p < OB3) {  // w3[j] = sum relu(z2_j) dy
     const int j = p - OW3;
-    for (uint32_t s = 0; s < n; ++s) {
+    for (uint32_t s = s0; s < n; ++s) {
       const float a = Z2[(size_t)s * H + j];
       if (a > 0.f) acc = fmaf(a, DY[s], acc);
     }
   } else {  // b3
-    for (uint32_t s = 0; s < n; ++s) acc += DY[s];
+    for (uint32_t s = s0; s < n; ++s) acc += DY[s];
   }
+  G[p] = acc;
+}
+
+__global__ __launch_bounds__(256) void mlp_wgrad_reduce(const float* __restrict__ Gp, float* __restrict__ G) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= NPARAM) return;
+  float acc = 0.f;
+  for (int q = 0; q < WG_SLICES; ++q) acc += Gp[(size_t)q * NPARAM + p];
   G[p] = acc;
 }
 
@@ -119,7 +133,9 @@ __global__ __launch_bounds__(256) void mlp_adam(float* __restrict__ P, const flo
   P[p] -= lr * mh / (sqrtf(vh) + 1e-8f);
 }
 
-void pack(const gmk_mlp& m, float* P) {
+}  // namespace
+
+void gmk::mlp_pack(const gmk_mlp& m, float* P) {
   memcpy(P + OW1, m.w1, sizeof m.w1);
   memcpy(P + OB1, m.b1, sizeof m.b1);
   memcpy(P + OW2, m.w2, sizeof m.w2);
@@ -127,6 +143,9 @@ void pack(const gmk_mlp& m, float* P) {
   memcpy(P + OW3, m.w3, sizeof m.w3);
   P[OB3] = m.b3;
 }
+
+namespace {
+void pack(const gmk_mlp& m, float* P) { gmk::mlp_pack(m, P); }
 void unpack(const float* P, gmk_mlp& m) {
   memcpy(m.w1, P + OW1, sizeof m.w1);
   memcpy(m.b1, P + OB1, sizeof m.b1);
@@ -138,9 +157,9 @@ void unpack(const float* P, gmk_mlp& m) {
 
 struct MlpBuffers {
   float *P = nullptr, *G = nullptr, *M = nullptr, *V = nullptr, *X = nullptr, *T = nullptr, *Y = nullptr;
-  float *Z1 = nullptr, *Z2 = nullptr, *DZ1 = nullptr, *DZ2 = nullptr, *DY = nullptr;
+  float *Z1 = nullptr, *Z2 = nullptr, *DZ1 = nullptr, *DZ2 = nullptr, *DY = nullptr, *GP = nullptr;
   ~MlpBuffers() {
-    for (float* p : {P, G, M, V, X, T, Y, Z1, Z2, DZ1, DZ2, DY})
+    for (float* p : {P, G, M, V, X, T, Y, Z1, Z2, DZ1, DZ2, DY, GP})
       if (p) (void)hipFree(p);
   }
 };
@@ -159,6 +178,7 @@ int alloc_mlp(gmk_ctx* c, MlpBuffers& b, uint32_t n) {
   HIPCHK(c, al(&b.DZ1, (size_t)n * H));
   HIPCHK(c, al(&b.DZ2, (size_t)n * H));
   HIPCHK(c, al(&b.DY, n));
+  HIPCHK(c, al(&b.GP, (size_t)WG_SLICES * NPARAM));
   return GMK_OK;
 }
 
@@ -176,7 +196,9 @@ int run_fb(gmk_ctx* c, MlpBuffers& b, uint32_t n, bool grad) {
   if (grad) {
     hipLaunchKernelGGL(mlp_backward, gs, blk, 0, c->stream, b.P, b.T, n, b.Z1, b.Z2, b.Y, b.DZ1, b.DZ2, b.DY);
     HIPCHK(c, hipGetLastError());
-    hipLaunchKernelGGL(mlp_wgrad, gp, blk, 0, c->stream, b.X, n, b.Z1, b.Z2, b.DZ1, b.DZ2, b.DY, b.G);
+    hipLaunchKernelGGL(mlp_wgrad, dim3(gp.x, WG_SLICES), blk, 0, c->stream, b.X, n, b.Z1, b.Z2, b.DZ1, b.DZ2, b.DY,
+                       b.GP);
+    hipLaunchKernelGGL(mlp_wgrad_reduce, gp, blk, 0, c->stream, b.GP, b.G);
     HIPCHK(c, hipGetLastError());
   }
   return GMK_OK;

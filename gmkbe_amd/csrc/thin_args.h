@@ -32,6 +32,7 @@ struct ThinArgs {
   const u32* ridx;      // per 2^RIDX_LOG ranks of the launch's range: the family of its first rank
   u32 nfam;
   u64 r0, r1;           // rank range of this launch
+  u64 rank_base;        // offset of the launch's ranks inside every family (rank slices)
   int keep;
   int swap_sides;       // 1 when side A is the right side (records list L first)
   u32* res;
@@ -42,6 +43,9 @@ struct ThinArgs {
   u64* fp;
   u64* stat;
   DedupTable dedup;
+  // Item log (probe runs of the learned schedule, null otherwise): per tuple of
+  // the launch, results emitted and device lane time (wall-clock ticks).
+  u64* item_log;
 };
 
 hipError_t launch_thin_build(const ThinArgs& a, hipStream_t s);
@@ -58,7 +62,11 @@ hipError_t launch_build_neighbour_hash(const u32* off, const u32* adj, const u32
 // Launches of fewer one-vertex tuples than this build them a wave per tuple (a
 // lane per tuple would not fill the device); ThinArgs.wave1_max holds it.
 constexpr u32 WAVE1_MAX_TUPLES = 16384;
-hipError_t launch_thin_scan(const thin::Family* fams, u32 nfam, u64* prefix, hipStream_t s);
+// Prefix sums of the family totals over the rank slice [base, base + cap) of
+// every family: min(max(total - base, 0), cap) (probe runs check a family's
+// first ranks, learned runs take the families slice by slice; base 0, cap ~0:
+// whole families).
+hipError_t launch_thin_scan(const thin::Family* fams, u32 nfam, u64* prefix, u64 base, u64 cap, hipStream_t s);
 // Rank index of a check launch: ridx[b] = family of rank min(r0 + b 2^RIDX_LOG,
 // r1 - 1), b = 0 .. ceil((r1 - r0) / 2^RIDX_LOG).
 constexpr u32 RIDX_LOG = 8;

@@ -4,6 +4,7 @@
 // by rank range (each range: rank index, first pass, deferred pass), until the
 // result limit is reached.  Also builds the graph's neighbour hashes at upload.
 #include <algorithm>
+#include <chrono>
 #include <cstddef>
 #include <vector>
 
@@ -26,6 +27,8 @@ constexpr u64 ARENA = 1ull << 27;      // arena words (512 MiB)
 constexpr u32 FCAP = 1u << 18;         // families per launch
 constexpr u64 RCH = 1ull << 22;        // ranks per check launch
 constexpr u32 DEFER_BUDGET = 64;       // candidates one lane tests before a rank goes to a whole wave
+constexpr u32 LEARNED_FRONT = 4096;    // about twice this many one-vertex items are ordered by the learned schedule (the rest follow in ID order)
+constexpr u64 LEARNED_SLICE = 1u << 16; // ranks of each family per slice in learned runs
 
 struct ThinRun {
   gmk_ctx* c;
@@ -34,7 +37,53 @@ struct ThinRun {
   ThinArgs a{};
   std::vector<u64> totals, prefix;
   bool stop = false;
+  // probe runs (gmk_probe_items): ranks checked per family, and per item
+  // (vertex - probe_first) the results and lane ticks
+  u64 probe_cap = ~0ull;
+  u64 slice = 0;  // learned runs: ranks of each family per slice (0: whole families)
+  u32 probe_first = 0;
+  std::vector<u64>* probe_res = nullptr;
+  std::vector<u64>* probe_ticks = nullptr;
 };
+
+// Checks ranks [0, total) of the current family table (its prefix sums are
+// on the device), RCH ranks per launch: rank index, first pass, deferred pass.
+int check_ranks(ThinRun& T, u64 total) {
+  gmk_ctx* c = T.c;
+  hipStream_t s = c->stream;
+  ThinArgs& a = T.a;
+  RunCounters* dc = (RunCounters*)c->counters.p;
+  const u64 nr = (total + RCH - 1) / RCH;
+  // GMK_DEDUP_REPLAY: every rank range a second time (its results repeat)
+  for (u64 q = 0, nq = T.rs->replay ? 2 : 1; q < nq * nr; ++q) {
+    a.r0 = (q % nr) * RCH;
+    a.r1 = std::min(total, a.r0 + RCH);
+    int rr;
+    if ((rr = dedup_reserve(c, *T.rs, c->h_cnt->res_count + (a.r1 - a.r0)))) return rr;
+    a.dedup = T.rs->dedup;
+    // work, defer_n and dwork are consecutive
+    HIPCHK(c, hipMemsetAsync(&((ThinCounters*)c->tcnt.p)->work, 0, 3 * sizeof(u32), s));
+    HIPCHK(c, hipEventRecord(c->ev0, s));
+    HIPCHK(c, launch_thin_rank_index((const u64*)c->prefix.p, a.nfam, a.r0, a.r1, (u32*)c->ridx.p, s));
+    HIPCHK(c, launch_thin_check(a, s, c->n_cu * 8, c->tune.check_form[a.k < 4 ? a.k : 3]));
+    HIPCHK(c, launch_thin_check_deferred(a, s, c->n_cu * 4));
+    HIPCHK(c, hipEventRecord(c->ev1, s));
+    HIPCHK(c, hipMemcpyAsync(c->h_cnt, dc, offsetof(RunCounters, fp), hipMemcpyDeviceToHost, s));
+    HIPCHK(c, hipStreamSynchronize(s));
+    float ms = 0;
+    HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    T.rs->ms_expand += ms;
+    T.rs->ms_kernel[2] += ms;
+    T.rs->launches_kernel[2]++;
+    T.rs->launches++;
+    T.rs->progress(std::min<u64>(c->h_cnt->res_count, T.rs->limit));
+    T.rs->report(a.swap_sides ? "thin (tuples on the right)" : "thin (tuples on the left)",
+                 std::min<u64>(c->h_cnt->res_count, T.rs->limit), 0);
+    if (c->h_cnt->flags & FLAG_RES_OVERFLOW) { c->err = "result buffer overflow (raise result_capacity_words)"; return GMK_EOVERFLOW; }
+    if (c->h_cnt->flags & FLAG_LIMIT) { T.stop = true; T.rs->hit_limit = true; return GMK_OK; }
+  }
+  return GMK_OK;
+}
 
 int run_chunk(ThinRun& T, const u32* tuples, u32 count, u32 j) {
   gmk_ctx* c = T.c;
@@ -69,52 +118,40 @@ int run_chunk(ThinRun& T, const u32* tuples, u32 count, u32 j) {
   }
   const u32 nfam = tc.fam_n;
   if (nfam == 0) return GMK_OK;
-  HIPCHK(c, launch_thin_scan((const Family*)c->fams.p, nfam, (u64*)c->prefix.p, s));
-  u64 total = 0;
-  HIPCHK(c, hipMemcpyAsync(&total, (u64*)c->prefix.p + nfam, 8, hipMemcpyDeviceToHost, s));
-  HIPCHK(c, hipStreamSynchronize(s));
   a.nfam = nfam;
-  RunCounters* dc = (RunCounters*)c->counters.p;
-  // GMK_DEDUP_REPLAY: every rank range a second time (its results repeat)
-  for (u64 q = 0, nq = T.rs->replay ? 2 : 1; q < nq * ((total + RCH - 1) / RCH); ++q) {
-    const u64 r0 = (q % ((total + RCH - 1) / RCH)) * RCH;
-    a.r0 = r0;
-    a.r1 = std::min(total, r0 + RCH);
-    int rr;
-    if ((rr = dedup_reserve(c, *T.rs, c->h_cnt->res_count + (a.r1 - a.r0)))) return rr;
-    a.dedup = T.rs->dedup;
-    // work, defer_n and dwork are consecutive
-    HIPCHK(c, hipMemsetAsync(&((ThinCounters*)c->tcnt.p)->work, 0, 3 * sizeof(u32), s));
-    HIPCHK(c, hipEventRecord(c->ev0, s));
-    HIPCHK(c, launch_thin_rank_index((const u64*)c->prefix.p, nfam, a.r0, a.r1, (u32*)c->ridx.p, s));
-    HIPCHK(c, launch_thin_check(a, s, c->n_cu * 8, c->tune.check_form[a.k < 4 ? a.k : 3]));
-    HIPCHK(c, launch_thin_check_deferred(a, s, c->n_cu * 4));
-    HIPCHK(c, hipEventRecord(c->ev1, s));
-    HIPCHK(c, hipMemcpyAsync(c->h_cnt, dc, offsetof(RunCounters, fp), hipMemcpyDeviceToHost, s));
+  // Rank slices: learned runs take every family's ranks [base, base + slice)
+  // in turn, the families in schedule order within each slice; other runs
+  // take each family whole (probes: its first probe_cap ranks) in one slice.
+  for (u64 base = 0;; base += T.slice) {
+    HIPCHK(c, launch_thin_scan((const Family*)c->fams.p, nfam, (u64*)c->prefix.p, base,
+                               T.slice ? T.slice : T.probe_cap, s));
+    u64 total = 0;
+    HIPCHK(c, hipMemcpyAsync(&total, (u64*)c->prefix.p + nfam, 8, hipMemcpyDeviceToHost, s));
     HIPCHK(c, hipStreamSynchronize(s));
-    float ms = 0;
-    HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
-    T.rs->ms_expand += ms;
-    T.rs->ms_kernel[2] += ms;
-    T.rs->launches_kernel[2]++;
-    T.rs->launches++;
-    T.rs->progress(std::min<u64>(c->h_cnt->res_count, T.rs->limit));
-    T.rs->report(a.swap_sides ? "thin (tuples on the right)" : "thin (tuples on the left)",
-                 std::min<u64>(c->h_cnt->res_count, T.rs->limit), 0);
-    if (c->h_cnt->flags & FLAG_RES_OVERFLOW) { c->err = "result buffer overflow (raise result_capacity_words)"; return GMK_EOVERFLOW; }
-    if (c->h_cnt->flags & FLAG_LIMIT) { T.stop = true; T.rs->hit_limit = true; return GMK_OK; }
+    if (total == 0) break;
+    a.rank_base = base;
+    const int r = check_ranks(T, total);
+    if (r || T.stop) return r;
+    if (!T.slice) break;
+  }
+  if (a.item_log) {
+    // probe: this launch's per-tuple log, by item
+    std::vector<u64> lg(2ull * count);
+    HIPCHK(c, hipMemcpyAsync(lg.data(), a.item_log, lg.size() * 8, hipMemcpyDeviceToHost, s));
+    HIPCHK(c, hipMemsetAsync(a.item_log, 0, lg.size() * 8, s));
+    HIPCHK(c, hipStreamSynchronize(s));
+    for (u32 t = 0; t < count; ++t) {
+      const u32 it = tuples[(size_t)t * j] - T.probe_first;
+      (*T.probe_res)[it] += lg[2 * t];
+      (*T.probe_ticks)[it] += lg[2 * t + 1];
+    }
   }
   return GMK_OK;
 }
 
-}  // namespace
-
-int gmk_run_thin(gmk_ctx* c, const gmk_params* prm, gmk_stats* st, RunState& rs, int side_a, uint32_t min_b) {
-  (void)st;
+// Kernel arguments and buffers of a thin run with the tuple on side_a.
+int thin_setup(gmk_ctx* c, const gmk_params* prm, RunState& rs, int side_a, uint32_t min_b, ThinRun& T) {
   const u32 k = prm->k;
-  const u32 jmin = std::max<u32>(prm->theta, 1);
-  const u32 jmax = std::min<u32>(2 * k, thin::MAXJ);
-  if (jmin > jmax) return GMK_OK;
   int r;
   if ((r = ensure(c, c->tuples, (size_t)TCH * thin::MAXJ * 4))) return r;
   if ((r = ensure(c, c->arena, ARENA * 4))) return r;
@@ -124,7 +161,6 @@ int gmk_run_thin(gmk_ctx* c, const gmk_params* prm, gmk_stats* st, RunState& rs,
   if ((r = ensure(c, c->tdefer, RCH * 8))) return r;
   if ((r = ensure(c, c->ridx, ((RCH >> RIDX_LOG) + 2) * 4))) return r;
   if ((r = ensure(c, c->tcount, (size_t)TCH * 8 * 4))) return r;
-  ThinRun T;
   T.c = c; T.prm = prm; T.rs = &rs;
   ThinArgs& a = T.a;
   a.g.off = c->d_off;
@@ -173,44 +209,102 @@ int gmk_run_thin(gmk_ctx* c, const gmk_params* prm, gmk_stats* st, RunState& rs,
   a.fp = dc->fp;
   a.stat = dc->stat;
   a.dedup = rs.dedup;
+  a.item_log = nullptr;
+  return GMK_OK;
+}
 
-  const u32 a_lo = a.g.a_lo, a_hi = a.g.a_hi;
-  const u32* off = c->off.data();
+// Streams tuples into launches: `count` tuples of j vertices built by
+// `next(idx)` (false at the end), launch sizes ramping from 64 tuples (x4 per
+// launch) to TCH so a run that reaches its limit within the first tuples stops
+// early; `per_seed` flushes after every first vertex.
+struct TupleFeed {
+  ThinRun& T;
+  u32 j;
+  u32& chunk;
+  std::vector<u32> buf;
+  u32 count = 0;
+  u64 est = 0;
+  TupleFeed(ThinRun& t, u32 jj, u32& ch) : T(t), j(jj), chunk(ch) { buf.reserve((size_t)TCH * thin::MAXJ); }
+  int flush() {
+    if (count == 0) return GMK_OK;
+    const int rr = run_chunk(T, buf.data(), count, j);
+    buf.clear(); count = 0; est = 0;
+    chunk = std::min<u32>(chunk * 4, TCH);
+    return rr;
+  }
+  int push(const u32* idx) {
+    const u32* off = T.c->off.data();
+    u64 need = 0;
+    for (u32 i = 0; i < j; ++i) need += 2ull * (off[idx[i] + 1] - off[idx[i]]);
+    if (count >= chunk || est + need > ARENA) {
+      const int rr = flush();
+      if (rr || T.stop) return rr;
+    }
+    buf.insert(buf.end(), idx, idx + j);
+    ++count;
+    est += need;
+    return GMK_OK;
+  }
+};
+
+}  // namespace
+
+int gmk_run_thin(gmk_ctx* c, const gmk_params* prm, gmk_stats* st, RunState& rs, int side_a, uint32_t min_b) {
+  (void)st;
+  const u32 k = prm->k;
+  const u32 jmin = std::max<u32>(prm->theta, 1);
+  const u32 jmax = std::min<u32>(2 * k, thin::MAXJ);
+  if (jmin > jmax) return GMK_OK;
+  ThinRun T;
+  int r;
+  if ((r = thin_setup(c, prm, rs, side_a, min_b, T))) return r;
+  const u32 a_lo = T.a.g.a_lo, a_hi = T.a.g.a_hi;
+  const bool per_seed = prm->mode == GMK_MODE_PLAIN;
+  u32 chunk = 64;
+  u32 jfirst = jmin;
+  // Learned order (GMK_MODE_LEARNED with an item model): the one-vertex items
+  // with the best predicted results per unit of device time come first, then
+  // every other one-vertex item in ID order; tuples of two or more vertices
+  // follow in blocks as below.
+  if (prm->mode == GMK_MODE_LEARNED && c->has_item_model && jmin == 1) {
+    T.slice = LEARNED_SLICE;
+    LearnedOrder order;
+    const auto t0 = std::chrono::steady_clock::now();
+    if ((r = learned_front(c, a_lo, a_hi, k, LEARNED_FRONT, order.cand))) return r;
+    rs.ms_schedule += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    TupleFeed feed(T, 1, chunk);
+    std::vector<u32> taken;
+    while (!order.done()) {
+      const auto t1 = std::chrono::steady_clock::now();
+      u32 u = order.next(a_lo);
+      rs.ms_schedule += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t1).count();
+      taken.push_back(u);
+      if ((r = feed.push(&u)) || T.stop) return r;
+    }
+    std::vector<uint8_t> done(a_hi - a_lo, 0);
+    for (u32 u : taken) done[u - a_lo] = 1;
+    for (u32 u = a_lo; u < a_hi; ++u)
+      if (!done[u - a_lo] && ((r = feed.push(&u)) || T.stop)) return r;
+    if ((r = feed.flush()) || T.stop) return r;
+    T.slice = 0;
+    jfirst = 2;
+  }
   // Tuples come in blocks of 4096 first vertices, tuple size outermost within a
   // block.  Plain mode takes the same order one first vertex per launch
   // (per-seed processing); the other modes fill each launch.
   const u32 block = 4096u;
-  const bool per_seed = prm->mode == GMK_MODE_PLAIN;
-  // Launch sizes ramp up from 64 tuples (x4 per launch) to TCH, so a run that
-  // reaches its limit within the first tuples stops early; the order is the same.
-  u32 chunk = 64;
-  std::vector<u32> buf;
-  buf.reserve((size_t)TCH * thin::MAXJ);
   for (u32 s0 = a_lo; s0 < a_hi && !T.stop; s0 += block) {
     const u32 s1 = std::min(a_hi, s0 + block);
-    for (u32 j = jmin; j <= jmax && !T.stop; ++j) {
-      buf.clear();
-      u32 count = 0;
-      u64 est = 0;
-      auto flush = [&]() -> int {
-        if (count == 0) return GMK_OK;
-        int rr = run_chunk(T, buf.data(), count, j);
-        buf.clear(); count = 0; est = 0;
-        chunk = std::min<u32>(chunk * 4, TCH);
-        return rr;
-      };
+    for (u32 j = jfirst; j <= jmax && !T.stop; ++j) {
+      TupleFeed feed(T, j, chunk);
       for (u32 s = s0; s < s1 && !T.stop; ++s) {
         if (j > 1 && s + (j - 1) >= a_hi) continue;
         u32 idx[thin::MAXJ];
         idx[0] = s;
         for (u32 i = 1; i < j; ++i) idx[i] = s + i;
         for (;;) {
-          u64 need = 0;
-          for (u32 i = 0; i < j; ++i) need += 2ull * (off[idx[i] + 1] - off[idx[i]]);
-          if (count >= chunk || est + need > ARENA) { if ((r = flush())) return r; if (T.stop) break; }
-          buf.insert(buf.end(), idx, idx + j);
-          ++count;
-          est += need;
+          if ((r = feed.push(idx))) return r;
+          if (T.stop) break;
           if (j == 1) break;
           int p = (int)j - 1;
           while (p >= 1 && idx[p] == a_hi - (j - p)) --p;
@@ -218,10 +312,63 @@ int gmk_run_thin(gmk_ctx* c, const gmk_params* prm, gmk_stats* st, RunState& rs,
           ++idx[p];
           for (u32 q = p + 1; q < j; ++q) idx[q] = idx[q - 1] + 1;
         }
-        if (per_seed && (r = flush())) return r;
+        if (per_seed && (r = feed.flush())) return r;
       }
-      if (!T.stop && (r = flush())) return r;
+      if (!T.stop && (r = feed.flush())) return r;
     }
+  }
+  return GMK_OK;
+}
+
+extern "C" int gmk_probe_items(gmk_ctx* c, const gmk_params* prm, uint32_t first, uint32_t count, uint64_t ranks,
+                               uint64_t* results, double* lane_ms) {
+  if (!c || !prm || !results || !lane_ms || ranks == 0) return GMK_EINVAL;
+  if (c->n == 0) { c->err = "no graph loaded"; return GMK_ENOGRAPH; }
+  const int side_a = first < c->n_left ? 0 : 1;
+  if ((u64)first + count > (side_a ? c->n : c->n_left)) { c->err = "items must lie on one side of the graph"; return GMK_EINVAL; }
+  if (prm->k < 1 || prm->k > 3 || prm->theta > 1) {
+    c->err = "probes cover one-vertex items: 1 <= k <= 3 and theta <= 1";
+    return GMK_EINVAL;
+  }
+  HIPCHK(c, hipSetDevice(c->device));
+  int khz = 0;
+  HIPCHK(c, hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, c->device));
+  RunState rs{};
+  rs.keep = false;
+  rs.limit = ~0ull;
+  int r;
+  if ((r = ensure(c, c->counters, sizeof(RunCounters)))) return r;
+  HIPCHK(c, hipMemsetAsync(c->counters.p, 0, sizeof(RunCounters), c->stream));
+  if ((r = ensure(c, c->res, 4))) return r;
+  c->res_words = 0;
+  c->h_cnt->res_count = 0;
+  DeviceVec log;
+  if ((r = ensure(c, log, (size_t)TCH * 16))) return r;
+  HIPCHK(c, hipMemsetAsync(log.p, 0, (size_t)TCH * 16, c->stream));
+  std::vector<u64> res(count, 0), ticks(count, 0);
+  ThinRun T;
+  gmk_params p2 = *prm;
+  p2.mode = GMK_MODE_BATCHED;
+  p2.limit = 0;
+  const u32 min_b = side_a ? std::max<u32>(std::max<u32>(prm->theta, 1), 2 * prm->k + 1) : std::max<u32>(prm->theta, 1);
+  if ((r = thin_setup(c, &p2, rs, side_a, min_b, T))) { (void)hipFree(log.p); return r; }
+  T.probe_cap = ranks;
+  T.probe_first = first;
+  T.probe_res = &res;
+  T.probe_ticks = &ticks;
+  T.a.item_log = (u64*)log.p;
+  u32 chunk = TCH;
+  {
+    TupleFeed feed(T, 1, chunk);
+    for (u32 u = first; u < first + count && !r; ++u) r = feed.push(&u);
+    if (!r) r = feed.flush();
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  (void)hipFree(log.p);
+  if (r) return r;
+  for (u32 i = 0; i < count; ++i) {
+    results[i] = res[i];
+    lane_ms[i] = khz > 0 ? (double)ticks[i] / (double)khz : 0.0;
   }
   return GMK_OK;
 }

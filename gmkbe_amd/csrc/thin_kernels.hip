@@ -308,10 +308,14 @@ __global__ __launch_bounds__(1024) void thin_scan_counts(u32* cnt, u32 n, u32* f
 // then each thread's segment from the sum of the segments before it (the
 // phases are thin::scan_phase_*).  A family whose prefix is finite keeps it
 // even when the running sum saturates inside its segment.
-__global__ __launch_bounds__(thin::TSCAN) void thin_scan_totals(const thin::Family* fams, u32 nfam, u64* prefix) {
+__global__ __launch_bounds__(thin::TSCAN) void thin_scan_totals(const thin::Family* fams, u32 nfam, u64* prefix,
+                                                                 u64 base, u64 cap) {
   __shared__ u64 part[thin::TSCAN];
   const u32 t = threadIdx.x;
-  auto total = [fams](u32 i) { return fams[i].total; };
+  auto total = [fams, base, cap](u32 i) {
+    const u64 v = fams[i].total, rest = v > base ? v - base : 0;
+    return rest < cap ? rest : cap;
+  };
   part[t] = thin::scan_phase_sum(nfam, t, total);
   __syncthreads();
   for (u32 d = 1; d < thin::TSCAN; d <<= 1) {
@@ -333,6 +337,15 @@ __device__ __forceinline__ void flush_check_stats(const ThinArgs& a, const thin:
     atomicAdd((unsigned long long*)&sp[2], (unsigned long long)se);
     atomicAdd((unsigned long long*)&sp[5], (unsigned long long)sb);
   }
+}
+
+// Probe runs: adds a rank's outcome and the lane time since t0 to its tuple's
+// log entry.
+__device__ __forceinline__ void log_item(const ThinArgs& a, const thin::Family* f, bool emitted, u64 t0) {
+  if (!a.item_log || !f) return;
+  const u64 dt = (u64)wall_clock64() - t0;
+  if (emitted) atomicAdd((unsigned long long*)&a.item_log[2 * f->tuple], 1ull);
+  atomicAdd((unsigned long long*)&a.item_log[2 * f->tuple + 1], (unsigned long long)dt);
 }
 
 // K: misses allowed per vertex (1..3).  A family chooses at most 2 K^2 vertices
@@ -359,7 +372,7 @@ __device__ __forceinline__ const thin::Family* unrank_rank(const ThinArgs& a, u6
     if (a.prefix[mid] <= r) lo = mid; else hi = mid;
   }
   const thin::Family* f = a.fams + lo;
-  u64 rr = r - a.prefix[lo];
+  u64 rr = r - a.prefix[lo] + a.rank_base;
   cost.bytes += 8ull * 8 + 16ull * f->ntypes;  // prefix probes, family header and type fields
   if (rr >= f->total) return nullptr;  // past the family's combinations: no rank
   for (u32 t = 0; t < f->ntypes; ++t) {
@@ -541,6 +554,7 @@ __global__ __launch_bounds__(256) void thin_check_deferred(ThinArgs a) {
     i = (u32)wave_bcast_u64(i, 0);
     if (i >= nd) break;
     if (limit_reached(a)) break;
+    const u64 t0 = a.item_log ? (u64)wall_clock64() : 0;
     const u64 r = a.defer[i];
     u32 ch[NM], ctsz[NM];
     u32 nch = 0;
@@ -575,7 +589,9 @@ __global__ __launch_bounds__(256) void thin_check_deferred(ThinArgs a) {
     }
     const bool maximal = cs.decided >= 0 ? cs.decided == 1 : !__ballot(joins);
     if (lane == 0) ++n_tested;
-    if (emit_wave<K>(a, lane == 0 && maximal, f, ch, nch, blockIdx.x * 4 + (threadIdx.x >> 6), cost)) ++n_emit;
+    const bool em = emit_wave<K>(a, lane == 0 && maximal, f, ch, nch, blockIdx.x * 4 + (threadIdx.x >> 6), cost);
+    if (em) ++n_emit;
+    if (lane == 0) log_item(a, f, em, t0);
   }
   flush_check_stats<K>(a, cost, n_emit, n_tested);
 }
@@ -588,10 +604,13 @@ __global__ __launch_bounds__(256) void thin_check(ThinArgs a) {
   u64 n_emit = 0, n_tested = 0;
   thin::Cost cost;
   if (!limit_reached(a)) {
+    const u64 t0 = a.item_log ? (u64)wall_clock64() : 0;
     const thin::Family* f = nullptr;
     u32 ch[NM], nch = 0;
     const bool want = check_rank<K>(a, r, f, ch, nch, cost, n_tested);
-    if (emit_wave<K>(a, want, f, ch, nch, blockIdx.x * 4 + (threadIdx.x >> 6), cost)) ++n_emit;
+    const bool em = emit_wave<K>(a, want, f, ch, nch, blockIdx.x * 4 + (threadIdx.x >> 6), cost);
+    if (em) ++n_emit;
+    log_item(a, f, em, t0);
   }
   flush_check_stats<K>(a, cost, n_emit, n_tested);
 }
@@ -612,10 +631,13 @@ __global__ __launch_bounds__(256) void thin_check_chunked(ThinArgs a) {
     const u64 c0 = a.r0 + (u64)chunk * 64;
     if (c0 >= a.r1) break;
     if (limit_reached(a)) break;
+    const u64 t0 = a.item_log ? (u64)wall_clock64() : 0;
     const thin::Family* f = nullptr;
     u32 ch[NM], nch = 0;
     const bool want = check_rank<K>(a, c0 + lane, f, ch, nch, cost, n_tested);
-    if (emit_wave<K>(a, want, f, ch, nch, blockIdx.x * 4 + (threadIdx.x >> 6), cost)) ++n_emit;
+    const bool em = emit_wave<K>(a, want, f, ch, nch, blockIdx.x * 4 + (threadIdx.x >> 6), cost);
+    if (em) ++n_emit;
+    log_item(a, f, em, t0);
   }
   flush_check_stats<K>(a, cost, n_emit, n_tested);
 }
@@ -670,8 +692,8 @@ hipError_t launch_thin_rank_index(const u64* prefix, u32 nfam, u64 r0, u64 r1, u
   return hipGetLastError();
 }
 
-hipError_t launch_thin_scan(const thin::Family* fams, u32 nfam, u64* prefix, hipStream_t s) {
-  hipLaunchKernelGGL(thin_scan_totals, dim3(1), dim3(thin::TSCAN), 0, s, fams, nfam, prefix);
+hipError_t launch_thin_scan(const thin::Family* fams, u32 nfam, u64* prefix, u64 base, u64 cap, hipStream_t s) {
+  hipLaunchKernelGGL(thin_scan_totals, dim3(1), dim3(thin::TSCAN), 0, s, fams, nfam, prefix, base, cap);
   return hipGetLastError();
 }
 

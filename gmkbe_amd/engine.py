@@ -40,7 +40,8 @@ class Stats(ctypes.Structure):
                 ("dedup_probes", ctypes.c_uint64), ("dedup_dups", ctypes.c_uint64), ("ms", ctypes.c_double),
                 ("ms_expand", ctypes.c_double), ("hit_limit", ctypes.c_int32), ("spills", ctypes.c_uint32),
                 ("ms_kernel", ctypes.c_double * 4), ("launches_kernel", ctypes.c_uint64 * 4),
-                ("bytes_kernel", ctypes.c_uint64 * 4), ("dedup_undecided", ctypes.c_uint64)]
+                ("bytes_kernel", ctypes.c_uint64 * 4), ("dedup_undecided", ctypes.c_uint64),
+                ("ms_schedule", ctypes.c_double)]
 
     KERNELS = ("bb_expand", "thin_build_families", "thin_check", "partition_build")
 
@@ -171,6 +172,15 @@ def load_library():
     L.gmk_mlp_grad.restype = ctypes.c_int
     L.gmk_set_cost_model.argtypes = [ctypes.c_void_p, P(Mlp)]
     L.gmk_set_cost_model.restype = ctypes.c_int
+    L.gmk_set_item_model.argtypes = [ctypes.c_void_p, P(Mlp)]
+    L.gmk_set_item_model.restype = ctypes.c_int
+    L.gmk_item_features.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, fp]
+    L.gmk_item_features.restype = ctypes.c_int
+    L.gmk_item_predict.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, fp]
+    L.gmk_item_predict.restype = ctypes.c_int
+    L.gmk_probe_items.argtypes = [ctypes.c_void_p, P(Params), ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64,
+                                  P(ctypes.c_uint64), P(ctypes.c_double)]
+    L.gmk_probe_items.restype = ctypes.c_int
     L.gmk_set_tuning.argtypes = [ctypes.c_void_p, P(Tuning)]
     L.gmk_set_tuning.restype = ctypes.c_int
     L.gmk_get_tuning.argtypes = [ctypes.c_void_p, P(Tuning)]
@@ -361,6 +371,37 @@ def _engine_batch_methods():
         self._check(self._L.gmk_set_cost_model(self._h, ctypes.byref(model) if model is not None else None),
                     "gmk_set_cost_model")
 
+    def item_features(self, first: int, count: int, k: int):
+        """Features of the one-vertex work items first .. first+count-1 (gmk_item_features)."""
+        out = np.zeros((count, MLP_IN), dtype=np.float32)
+        self._check(self._L.gmk_item_features(self._h, first, count, k, _fp(out)), "gmk_item_features")
+        return out
+
+    def item_predict(self, first: int, count: int, k: int):
+        """The set item model's scores of items first .. first+count-1 (gmk_item_predict)."""
+        out = np.zeros(count, dtype=np.float32)
+        self._check(self._L.gmk_item_predict(self._h, first, count, k, _fp(out)), "gmk_item_predict")
+        return out
+
+    def probe_items(self, first: int, count: int, k: int, ranks: int = 1024):
+        """Results and device lane ms of the first `ranks` combinations of each
+        family of items first .. first+count-1 (gmk_probe_items)."""
+        prm = Params(k, 1, 0, MODES["batched"], PARTS["seeded"], 0, 0, 0, 0, 0)
+        res = np.zeros(count, dtype=np.uint64)
+        ms = np.zeros(count, dtype=np.float64)
+        self._check(self._L.gmk_probe_items(self._h, ctypes.byref(prm), first, count, ranks,
+                                            res.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
+                                            ms.ctypes.data_as(ctypes.POINTER(ctypes.c_double))), "gmk_probe_items")
+        return res, ms
+
+    def set_item_model(self, model):
+        self._check(self._L.gmk_set_item_model(self._h, ctypes.byref(model) if model is not None else None),
+                    "gmk_set_item_model")
+
+    Engine.item_features = item_features
+    Engine.item_predict = item_predict
+    Engine.probe_items = probe_items
+    Engine.set_item_model = set_item_model
     Engine.enumerate_batch = enumerate_batch
     Engine.mlp_train = mlp_train
     Engine.mlp_predict = mlp_predict

@@ -29,13 +29,23 @@ enum {
 };
 
 /* Work order of the enumeration.
- *   GMK_MODE_PLAIN   one partition per launch sequence (the reference's
+ *   GMK_MODE_PLAIN   one work item per launch sequence: one first vertex of
+ *                    the thin enumerator, one seed partition (the reference's
  *                    one-candidate-at-a-time structure, src/main.cu:1078-1506)
- *   GMK_MODE_BATCHED all partitions share one device work stack drained by one
- *                    persistent launch with work stealing
- *   GMK_MODE_LEARNED as BATCHED, partitions ordered and split by the cost model
- *                    loaded with gmk_set_cost_model (falls back to the degree
- *                    heuristic when none is loaded) */
+ *   GMK_MODE_BATCHED work items fill each launch: thin tuples by the tens of
+ *                    thousands, search states of all partitions in one device
+ *                    stack per bitset width (one workgroup per state, or a
+ *                    persistent grid taking states from a device counter, as
+ *                    the tuning table says); items in the degree order of the
+ *                    vertex IDs (the heuristic order)
+ *   GMK_MODE_LEARNED as BATCHED, with the one-vertex items ordered by the item
+ *                    cost model set with gmk_set_item_model: best predicted
+ *                    results per unit of device time first (single-graph
+ *                    runs); batches order their graphs by the model set with
+ *                    gmk_set_cost_model.  Without a model the heuristic order
+ *                    is kept.  Every item is still enumerated exactly: the
+ *                    order decides only which results a limited run reaches
+ *                    first, and how the work packs into launches. */
 enum { GMK_MODE_PLAIN = 0, GMK_MODE_BATCHED = 1, GMK_MODE_LEARNED = 2 };
 
 /* Partition scheme.
@@ -85,6 +95,8 @@ typedef struct {
   uint64_t bytes_kernel[4];  /* algorithmic bytes read + written */
   uint64_t dedup_undecided;  /* results kept because a just-claimed slot with the same
                                 fingerprint had not yet shown its check word (dedup) */
+  double ms_schedule;        /* GMK_MODE_LEARNED: host time of the item ordering (scoring
+                                kernel, selection, sort), inside the window `ms` */
 } gmk_stats;
 
 /* Create a context on HIP device `device` (-d, src/main.cu:1765-1766). */
@@ -188,15 +200,42 @@ int gmk_mlp_predict(gmk_ctx* ctx, const gmk_mlp* model, const float* X, uint32_t
  * gradient are left zero), for gradient checks. */
 int gmk_mlp_grad(gmk_ctx* ctx, const gmk_mlp* model, const float* X, const float* y, uint32_t n, gmk_mlp* grad,
                  float* loss);
-/* Cost model used by GMK_MODE_LEARNED (null: the size heuristic). */
+/* Cost model used by GMK_MODE_LEARNED batches (null: the size heuristic). */
 int gmk_set_cost_model(gmk_ctx* ctx, const gmk_mlp* model);
+
+/* ---- Learned schedule of single-graph runs (no counterpart in the reference)
+ * A work item is a one-vertex tuple u of the thin enumerator: u, all of N(u)
+ * and up to k vertices u misses, checked for maximality combination by
+ * combination.  The item model predicts log(results per ms of device lane
+ * time) of an item from its features.
+ *
+ * Features of items first .. first + count - 1 (global IDs, one side), for k:
+ * log(1+d(u)), log(1+|other side| - d(u)), mean, max and standard deviation of
+ * log(1+d(v)) over N(u), log(1+two-hop paths from u), log(1+min d(v)), k
+ * (moments over at most 32 neighbours spread evenly over N(u)).  out is
+ * count x GMK_MLP_IN. */
+int gmk_item_features(gmk_ctx* ctx, uint32_t first, uint32_t count, uint32_t k, float* out);
+/* Predictions of the item model set with gmk_set_item_model for items first ..
+ * first + count - 1 (the scores the learned order sorts by; higher first). */
+int gmk_item_predict(gmk_ctx* ctx, uint32_t first, uint32_t count, uint32_t k, float* out);
+/* Measured yield and cost of items first .. first + count - 1 (one side of the
+ * loaded graph; k = params->k, theta <= 1): the first `ranks` combinations of
+ * each of an item's families are checked, and per item the maximal results
+ * found and the device lane time spent (ms, summed over the lanes that checked
+ * its combinations, from the kernels' wall clock) are returned.  This is the
+ * method's own log that the item model trains on. */
+int gmk_probe_items(gmk_ctx* ctx, const gmk_params* params, uint32_t first, uint32_t count, uint64_t ranks,
+                    uint64_t* results, double* lane_ms);
+/* Item model of GMK_MODE_LEARNED single-graph runs (null: heuristic order). */
+int gmk_set_item_model(gmk_ctx* ctx, const gmk_mlp* model);
 
 /* ---- Tuning (no counterpart in the reference) -----------------------------
  * Launch parameters of the branch-and-bound expansion per bitset width
  * (index l: 2^l words, l = 0..6), as chosen per size bucket by the autotuner
  * (gmkbe_amd/autotune.py, table under gmkbe_amd/tune/).  Results do not depend
- * on them.  Defaults: the table measured on MI355X (one workgroup per state,
- * 16384 states per launch, dedup load 0.5, chunked thin checks for k = 3). */
+ * on them.  Built-in defaults: one workgroup per state, 16384 states per
+ * launch, dedup load 0.5, one lane per rank in the thin checks; the engine of
+ * gmkbe_amd/engine.py loads the committed MI355X table instead. */
 #define GMK_TUNING_VERSION 2
 typedef struct {
   uint32_t version;                 /* GMK_TUNING_VERSION */

@@ -1,6 +1,7 @@
 """The large single-graph workloads at full size, to their limits: the
 2M x 1M Chung-Lu graph at k = 2 and the 500k x 500k graph with planted
-near-bicliques at k = 3, r = 10,000,000 each, in plain and batched mode.
+near-bicliques at k = 3, r = 10,000,000 each, in plain, batched and learned
+mode (the item model trained as the benchmark trains it).
 Every run returns exactly r results, all distinct (per-record fingerprints),
 with the digest the device accumulated, and a spread sample of 2000 records
 passes the oracle validator (k-biplex, maximal in the whole graph, sides >= 1).
@@ -18,6 +19,17 @@ pytestmark = pytest.mark.gpu
 
 def _check(g, k, limit, mode):
     with Engine(0) as e:
+        if mode == "learned":
+            # the item model as the benchmark trains it: the 50k x 30k graph's
+            # items and this graph's first 5%
+            from gmkbe_amd import schedule
+            samples = []
+            for gt, kt, frac in ((gen.workload("cl50kx30k"), 1, 1.0), (g, k, 0.05)):
+                e.set_graph(gt)
+                X, y, _ = schedule.probe_samples(e, kt, 0, int(gt.n_left * frac))
+                samples.append((X, y))
+            model, _ = schedule.train_item_model(e, samples)
+            e.set_item_model(model)
         e.set_graph(g)
         res = e.enumerate(k=k, theta=1, limit=limit, mode=mode, partition="seeded")
     assert res.count == limit and res.stats["hit_limit"]
@@ -40,13 +52,13 @@ def c5():
     return gen.workload("planted500k")
 
 
-@pytest.mark.parametrize("mode", ["plain", "batched"])
+@pytest.mark.parametrize("mode", ["plain", "batched", "learned"])
 def test_cl2Mx1M_k2_limit(c4, mode):
     print(f"[gpu] cl2Mx1M k=2 r=10000000 {mode}", flush=True)
     _check(c4, 2, 10_000_000, mode)
 
 
-@pytest.mark.parametrize("mode", ["plain", "batched"])
+@pytest.mark.parametrize("mode", ["plain", "batched", "learned"])
 def test_planted500k_k3_limit(c5, mode):
     print(f"[gpu] planted500k k=3 r=10000000 {mode}", flush=True)
     _check(c5, 3, 10_000_000, mode)
