@@ -35,6 +35,13 @@ extern "C" int gmk_create(int device, gmk_ctx** out) {
     delete c;
     return GMK_EDEVICE;
   }
+  // this device's copy of the static scratch array, and pinned staging for it
+  size_t sfl = 0;
+  if (gmk::static_scratch(&c->scratch, &sfl) != GMK_OK ||
+      hipHostMalloc((void**)&c->h_stage, sfl * sizeof(float), hipHostMallocDefault) != hipSuccess) {
+    gmk_destroy(c);
+    return GMK_EDEVICE;
+  }
   *out = c;
   return GMK_OK;
 }
@@ -79,6 +86,7 @@ extern "C" void gmk_destroy(gmk_ctx* c) {
   if (c->d_off) (void)hipFree(c->d_off);
   if (c->d_adj) (void)hipFree(c->d_adj);
   if (c->h_cnt) (void)hipHostFree(c->h_cnt);
+  if (c->h_stage) (void)hipHostFree(c->h_stage);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   if (c->ev2) (void)hipEventDestroy(c->ev2);

@@ -125,7 +125,9 @@ struct gmk_ctx {
   gmk_mlp model{};
   bool has_item_model = false;   // cost model of GMK_MODE_LEARNED single-graph runs (per work item)
   gmk_mlp item_model{};
-  DeviceVec sched_p, sched_hist, sched_sel;  // learned schedule: model parameters, score histogram, selection
+  DeviceVec sched_p, sched_hist, sched_sel;  // learned schedule: model parameters, score keys, selection
+  float* scratch = nullptr;      // device address of the library's static array on this device (gmk_item_predict_static)
+  float* h_stage = nullptr;      // pinned host staging of that array's contents
 };
 
 inline int ensure(gmk_ctx* c, DeviceVec& v, size_t bytes) {
@@ -150,6 +152,9 @@ namespace gmk {
 constexpr int MLP_NPARAM = GMK_MLP_IN * GMK_MLP_HID + GMK_MLP_HID + GMK_MLP_HID * GMK_MLP_HID + GMK_MLP_HID +
                            GMK_MLP_HID + 1;
 void mlp_pack(const gmk_mlp& m, float* P);
+// The device address of the static scratch array of gmk_item_predict_static on
+// the current device, and the array's size in floats.
+int static_scratch(float** ptr, size_t* floats);
 // Learned schedule of single-graph runs: about the 2 `want` one-vertex work
 // items of side A ([a_lo, a_hi)) with the highest predicted results per unit
 // of device time under the item cost model, unsorted, as (score key << 32 |

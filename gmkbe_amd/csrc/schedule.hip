@@ -13,6 +13,8 @@
 // sorts only the gathered items.
 #include <algorithm>
 #include <cmath>
+#include <cstring>
+#include <string>
 #include <vector>
 
 #include "gmk_host.h"
@@ -242,6 +244,56 @@ static int item_eval(gmk_ctx* c, uint32_t first, uint32_t count, uint32_t k, flo
   (void)hipFree(f.p);
   if (p.p) (void)hipFree(p.p);
   if (e != hipSuccess) { c->err = std::string("item kernel: ") + hipGetErrorString(e); return GMK_EDEVICE; }
+  return GMK_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Predictions with no allocation (gmk_item_predict_static).  Its scratch is a
+// file-scope device array: the model parameters (gmk::mlp_pack order, then mu
+// and sd), then the scores of up to GMK_STATIC_ITEMS items.  In host code the
+// array's name is not a device address: gmk_create takes this device's address
+// of it once (static_scratch, with the device current) and keeps it in the
+// context; host code uses only that address.
+// ---------------------------------------------------------------------------
+constexpr size_t SCRATCH_PARAMS = (size_t)MLP_NPARAM + 2 * IN;
+constexpr size_t SCRATCH_FLOATS = SCRATCH_PARAMS + GMK_STATIC_ITEMS;
+static __device__ __align__(256) float scratch[SCRATCH_FLOATS];
+
+int gmk::static_scratch(float** ptr, size_t* floats) {
+  void* p = nullptr;
+  if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return GMK_EDEVICE;
+  *ptr = (float*)p;
+  *floats = SCRATCH_FLOATS;
+  return GMK_OK;
+}
+
+extern "C" int gmk_item_predict_static(gmk_ctx* c, uint32_t first, uint32_t count, uint32_t k, float* out) {
+  if (!c || (!out && count)) return GMK_EINVAL;
+  if (c->n == 0) { c->err = "no graph loaded"; return GMK_ENOGRAPH; }
+  if (!c->has_item_model) { c->err = "no item model set"; return GMK_EINVAL; }
+  if (count > GMK_STATIC_ITEMS) {
+    c->err = "gmk_item_predict_static: " + std::to_string(count) + " items asked, at most " +
+             std::to_string(GMK_STATIC_ITEMS) + " per call (the static scratch array's size)";
+    return GMK_EINVAL;
+  }
+  if (count == 0) return GMK_OK;
+  const bool left = first < c->n_left;
+  if ((u64)first + count > (left ? c->n_left : c->n)) { c->err = "items must lie on one side of the graph"; return GMK_EINVAL; }
+  if (!c->scratch || !c->h_stage) { c->err = "static scratch not set up"; return GMK_EDEVICE; }
+  HIPCHK(c, hipSetDevice(c->device));
+  hipStream_t s = c->stream;
+  // the parameters go through the context's pinned staging (no allocation)
+  mlp_pack(c->item_model, c->h_stage);
+  for (int q = 0; q < IN; ++q) { c->h_stage[MLP_NPARAM + q] = c->item_model.mu[q]; c->h_stage[MLP_NPARAM + IN + q] = c->item_model.sd[q]; }
+  HIPCHK(c, hipMemcpyAsync(c->scratch, c->h_stage, SCRATCH_PARAMS * 4, hipMemcpyHostToDevice, s));
+  float* score = c->scratch + SCRATCH_PARAMS;
+  ItemArgs a{c->d_off, c->d_adj, first, count, left ? c->n - c->n_left : c->n_left, (float)k, c->scratch, nullptr,
+             nullptr, nullptr, score};
+  hipLaunchKernelGGL(item_score, dim3((count + 255) / 256), dim3(256), 0, s, a);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipMemcpyAsync(c->h_stage + SCRATCH_PARAMS, score, (size_t)count * 4, hipMemcpyDeviceToHost, s));
+  HIPCHK(c, hipStreamSynchronize(s));
+  memcpy(out, c->h_stage + SCRATCH_PARAMS, (size_t)count * 4);
   return GMK_OK;
 }
 

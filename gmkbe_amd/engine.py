@@ -56,6 +56,7 @@ class Stats(ctypes.Structure):
 
 
 MLP_IN, MLP_HID = 8, 64
+STATIC_ITEMS = 65536  # GMK_STATIC_ITEMS
 
 
 class GraphDesc(ctypes.Structure):
@@ -178,6 +179,8 @@ def load_library():
     L.gmk_item_features.restype = ctypes.c_int
     L.gmk_item_predict.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, fp]
     L.gmk_item_predict.restype = ctypes.c_int
+    L.gmk_item_predict_static.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, fp]
+    L.gmk_item_predict_static.restype = ctypes.c_int
     L.gmk_probe_items.argtypes = [ctypes.c_void_p, P(Params), ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64,
                                   P(ctypes.c_uint64), P(ctypes.c_double)]
     L.gmk_probe_items.restype = ctypes.c_int
@@ -383,6 +386,13 @@ def _engine_batch_methods():
         self._check(self._L.gmk_item_predict(self._h, first, count, k, _fp(out)), "gmk_item_predict")
         return out
 
+    def item_predict_static(self, first: int, count: int, k: int):
+        """As item_predict, through the entry point that allocates nothing
+        (gmk_item_predict_static; at most STATIC_ITEMS items per call)."""
+        out = np.zeros(count, dtype=np.float32)
+        self._check(self._L.gmk_item_predict_static(self._h, first, count, k, _fp(out)), "gmk_item_predict_static")
+        return out
+
     def probe_items(self, first: int, count: int, k: int, ranks: int = 1024):
         """Results and device lane ms of the first `ranks` combinations of each
         family of items first .. first+count-1 (gmk_probe_items)."""
@@ -400,6 +410,7 @@ def _engine_batch_methods():
 
     Engine.item_features = item_features
     Engine.item_predict = item_predict
+    Engine.item_predict_static = item_predict_static
     Engine.probe_items = probe_items
     Engine.set_item_model = set_item_model
     Engine.enumerate_batch = enumerate_batch

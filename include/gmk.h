@@ -218,6 +218,15 @@ int gmk_item_features(gmk_ctx* ctx, uint32_t first, uint32_t count, uint32_t k, 
 /* Predictions of the item model set with gmk_set_item_model for items first ..
  * first + count - 1 (the scores the learned order sorts by; higher first). */
 int gmk_item_predict(gmk_ctx* ctx, uint32_t first, uint32_t count, uint32_t k, float* out);
+/* As gmk_item_predict, calling no allocator (no hipMalloc, hipFree or
+ * allocating library routine): for callers that cannot allocate, such as
+ * tight serving loops.  It has no counterpart in the reference.  Its device
+ * scratch (the staged model and the scores) is a fixed array of the library,
+ * one per GPU, held until the process ends: at most GMK_STATIC_ITEMS items per
+ * call (larger counts return GMK_EINVAL).  Calls on one GPU share that array,
+ * so calls that overlap on one GPU race; `out` is host memory. */
+#define GMK_STATIC_ITEMS 65536
+int gmk_item_predict_static(gmk_ctx* ctx, uint32_t first, uint32_t count, uint32_t k, float* out);
 /* Measured yield and cost of items first .. first + count - 1 (one side of the
  * loaded graph; k = params->k, theta <= 1): the first `ranks` combinations of
  * each of an item's families are checked, and per item the maximal results
