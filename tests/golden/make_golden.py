@@ -11,6 +11,8 @@ Outputs (data only):
                             from the edge list this script writes
   golden.json               per (graph, k, theta): result count and digests of
                             the reference BPPivot's printed result set
+  c3_slice.json             (--c3) per graph of the first 40 of the many-graphs
+                            workload's draw: theta and the reference's digest
   sets/<case>.u32           the full record stream for small cases
 """
 from __future__ import annotations
@@ -71,28 +73,40 @@ def example_graph_edges(path: str) -> tuple[np.ndarray, np.ndarray]:
     return np.asarray(us, dtype=np.uint32), np.asarray(vs, dtype=np.uint32)
 
 
-def run_bppivot(graph_path: str, k: int, theta: int) -> np.ndarray:
+def _unlimit_stack():
+    import resource
+    resource.setrlimit(resource.RLIMIT_STACK, (resource.RLIM_INFINITY, resource.RLIM_INFINITY))
+
+
+def run_bppivot(graph_path: str, k: int, theta: int, timeout: int = 600, limit: int = 0) -> np.ndarray:
     """Runs the reference BPPivot with -o 1 and returns its result set as a
     record stream with global IDs (right labels are id - nLeft in its output:
-    load_bipartite.hpp:58)."""
+    load_bipartite.hpp:58).  The binary is the direct child (no shell), so a
+    timeout ends it; limit > 0 passes -n limit."""
     g = graphio.read_graph_bin(graph_path)
     with tempfile.TemporaryDirectory() as td:
         gp = os.path.join(td, "g.graph.bin")
         with open(graph_path, "rb") as f, open(gp, "wb") as o:
             o.write(f.read())
         out = os.path.join(td, "out.txt")
-        subprocess.run(f"ulimit -s unlimited; {os.path.join(BIN, 'bppivot')} -d {gp} -q {theta} -k {k} -o 1 > {out}",
-                       shell=True, check=True, timeout=600)
-        parts: list[np.ndarray] = []
-        L = None
+        cmd = [os.path.join(BIN, "bppivot"), "-d", gp, "-q", str(theta), "-k", str(k), "-o", "1"]
+        if limit:
+            cmd += ["-n", str(limit)]
+        with open(out, "w") as fo:
+            subprocess.run(cmd, stdout=fo, check=True, timeout=timeout, preexec_fn=_unlimit_stack)
+        Ls, Rs = [], []
         with open(out) as f:
             for ln in f:
                 s = ln.strip()
                 if s.startswith("SL:"):
-                    L = np.sort(np.array(s[3:].replace(",", " ").split(), dtype=np.uint32))
+                    Ls.append(s[3:])
                 elif s.startswith("SR:"):
-                    R = np.sort(np.array(s[3:].replace(",", " ").split(), dtype=np.uint32)) + np.uint32(g.n_left)
-                    parts.append(np.concatenate([[len(L)], L, [len(R)], R]).astype(np.uint32))
+                    Rs.append(s[3:])
+        parts: list[np.ndarray] = []
+        for ls, rs in zip(Ls, Rs):
+            L = np.sort(np.array(ls.replace(",", " ").split(), dtype=np.uint32))
+            R = np.sort(np.array(rs.replace(",", " ").split(), dtype=np.uint32)) + np.uint32(g.n_left)
+            parts.append(np.concatenate([[len(L)], L, [len(R)], R]).astype(np.uint32))
     return np.concatenate(parts) if parts else np.zeros(0, dtype=np.uint32)
 
 
@@ -142,5 +156,47 @@ def main() -> None:
             json.dump(out, f, indent=1)
 
 
+C3_SLICE = 40          # the first graphs of the many-graphs workload's draw
+C3_MAX_COUNT = 3_000_000
+C3_TIMEOUT = 60
+
+
+def c3_case(i: int):
+    """Digest of graph i of the many-graphs workload (gen.er_batch_specs with
+    its committed seed) at the smallest theta in 1, 2k+1, 2k+3 at which the
+    reference BPPivot finishes within C3_TIMEOUT seconds with at most
+    C3_MAX_COUNT results."""
+    a, b, p, k, sd = gen.er_batch_specs(2000, gen.SEED_C3)[i]
+    g = gen.er(a, b, p, sd)
+    with tempfile.TemporaryDirectory() as td:
+        path = os.path.join(td, "g.graph.bin")
+        graphio.write_graph_bin(path, g)
+        for theta in (1, 2 * k + 1, 2 * k + 3):
+            try:
+                words = run_bppivot(path, k, theta, timeout=C3_TIMEOUT, limit=C3_MAX_COUNT + 1)
+            except subprocess.TimeoutExpired:
+                continue
+            dg = oracle_lib.digest(words)
+            if dg.count > C3_MAX_COUNT:
+                continue
+            return {"index": i, "n_left": a, "n_right": b, "p": p, "k": k, "theta": theta, "count": int(dg.count),
+                    "fp_sum": str(dg.fp_sum), "fp_xor": str(dg.fp_xor)}
+    return {"index": i, "n_left": a, "n_right": b, "p": p, "k": k, "theta": None}
+
+
+def main_c3() -> None:
+    from concurrent.futures import ProcessPoolExecutor
+    with ProcessPoolExecutor(8) as ex:
+        cases = list(ex.map(c3_case, range(C3_SLICE)))
+    for c in cases:
+        print(c["index"], c["k"], c["theta"], c.get("count"), flush=True)
+    with open(os.path.join(HERE, "c3_slice.json"), "w") as f:
+        json.dump({"seed": gen.SEED_C3, "draw": 2000, "max_count": C3_MAX_COUNT, "timeout_s": C3_TIMEOUT,
+                   "cases": cases}, f, indent=1)
+
+
 if __name__ == "__main__":
-    main()
+    if "--c3" in sys.argv:
+        main_c3()
+    else:
+        main()
