@@ -44,6 +44,7 @@ struct __align__(16) Smem {
   uint8_t ms[2][64 * W];
   uint8_t widx[W];         // words of the opposite side that the current pass reads
   u64 rm[2][W];            // candidates removed by the degree pruning of this state
+  u32 c1[64 * W];          // child bound: neighbours of the include missing only this S member
 };
 
 // Lists in sm.widx the words w < wb with mask_a[w] | mask_b[w] != 0; returns
@@ -224,10 +225,12 @@ __device__ void emit_state(Smem<W>& sm, const PartDesc& P, const u32* maps, cons
   }
 }
 
-// Write one child.  The child is the parent with `moved` (on side ms_side)
-// moved from C to X, then (when inc != NONE) vertex inc of side us included.
+// Writes one child: the parent with `moved` (on side mv_side) moved from C to
+// X, then (when inc != NONE) vertex inc of side us included.  A child that
+// fails its bounds (below) is written as a hole (partition INVALID_PID), which
+// the next expansion skips; returns whether it passed.
 template <int W>
-__device__ void write_child(Smem<W>& sm, const PartDesc& P, const u64* rows, u32 k, u64* out, u32 pid,
+__device__ bool write_child(Smem<W>& sm, const PartDesc& P, const u64* rows, u32 k, u64* out, u32 pid,
                             u32 inc, int us, int mv_side, const u64* moved /* W words or null */,
                             u32 moved_single /* used when moved == null, NONE = none */) {
   const int lane = threadIdx.x;
@@ -263,7 +266,7 @@ __device__ void write_child(Smem<W>& sm, const PartDesc& P, const u64* rows, u32
     }
     wave_barrier();
   }
-  if (lane == 0) out[0] = pid;
+  u32 sc0 = 0, sc1 = 0;  // |S| + |C| of the child per side
   for (int j = lane; j < NWS; j += 64) {
     const int side = j / (3 * W);
     const int which = (j / W) % 3;
@@ -288,8 +291,94 @@ __device__ void write_child(Smem<W>& sm, const PartDesc& P, const u64* rows, u32
         else val &= ~bit & sm.andsat[w];
       }
     }
-    out[1 + j] = val;
+    if (which != 2) {
+      if (side == 0) sc0 += __popcll(val);
+      else sc1 += __popcll(val);
+    }
+    if (out) out[1 + j] = val;
   }
+  // Bounds of the child (biplex.cpp:949-951, 1040-1042, 1054-1056): S + C must
+  // reach lb on both sides, and with an include u the opposite side must be
+  // able to reach lb.  That side gains at most k - ms(u) non-neighbours of u,
+  // and neighbours w of u in C each missing ms(w) members of S + u; those
+  // misses draw on the members' remaining budgets k - ms.  Counted as
+  // biplexUpperBound (biplex.cpp:556-603) counts them: every w missing no
+  // member, per member x at most its budget of the w missing only x, then w
+  // missing two or more while their misses fit what is left of the budgets'
+  // sum (where biplexUpperBound also checks one member per w, so this never
+  // bounds below it).
+  const u32 lb = P.lb;
+  if (lb <= 1) {  // every side of S + C is non-empty in any state that can emit: nothing to bound
+    if (out && lane == 0) out[0] = pid;
+    return true;
+  }
+  bool keep = (u32)wave_sum_u64(sc0) >= lb && (u32)wave_sum_u64(sc1) >= lb;
+  const u32 ms_u = inc != NONE ? sm.ms[us][inc] : 0;
+  const u32 nso = inc != NONE ? popc_set<W>(setp(sm.set, os, 0, W), wos) : 0;
+  // the bound only matters when |S| and u's own budget do not reach lb already
+  if (keep && inc != NONE && nso + (k > ms_u ? k - ms_u : 0) < lb) {
+    const u64* Sus = setp(sm.set, us, 0, W);
+    const u64* Cos = setp(sm.set, os, 1, W);
+    const u32 wus = us ? P.wr : P.wl, nus = us ? P.nr : P.nl, nos = os ? P.nr : P.nl;
+    u64 ub = (k > ms_u ? k - ms_u : 0);
+    u64 sup = ub;  // u's own budget
+    for (u32 i = 0; i < wus; ++i) {
+      const u32 v = i * 64 + lane;
+      if (v < nus && ((Sus[i] >> lane) & 1ull)) sup += k - sm.ms[us][v];
+    }
+    sup = wave_sum_u64(sup);
+    ub += nso;
+    // neighbours of u in C of the other side, counted by their ms (cost)
+    const u64* mvos = (mv_side == os && moved) ? moved : nullptr;
+    for (u32 cst = 0; cst <= k && ub < lb; ++cst) {
+      if (cst == 1) {
+        // each w missing exactly one member x of S uses one unit of x's budget
+        // k - ms(x) (as biplexUpperBound, biplex.cpp:583-597): at most
+        // min(count of such w, budget) of them per x
+        for (u32 i = 0; i < (u32)W; ++i) sm.c1[i * 64 + lane] = 0;
+        wave_barrier();
+        const u64* rowsO = rows + (os ? P.rows_r : P.rows_l);
+        for (u32 i = 0; i < wos; ++i) {
+          const u64 nb = rowU[i] & Cos[i] & (mvos ? ~mvos[i] : ~0ull);
+          const u32 v = i * 64 + lane;
+          if (((nb >> lane) & 1ull) && v < nos && sm.ms[os][v] == 1) {
+            const u64* rw = rowsO + (u64)v * wus;
+            for (u32 q = 0; q < wus; ++q) {
+              const u64 miss = Sus[q] & ~rw[q];
+              if (miss) { atomicAdd(&sm.c1[q * 64 + (__ffsll((long long)miss) - 1)], 1u); break; }
+            }
+          }
+        }
+        wave_barrier();
+        u64 take = 0;
+        for (u32 i = 0; i < wus; ++i) {
+          const u32 x = i * 64 + lane;
+          if (x < nus && ((Sus[i] >> lane) & 1ull)) {
+            const u32 bx = k - sm.ms[us][x];
+            take += sm.c1[x] < bx ? sm.c1[x] : bx;
+          }
+        }
+        take = wave_sum_u64(take);
+        ub += take;
+        sup -= take;  // take <= sum of the members' budgets <= sup
+        wave_barrier();
+        continue;
+      }
+      u32 cnt = 0;
+      for (u32 i = 0; i < wos; ++i) {
+        const u64 nb = rowU[i] & Cos[i] & (mvos ? ~mvos[i] : ~0ull);
+        const u32 v = i * 64 + lane;
+        cnt += __popcll(__ballot(((nb >> lane) & 1ull) && v < nos && sm.ms[os][v] == cst));
+      }
+      const u64 take = cst == 0 ? cnt : (sup / cst < cnt ? sup / cst : cnt);
+      ub += take;
+      sup -= take * cst;
+      if (cst > 0 && sup < cst) break;
+    }
+    if (ub < lb) keep = false;
+  }
+  if (out && lane == 0) out[0] = keep ? pid : INVALID_PID;
+  return keep;
 }
 
 // Expands state sidx of the batch; counters accumulate into st.
@@ -536,10 +625,10 @@ finish:
           s &= s - 1;
           const u32 u = i * 64 + b;
           write_child<W>(sm, P, a.rows, k, a.out + (u64)(base + c) * STRIDE, pid, u, os, os, prefix, NONE);
+          ++c;
           wave_barrier();
           if (lane == 0) prefix[i] |= 1ull << b;
           wave_barrier();
-          ++c;
         }
       }
       if (pivot_in_c) {

@@ -533,6 +533,7 @@ void build_tasks(std::vector<SeedJob>& jobs, const std::vector<BuildTask>& tasks
 //             while the stacks run, jobs in order with a doubling quota.
 int run_seed_jobs(gmk_ctx* c, std::vector<SeedJob>& jobs, const gmk_params* prm, gmk_stats* st, RunState& rs,
                   double* ms_per_job) {
+  if (jobs.empty()) return GMK_OK;  // (a batch whose graphs all run as global partitions)
   const bool plain = prm->mode == GMK_MODE_PLAIN;
   BBEngine eng(c, prm, rs, !plain);
   int r;

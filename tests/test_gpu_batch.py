@@ -62,3 +62,13 @@ def test_batch_with_trained_model_and_caps(eng):
     per_c, _ = eng.enumerate_batch(graphs, mode="batched", limit=cap)
     for a, b in zip(per_plain, per_c):
         assert b["count"] == min(a["count"], cap) and b["hit_cap"] == (a["count"] >= cap)
+
+
+@pytest.mark.parametrize("mode", ["plain", "batched", "learned"])
+def test_batch_of_one_global_graph(eng, mode):
+    """A batch whose only graph runs as a global partition (theta <= 2k) has no
+    seed jobs: it returns that graph's reference set."""
+    c = [c for c in GC.cases() if c["graph"] == "er40x50" and c["k"] == 1 and c["theta"] == 1][0]
+    print(f"[gpu] batch of one global graph, {mode}", flush=True)
+    per, tot = eng.enumerate_batch([(GC.graph("er40x50"), 1, 1)], mode=mode)
+    assert (per[0]["count"], str(per[0]["fp_sum"]), str(per[0]["fp_xor"])) == (c["count"], c["fp_sum"], c["fp_xor"])
