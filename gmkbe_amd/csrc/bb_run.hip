@@ -551,12 +551,13 @@ int run_seed_jobs(gmk_ctx* c, std::vector<SeedJob>& jobs, const gmk_params* prm,
   // graph, or the batch's graphs concatenated
   std::vector<u64> base(2 * (size_t)ngraphs, 0);
   std::vector<u32> coff, cadj;
-  bool ctx_graph = true;
-  for (auto& j : jobs) ctx_graph = ctx_graph && j.g.off == c->off.data() && j.graph == 0;
-  if (ctx_graph) {
+  // one resident graph (the context's, or its core): rows come from its device CSR
+  bool resident = jobs[0].d_off != nullptr;
+  for (auto& j : jobs) resident = resident && j.d_off == jobs[0].d_off && j.graph == 0;
+  if (resident) {
     if ((r = ensure(c, c->bcsr_base, 16))) return r;
-    HIPCHK(c, hipMemcpyAsync(c->bcsr_base.p, base.data(), 16, hipMemcpyHostToDevice, c->stream));
-    eng.set_row_source(c->d_off, c->d_adj, (const u64*)c->bcsr_base.p, (const u64*)c->bcsr_base.p + 1);
+    HIPCHK(c, hipMemsetAsync(c->bcsr_base.p, 0, 16, c->stream));
+    eng.set_row_source(jobs[0].d_off, jobs[0].d_adj, (const u64*)c->bcsr_base.p, (const u64*)c->bcsr_base.p + 1);
   } else {
     std::vector<char> seen(ngraphs, 0);
     for (auto& j : jobs) {

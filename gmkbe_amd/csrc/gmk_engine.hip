@@ -78,7 +78,7 @@ extern "C" void gmk_destroy(gmk_ctx* c) {
   for (DeviceVec* v : {&c->rows, &c->maps, &c->parts, &c->res, &c->counters, &c->tuples, &c->arena, &c->fams,
                         &c->prefix, &c->tcnt, &c->tdefer, &c->ridx, &c->tcount, &c->nh_off, &c->nh, &c->nh_verts, &c->dedup, &c->gcnt,
                         &c->bcsr_off, &c->bcsr_adj, &c->bcsr_base, &c->rowpids, &c->sched_p, &c->sched_hist,
-                        &c->sched_sel})
+                        &c->sched_sel, &c->core_deg, &c->core_dead, &c->core_front, &c->core_off, &c->core_adj})
     if (v->p) (void)hipFree(v->p);
   for (int l = 0; l < 7; ++l)
     for (DeviceVec* v : {&c->lane_stack[l], &c->lane_next[l], &c->lane_retry[l], &c->lane_spill[l]})
@@ -400,6 +400,9 @@ extern "C" int gmk_enumerate(gmk_ctx* c, const gmk_params* prm, gmk_stats* st) {
   st->spills = rs.spills;
   st->ms_expand = rs.ms_expand;
   st->ms_schedule = rs.ms_schedule;
+  st->ms_reduce = rs.ms_reduce;
+  st->bytes_reduce = rs.bytes_reduce;
+  st->reduce_removed = rs.reduce_removed;
   st->hit_limit = rs.hit_limit || h.res_count >= rs.limit;
   st->ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   rs.progress(st->count);

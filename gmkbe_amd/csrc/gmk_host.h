@@ -24,6 +24,10 @@ struct RunState {
   double dedup_load = 0.5;  // load factor the dedup table is kept under
   bool replay = false;      // produce every result twice (dedup tests)
   double ms_schedule = 0;   // host time spent ordering items (learned runs)
+  double ms_reduce = 0;     // core reduction before the seed partitions: device time,
+  u64 bytes_reduce = 0;     //   algorithmic bytes,
+  u64 reduce_removed = 0;   //   vertices removed and peeling rounds
+  u64 reduce_rounds = 0;
   unsigned long long* gcount = nullptr;  // per-graph caps (multi-graph batches)
   u64* gfp = nullptr;
   u64 gcap = ~0ull;
@@ -52,6 +56,17 @@ struct RunState {
 // `bound` results fit under its load factor.  Callers pass the results counted
 // so far plus the most the next launch can add.
 int dedup_reserve(gmk_ctx* c, RunState& rs, u64 bound);
+
+// The loaded graph's (thr, thr)-core, same vertex IDs (removed vertices keep
+// empty lists): host copy for the seed construction, device copy (owned by
+// the context) for the row builder.
+struct CoreGraph {
+  std::vector<u32> off, adj;
+  const u32* d_off = nullptr;
+  const u32* d_adj = nullptr;
+  u32 removed = 0;
+};
+int core_reduce(gmk_ctx* c, u32 thr, RunState& rs, CoreGraph& out);
 
 // Whole graph as one partition (local IDs = global IDs, right side shifted).
 void build_global_partition(u32 n, u32 n_left, const u32* off, const u32* adj, u32 k, u32 theta, int W, PartitionSet& ps,
@@ -126,6 +141,7 @@ struct gmk_ctx {
   bool has_item_model = false;   // cost model of GMK_MODE_LEARNED single-graph runs (per work item)
   gmk_mlp item_model{};
   DeviceVec sched_p, sched_hist, sched_sel;  // learned schedule: model parameters, score keys, selection
+  DeviceVec core_deg, core_dead, core_front, core_off, core_adj;  // core reduction of seeded runs
   float* scratch = nullptr;      // device address of the library's static array on this device (gmk_item_predict_static)
   float* h_stage = nullptr;      // pinned host staging of that array's contents
 };
@@ -175,6 +191,8 @@ namespace gmk {
 // Seed partitions of one graph, consumed in degeneracy order.
 struct SeedJob {
   HostGraph g{};
+  const u32* d_off = nullptr;  // the graph's CSR on the device when it is resident there (rows are built from it)
+  const u32* d_adj = nullptr;
   u32 k = 1, lb = 3, graph = 0;
   std::vector<u32> order, seeds;
   size_t next = 0;

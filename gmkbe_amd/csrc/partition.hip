@@ -71,7 +71,8 @@ SeedJob gmk::make_seed_job(const HostGraph& g, u32 k, u32 lb, u32 graph) {
   for (u32 v = 0; v < g.n; ++v) maxd[v >= g.n_left] = std::max(maxd[v >= g.n_left], g.deg(v));
   const int us = maxd[0] < maxd[1] ? 0 : 1;
   const u32 lo = us ? g.n_left : 0, hi = us ? g.n : g.n_left;
-  for (u32 v = lo; v < hi; ++v) j.seeds.push_back(v);
+  for (u32 v = lo; v < hi; ++v)
+    if (g.deg(v) > 0) j.seeds.push_back(v);  // a vertex without neighbours seeds no result with both sides >= 1
   std::sort(j.seeds.begin(), j.seeds.end(), [&](u32 a, u32 b) { return j.order[a] < j.order[b]; });
   return j;
 }
@@ -80,7 +81,7 @@ SeedJob gmk::make_seed_job(const HostGraph& g, u32 k, u32 lb, u32 graph) {
 // results from seed partitions, seeds taken in degeneracy order.
 int gmk_run_seeded(gmk_ctx* c, const gmk_params* prm, gmk_stats* st, RunState& rs) {
   const u32 k = prm->k, theta = prm->theta;
-  int r;
+  int r = GMK_OK;
   if (theta <= 2 * k) {
     if (2 * k > (u32)thin::MAXJ) {
       c->err = "seeded mode with theta <= 2k supports k <= 3 (use the global partition)";
@@ -92,8 +93,14 @@ int gmk_run_seeded(gmk_ctx* c, const gmk_params* prm, gmk_stats* st, RunState& r
     if (rs.hit_limit) return GMK_OK;
   }
   const u32 lb = std::max(theta, 2 * k + 1);
-  HostGraph g{c->n, c->n_left, c->off.data(), c->adj.data()};
+  // the fat results live in the (lb - k)-core (BPPivot's core reduction, applied
+  // when lb > k: biplex.cpp:441-450); it is built on the device
+  CoreGraph core;
+  if ((r = core_reduce(c, lb - k, rs, core))) return r;
+  HostGraph g{c->n, c->n_left, core.off.data(), core.adj.data()};
   std::vector<SeedJob> jobs;
   jobs.push_back(make_seed_job(g, k, lb, 0));
+  jobs[0].d_off = core.d_off;
+  jobs[0].d_adj = core.d_adj;
   return run_seed_jobs(c, jobs, prm, st, rs, nullptr);
 }

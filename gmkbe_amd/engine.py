@@ -41,7 +41,8 @@ class Stats(ctypes.Structure):
                 ("ms_expand", ctypes.c_double), ("hit_limit", ctypes.c_int32), ("spills", ctypes.c_uint32),
                 ("ms_kernel", ctypes.c_double * 4), ("launches_kernel", ctypes.c_uint64 * 4),
                 ("bytes_kernel", ctypes.c_uint64 * 4), ("dedup_undecided", ctypes.c_uint64),
-                ("ms_schedule", ctypes.c_double)]
+                ("ms_schedule", ctypes.c_double), ("ms_reduce", ctypes.c_double),
+                ("bytes_reduce", ctypes.c_uint64), ("reduce_removed", ctypes.c_uint64)]
 
     KERNELS = ("bb_expand", "thin_build_families", "thin_check", "partition_build")
 

@@ -97,6 +97,10 @@ typedef struct {
                                 fingerprint had not yet shown its check word (dedup) */
   double ms_schedule;        /* GMK_MODE_LEARNED: host time of the item ordering (scoring
                                 kernel, selection, sort), inside the window `ms` */
+  double ms_reduce;          /* seeded runs: device time of the (lb-k)-core reduction
+                                before the seed partitions */
+  uint64_t bytes_reduce;     /* its algorithmic bytes */
+  uint64_t reduce_removed;   /* vertices it removed */
 } gmk_stats;
 
 /* Create a context on HIP device `device` (-d, src/main.cu:1765-1766). */

@@ -100,3 +100,38 @@ def test_dedup_table_finds_no_repeats(eng, partition):
     _check_set(g, 1, 1, res, expect_case=c)
     assert res.stats["dedup_inserts"] == c["count"] and res.stats["dedup_dups"] == 0
     assert res.stats["dedup_probes"] >= res.stats["dedup_inserts"]
+
+
+def _core_removed(g, alpha):
+    """Vertices outside the (alpha, alpha)-core: repeatedly remove vertices of
+    degree < alpha (numpy restatement of BPPivot's coreReduction,
+    biplex.cpp:123-174)."""
+    deg = g.degrees().astype(np.int64)
+    dead = deg < alpha
+    changed = True
+    while changed:
+        live_deg = np.zeros(g.n, dtype=np.int64)
+        src = np.repeat(np.arange(g.n), g.degrees().astype(np.int64))
+        keep = ~dead[g.adj.astype(np.int64)] & ~dead[src]
+        np.add.at(live_deg, src[keep], 1)
+        new_dead = dead | (live_deg < alpha)
+        changed = bool((new_dead != dead).any())
+        dead = new_dead
+    return int(dead.sum())
+
+
+@pytest.mark.parametrize("name,k,theta", [("cl600x400", 1, 3), ("cl600x400", 2, 9), ("moreno_crime", 1, 3),
+                                          ("er200x300", 2, 5)])
+def test_core_reduction_removes_the_core_complement(eng, name, k, theta):
+    """Seeded runs peel the (lb - k)-core on the device before seeding: the
+    count of removed vertices equals a host peel's, and the set is still the
+    reference's."""
+    c = [c for c in GC.cases() if c["graph"] == name and c["k"] == k and c["theta"] == theta][0]
+    g = GC.graph(name)
+    eng.set_graph(g)
+    print(f"[gpu] core reduction {name} k={k} theta={theta}", flush=True)
+    res = eng.enumerate(k=k, theta=theta, partition="seeded", mode="batched")
+    lb = max(theta, 2 * k + 1)
+    assert res.stats["reduce_removed"] == _core_removed(g, lb - k)
+    assert res.stats["ms_reduce"] > 0
+    _check_set(g, k, theta, res, expect_case=c)
