@@ -160,6 +160,29 @@ def traffic_from_profiles(kernel: str, workload: str):
     return float(m.group(1)) if m else None
 
 
+def cpu_model() -> str:
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_parallel(g, k, limit, threads):
+    """The multi-core CPU port of the thin-result method (oracle/cpu_port.cpp)
+    on `threads` host threads, to the same result limit as the GPU run."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib
+    if not os.path.exists(oracle_lib.PORT_PATH):
+        return None
+    n, sec, ranks = oracle_lib.cpu_thin_port(g, k, limit=limit, threads=threads)
+    return {"value": n / sec if sec > 0 else 0.0, "unit": "biplexes/s", "cores": threads, "kind": "port",
+            "sample": f"{n} results (the GPU run's limit) of the thin-result method's one-vertex tuples in ID order, "
+                      f"families split over the threads in 4096-rank chunks; {sec:.2f} s, {ranks} combinations tested"}
+
+
 def cpu_baseline(g, k, theta, sample):
     """Reference BPPivot (oracle/_ref/bppivot), one core, first `sample` results."""
     if sample <= 0:
@@ -355,6 +378,17 @@ def main() -> int:
     if not args.no_cpu:
         log("[bench] CPU baseline")
         cpu = cpu_baseline(g, cfg["k"], cfg["theta"], cfg["cpu_sample"])
+        if cpu is not None:
+            nproc = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+            cpu["cpu_model"] = cpu_model()
+            cpu["nproc"] = nproc
+            if cfg["train"] and cfg["limit"]:
+                threads = min(16, nproc)  # the host share of one GPU
+                log(f"[bench] CPU port on {threads} threads")
+                par = cpu_parallel(g, cfg["k"], cfg["limit"], threads)
+                if par:
+                    cpu["parallel"] = par
+                    log(f"[bench] CPU port: {par['value']:.3e} biplexes/s")
 
     out = {
         "metric": METRIC,

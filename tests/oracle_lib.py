@@ -214,3 +214,26 @@ def thin_model(g, k, theta, side_a, min_b, hash_min=0):
     finally:
         _model.thin_model_set_hash_min(0)
     return _take(wp, nw)
+
+
+PORT_PATH = os.path.join(ROOT, "oracle", "_build", "libcpuport.so")
+_port = None
+
+
+def cpu_thin_port(g, k, limit=0, threads=0):
+    """Multi-core CPU port of the thin-result method (oracle/cpu_port.cpp):
+    (results, seconds, ranks tested) over the left side's one-vertex tuples."""
+    global _port
+    if _port is None:
+        _port = ctypes.CDLL(PORT_PATH)
+        P = ctypes.POINTER
+        u32p = P(ctypes.c_uint32)
+        _port.cpu_thin_port.restype = ctypes.c_longlong
+        _port.cpu_thin_port.argtypes = [ctypes.c_uint32, ctypes.c_uint32, u32p, u32p, ctypes.c_uint32, ctypes.c_uint64,
+                                        ctypes.c_int, P(ctypes.c_double), P(ctypes.c_uint64)]
+    off = np.ascontiguousarray(g.off, dtype=np.uint32)
+    adj = np.ascontiguousarray(g.adj, dtype=np.uint32)
+    sec = ctypes.c_double(0)
+    rk = ctypes.c_uint64(0)
+    n = _port.cpu_thin_port(g.n, g.n_left, _p(off), _p(adj), k, limit, threads, ctypes.byref(sec), ctypes.byref(rk))
+    return int(n), float(sec.value), int(rk.value)
