@@ -10,7 +10,7 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(PKG, "csrc")
 OUT_DIR = os.path.join(PKG, "_lib")
 LIB = os.path.join(OUT_DIR, "libgmk.so")
-SOURCES = ["bb_kernel.hip", "bb_run.hip", "partition_kernels.hip", "gmk_engine.hip", "partition.hip", "thin_kernels.hip", "thin_host.hip", "seeds.cpp", "mlp.hip", "schedule.hip", "reduce.hip"]
+SOURCES = ["bb_kernel.hip", "bb_run.hip", "partition_kernels.hip", "gmk_engine.hip", "partition.hip", "thin_kernels.hip", "thin_host.hip", "seeds.cpp", "mlp.hip", "schedule.hip", "reduce.hip", "edge_parse.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # -packed-fp32-ops: keep the compiler from emitting packed FP32 instructions.
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++20", "-fPIC", "-Wall", "-Wno-unused-function",
@@ -36,7 +36,8 @@ def build(verbose: bool = False, jobs: int = 0) -> str:
         for f in [ex.submit(subprocess.run, cmd, check=True) for cmd in cmds]:
             f.result()
     tmp = LIB + ".tmp"
-    subprocess.run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-pthread", *objs, "-o", tmp], check=True)
+    subprocess.run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-pthread", *objs, "-o", tmp, "-lpthread"],
+                   check=True)
     os.replace(tmp, LIB)
     # command-line driver over the C ABI
     cli = os.path.join(OUT_DIR, "gmkbpe")

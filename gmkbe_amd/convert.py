@@ -36,20 +36,61 @@ def degree_bucket(d: np.ndarray) -> np.ndarray:
     return b
 
 
-def parse_edge_text(path: str) -> tuple[np.ndarray, np.ndarray]:
-    """Read ``u v`` lines; lines starting with '%' or '#' are skipped
-    (graph_conver.cpp:529-549).  Only the first two integers of a line count."""
+def parse_edge_text(path: str, threads: int = 0) -> tuple[np.ndarray, np.ndarray]:
+    """The raw (u, v) pairs of an edge list, parsed on `threads` host threads
+    (0: all cores) by the library's native parser (gmk_parse_edge_list), with
+    graph_convert's rules (graph_conver.cpp:505-552)."""
+    import ctypes
+    from .engine import load_library
+    L = load_library()
+    up, vp = ctypes.POINTER(ctypes.c_uint32)(), ctypes.POINTER(ctypes.c_uint32)()
+    m = ctypes.c_uint64(0)
+    rc = L.gmk_parse_edge_list(path.encode(), threads, ctypes.byref(up), ctypes.byref(vp), ctypes.byref(m))
+    if rc != 0:
+        raise OSError(f"gmk_parse_edge_list({path}): status {rc}")
+    try:
+        n = int(m.value)
+        u = np.ctypeslib.as_array(up, shape=(max(n, 1),))[:n].copy() if n else np.zeros(0, dtype=np.uint32)
+        v = np.ctypeslib.as_array(vp, shape=(max(n, 1),))[:n].copy() if n else np.zeros(0, dtype=np.uint32)
+    finally:
+        L.gmk_free(up)
+        L.gmk_free(vp)
+    return u, v
+
+
+def parse_edge_text_py(path: str) -> tuple[np.ndarray, np.ndarray]:
+    """The same rules in Python, over the whole file as one range: blanks and
+    line ends skipped, '%' / '#' lines are comments, otherwise two numbers are
+    read, each skipping any non-digits before it, and the rest of the line is
+    dropped (graph_conver.cpp:433-441, 533-549)."""
+    data = open(path, "rb").read()
+    n, p = len(data), 0
     us, vs = [], []
-    with open(path, "rb") as f:
-        for line in f:
-            s = line.lstrip()
-            if not s or s[:1] in (b"%", b"#"):
-                continue
-            parts = s.split()
-            if len(parts) < 2:
-                continue
-            us.append(int(parts[0]))
-            vs.append(int(parts[1]))
+
+    def num(p):
+        while p < n and not (48 <= data[p] <= 57):
+            p += 1
+        val = 0
+        while p < n and 48 <= data[p] <= 57:
+            val = (val * 10 + data[p] - 48) & 0xFFFFFFFF
+            p += 1
+        return val, p
+
+    while p < n:
+        ch = data[p]
+        if ch in (10, 13, 32, 9):
+            p += 1
+            continue
+        if ch in (37, 35):  # '%', '#'
+            while p < n and data[p] != 10:
+                p += 1
+            continue
+        u, p = num(p)
+        v, p = num(p)
+        us.append(u)
+        vs.append(v)
+        while p < n and data[p] != 10:
+            p += 1
     return np.asarray(us, dtype=np.uint32), np.asarray(vs, dtype=np.uint32)
 
 
@@ -109,8 +150,8 @@ def convert_edges(u: np.ndarray, v: np.ndarray) -> Graph:
     return Graph(n_left, n, off.astype(np.uint32), dst.astype(np.uint32))
 
 
-def convert_file(path: str) -> Graph:
-    return convert_edges(*parse_edge_text(path))
+def convert_file(path: str, threads: int = 0) -> Graph:
+    return convert_edges(*parse_edge_text(path, threads))
 
 
 def write_edge_text(path: str, u: np.ndarray, v: np.ndarray) -> None:

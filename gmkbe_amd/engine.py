@@ -146,6 +146,10 @@ def load_library():
     L.gmk_destroy.restype = None
     L.gmk_last_error.argtypes = [ctypes.c_void_p]
     L.gmk_last_error.restype = ctypes.c_char_p
+    L.gmk_parse_edge_list.argtypes = [ctypes.c_char_p, ctypes.c_int, P(u32p), P(u32p), P(ctypes.c_uint64)]
+    L.gmk_parse_edge_list.restype = ctypes.c_int
+    L.gmk_free.argtypes = [ctypes.c_void_p]
+    L.gmk_free.restype = None
     L.gmk_device_name.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_uint32]
     L.gmk_device_name.restype = ctypes.c_int
     L.gmk_load_graph_bin.argtypes = [ctypes.c_void_p, ctypes.c_char_p, u32p, u32p, P(ctypes.c_uint64)]

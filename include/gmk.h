@@ -139,6 +139,14 @@ int gmk_fetch_results(gmk_ctx* ctx, uint32_t* words, uint64_t cap_words, uint64_
  * the log length; at most cap entries are copied. */
 int gmk_fetch_progress(gmk_ctx* ctx, uint64_t* counts, double* ms, uint32_t cap, uint32_t* n);
 
+/* Parse an edge list (`u v` per line) on `threads` host threads (0: all
+ * cores), as graph_convert does (preprocess/graph_conver.cpp:505-552, same
+ * rules and byte ranges).  *u and *v receive malloc'd arrays of *m raw IDs
+ * each (free with gmk_free); the converter (gmkbe_amd/convert.py) relabels
+ * them into the .graph.bin layout.  Host only; needs no device. */
+int gmk_parse_edge_list(const char* path, int threads, uint32_t** u, uint32_t** v, uint64_t* m);
+void gmk_free(void* p);
+
 /* ---- Many graphs in one run -------------------------------------------- */
 
 typedef struct {

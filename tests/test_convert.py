@@ -44,3 +44,58 @@ def test_roundtrip_read_write(tmp_path):
     graphio.write_graph_bin(str(p), g)
     g2 = graphio.read_graph_bin(str(p))
     assert np.array_equal(g.off, g2.off) and np.array_equal(g.adj, g2.adj) and g.n_left == g2.n_left
+
+
+MESSY = (b"% a comment\n# another\n1 2\n3\t4 99 extra\r\n  5 6\n7,8\n\n9 10 % trailing\n"
+         b"11 12\n13\n14 15\n16 17")
+
+
+def test_native_parser_follows_the_reference_rules(tmp_path):
+    """The library's threaded parser on one range gives what the rules in
+    Python give, including a line with one number (which takes the next line's
+    first number, as the reference's reader does) and a last line without a
+    newline; on well-formed input every thread count agrees."""
+    p = tmp_path / "e.txt"
+    p.write_bytes(MESSY)
+    u1, v1 = convert.parse_edge_text(str(p), threads=1)
+    u2, v2 = convert.parse_edge_text_py(str(p))
+    assert list(zip(u1.tolist(), v1.tolist())) == list(zip(u2.tolist(), v2.tolist()))
+    assert (13, 14) in list(zip(u2.tolist(), v2.tolist())) and (16, 17) in list(zip(u2.tolist(), v2.tolist()))
+    u, v = gen.chung_lu_edges(3000, 2000, 20000, 2.1, 5)
+    q = tmp_path / "w.txt"
+    convert.write_edge_text(str(q), u, v)
+    for t in (1, 3, 8, 64):
+        a, b = convert.parse_edge_text(str(q), threads=t)
+        assert np.array_equal(a, u) and np.array_equal(b, v)
+
+
+@pytest.mark.parametrize("name", sorted(GOLDEN))
+def test_converter_from_text_byte_exact(name, tmp_path):
+    """Edge text -> threaded parse -> relabel -> .graph.bin equals the reference
+    converter's file for the same edges."""
+    p = tmp_path / "e.txt"
+    convert.write_edge_text(str(p), *GOLDEN[name]())
+    g = convert.convert_file(str(p), threads=4)
+    out = tmp_path / "g.graph.bin"
+    graphio.write_graph_bin(str(out), g)
+    ref = os.path.join(GC.HERE, "graphs", name + ".graph.bin")
+    assert open(out, "rb").read() == open(ref, "rb").read()
+
+
+REF_CONVERT = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle", "_ref",
+                           "graph_convert")
+
+
+@pytest.mark.skipif(not os.path.exists(REF_CONVERT), reason="the reference converter is built only where its sources are")
+def test_messy_text_matches_the_reference_converter(tmp_path):
+    """Comments, extra columns, CRLF, tabs and non-digit separators, through
+    the reference converter binary built from its sources (oracle/ref.mk)."""
+    import subprocess
+    p = tmp_path / "m.txt"
+    p.write_bytes(b"% header\n" + b"".join(f"{a}\t{b} 7\r\n".encode() for a, b in
+                                         zip(*[x.tolist() for x in gen.er_edges(30, 40, 0.2, 3)])) + b"# end\n")
+    subprocess.run([REF_CONVERT, "-i", str(p), "-o", str(tmp_path / "ref")], check=True, capture_output=True)
+    g = convert.convert_file(str(p), threads=5)
+    out = tmp_path / "mine.graph.bin"
+    graphio.write_graph_bin(str(out), g)
+    assert open(out, "rb").read() == open(tmp_path / "ref.graph.bin", "rb").read()
