@@ -32,6 +32,8 @@ struct ThinArgs {
   const u32* ridx;      // per 2^RIDX_LOG ranks of the launch's range: the family of its first rank
   u32 nfam;
   u64 r0, r1;           // rank range of this launch
+  int unlimited;        // 1 when the launch cannot reach the result limit (no slot check)
+  u32 guard_lists;      // 1: ranks of families with guard tables may take the guard lists
   u64 rank_base;        // offset of the launch's ranks inside every family (rank slices)
   int keep;
   int swap_sides;       // 1 when side A is the right side (records list L first)
@@ -54,6 +56,10 @@ hipError_t launch_thin_build(const ThinArgs& a, hipStream_t s);
 enum : u32 { CHECK_GRID = 0, CHECK_CHUNKED = 1 };
 hipError_t launch_thin_check(const ThinArgs& a, hipStream_t s, u32 max_blocks, u32 form);
 hipError_t launch_thin_check_deferred(const ThinArgs& a, hipStream_t s, u32 blocks);
+// Guard tables of the families just built (their number read from a.fam_n on
+// the device; `blocks` workgroups of 4 waves take them in turn); runs before the
+// family prefix sums, since a dominated family gets total 0.
+hipError_t launch_thin_guard(const ThinArgs& a, hipStream_t s, u32 blocks);
 // Neighbour hashes from nv work items (vertex, first neighbour index), each
 // inserting up to NH_CHUNK neighbours of its vertex (slots already NH_EMPTY).
 constexpr u32 NH_CHUNK = 2048;

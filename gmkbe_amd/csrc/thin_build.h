@@ -81,6 +81,11 @@ GMK_HD void emit_families(const u32* L, u32 j, u32 k, u32 min_b, u32 tuple_idx, 
     f->freep = (const u32*)poolp[0]; f->nlp = nlp; f->nl_size = d; f->tuple = tuple_idx;
     for (u32 q = 0; q < nfpick; ++q) f->fpick[q] = fpick[q];
     f->nfpick = nfpick;
+    f->gmode = GUARD_NONE;  // the guard build (guard_build, thin_guard) comes later
+    f->glog = 0;
+    f->gmin = 0;
+    f->s1cap = 0;
+    f->gtab = 0;
   };
   if (nt == 0) {
     if (nfree >= min_b && nfree > 0 && dom > k) {

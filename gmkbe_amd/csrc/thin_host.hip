@@ -44,6 +44,7 @@ struct ThinRun {
   u32 probe_first = 0;
   std::vector<u64>* probe_res = nullptr;
   std::vector<u64>* probe_ticks = nullptr;
+  bool guards = true;  // guard tables for the checks (GMK_GUARDS=0: plain checks only)
 };
 
 // Checks ranks [0, total) of the current family table (its prefix sums are
@@ -58,6 +59,9 @@ int check_ranks(ThinRun& T, u64 total) {
   for (u64 q = 0, nq = T.rs->replay ? 2 : 1; q < nq * nr; ++q) {
     a.r0 = (q % nr) * RCH;
     a.r1 = std::min(total, a.r0 + RCH);
+    // a launch yields at most one result per rank: below the limit, the
+    // kernels skip the per-result slot check
+    a.unlimited = T.rs->limit == ~0ull || c->h_cnt->res_count + (a.r1 - a.r0) < T.rs->limit;
     int rr;
     if ((rr = dedup_reserve(c, *T.rs, c->h_cnt->res_count + (a.r1 - a.r0)))) return rr;
     a.dedup = T.rs->dedup;
@@ -95,6 +99,7 @@ int run_chunk(ThinRun& T, const u32* tuples, u32 count, u32 j) {
   a.j = j;
   HIPCHK(c, hipEventRecord(c->ev0, s));
   HIPCHK(c, launch_thin_build(a, s));
+  if (T.guards) HIPCHK(c, launch_thin_guard(a, s, c->n_cu * 4));
   HIPCHK(c, hipEventRecord(c->ev1, s));
   ThinCounters tc;
   HIPCHK(c, hipMemcpyAsync(&tc, c->tcnt.p, sizeof(tc), hipMemcpyDeviceToHost, s));
@@ -191,6 +196,12 @@ int thin_setup(gmk_ctx* c, const gmk_params* prm, RunState& rs, int side_a, uint
   // are built a wave each (0: always a lane each), so tests cover both forms
   a.wave1_max = WAVE1_MAX_TUPLES;
   if (const char* e = getenv("GMK_WAVE1_TUPLES")) a.wave1_max = (u32)std::min<unsigned long>(strtoul(e, nullptr, 10), ~0u);
+  // GMK_GUARDS=0 turns the families' guard tables off (every rank takes the
+  // plain candidate lists), so tests can compare the two checks
+  if (const char* e = getenv("GMK_GUARDS")) T.guards = atoi(e) != 0;
+  // GMK_GUARD_LISTS=0 keeps the guard tables for whole-family decisions only
+  a.guard_lists = 1;
+  if (const char* e = getenv("GMK_GUARD_LISTS")) a.guard_lists = atoi(e) != 0;
   a.defer = (u64*)c->tdefer.p;
   a.ridx = (const u32*)c->ridx.p;
   a.tcount = (u32*)c->tcount.p;

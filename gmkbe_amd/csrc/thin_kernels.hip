@@ -375,18 +375,23 @@ __device__ __forceinline__ const thin::Family* unrank_rank(const ThinArgs& a, u6
   u64 rr = r - a.prefix[lo] + a.rank_base;
   cost.bytes += 8ull * 8 + 16ull * f->ntypes;  // prefix probes, family header and type fields
   if (rr >= f->total) return nullptr;  // past the family's combinations: no rank
-  for (u32 t = 0; t < f->ntypes; ++t) {
-    const u64 radix = thin::binom(f->psize[t], f->cnt[t]);
-    const u64 rt = rr % radix;
-    rr /= radix;
+  const u32 nt = f->ntypes;
+  for (u32 t = 0; t < nt; ++t) {
+    const u32 cnt = f->cnt[t], ps = f->psize[t];
+    // mixed radix over the types; the last digit is what remains (rr < its radix)
+    u64 rt = rr;
+    if (t + 1 < nt) {
+      const u64 radix = thin::binom(ps, cnt);
+      rt = rr % radix;
+      rr /= radix;
+    }
     u32 idx[3] = {0, 0, 0};
-    const u32 cnt = f->cnt[t];
-    thin::unrank(rt, cnt, f->psize[t], idx);
+    thin::unrank(rt, cnt, ps, idx);
     const u32 tsz = thin::popc(f->mask[t]);
-#pragma unroll
-    for (int q = 0; q < 3; ++q) {
-      if ((u32)q >= cnt) break;
-      const u32 y = thin::pool_member(*f, t, idx[q], a.g.b_lo, cost);
+    // a run-time loop over the chosen indices keeps one copy of the pool lookup
+    for (u32 q = 0; q < cnt; ++q) {
+      const u32 iq = q == 0 ? idx[0] : (q == 1 ? idx[1] : idx[2]);
+      const u32 y = thin::pool_member(*f, t, iq, a.g.b_lo, cost);
 #pragma unroll
       for (int s2 = 0; s2 < NM; ++s2)
         if ((u32)s2 == nch) { ch[s2] = y; ctsz[s2] = tsz; }
@@ -394,6 +399,15 @@ __device__ __forceinline__ const thin::Family* unrank_rank(const ThinArgs& a, u6
     }
   }
   return f;
+}
+
+// What the checks read of family f: its guard table is used for whole-family
+// decisions (no candidate) always, and for candidate lists when the launch
+// allows them (ThinArgs.guard_lists).
+__device__ __forceinline__ thin::FamRef fam_view(const ThinArgs& a, const thin::Family& f) {
+  thin::FamRef r = thin::fam_ref(f);
+  if (!a.guard_lists && r.gmode == thin::GUARD_TABLE) r.gmode = thin::GUARD_NONE;
+  return r;
 }
 
 // Exclusive prefix sum over the wave.
@@ -410,7 +424,72 @@ __device__ __forceinline__ u64 wave_excl_scan_u64(u64 v) {
 
 // Has the result limit been reached?  The same answer in every lane.
 __device__ __forceinline__ bool limit_reached(const ThinArgs& a) {
+  if (a.unlimited) return false;
   return __ballot((__hip_atomic_load(a.flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & FLAG_LIMIT) != 0) != 0;
+}
+
+// Fingerprint of the maximal (L, R) of family f with chosen vertices ch, and
+// its dedup insert when the table is on: returns whether the result stays
+// (false: a repeat the table already holds).
+template <int K>
+__device__ __forceinline__ bool lane_fingerprint(const ThinArgs& a, const thin::Family* f, const u32 (&ch)[2 * K * K],
+                                                 u32 nch, u32 stripe, u64& fpv, thin::Cost& cost) {
+  constexpr int NM = 2 * K * K;
+  const u32 nr0 = f->nfree + nch;
+  const u32* fr0 = f->freep;
+  u64 h = 0;
+  for (u32 i = 0; i < f->j; ++i) h += mix64(f->L[i]);
+  for (u32 i = 0; i < f->nfree; ++i) h += mix64(fr0[i]);
+#pragma unroll
+  for (int i = 0; i < NM; ++i)
+    if ((u32)i < nch) h += mix64(ch[i]);
+  cost.bytes += 4ull * f->nfree;
+  const u64 nlr = a.swap_sides ? nr0 : f->j, nrr = a.swap_sides ? f->j : nr0;
+  fpv = fmix64(h + nlr * FP_A + nrr * FP_B);
+  if (!a.dedup.on) return true;
+  // the table's check word: an independent hash of the same record
+  u64 h2 = 0;
+  for (u32 i = 0; i < f->j; ++i) h2 += mix64b(f->L[i]);
+  for (u32 i = 0; i < f->nfree; ++i) h2 += mix64b(fr0[i]);
+#pragma unroll
+  for (int i = 0; i < NM; ++i)
+    if ((u32)i < nch) h2 += mix64b(ch[i]);
+  cost.bytes += 4ull * f->nfree;
+  return dedup_insert(a.dedup, fpv, fmix64(h2 + nlr * FP_B + nrr * FP_A), a.flags, stripe);
+}
+
+// Writes the record [|L|, L..., |R|, R...] (left side first) of family f with
+// chosen vertices ch at out (device or LDS memory); R merges the free list
+// with the chosen vertices, sorted in place by an odd-even transposition
+// network over the first nch slots.
+template <int K>
+__device__ __forceinline__ void write_record(u32* out, const ThinArgs& a, const thin::Family* f, u32 (&ch)[2 * K * K],
+                                             u32 nch) {
+  constexpr int NM = 2 * K * K;
+  const u32 nl = f->j, nr = f->nfree + nch;
+#pragma unroll
+  for (int pass = 0; pass < NM; ++pass)
+#pragma unroll
+    for (int q = pass & 1; q + 1 < NM; q += 2)
+      if ((u32)(q + 1) < nch && ch[q] > ch[q + 1]) { const u32 tmp = ch[q]; ch[q] = ch[q + 1]; ch[q + 1] = tmp; }
+  const u32* fr = f->freep;
+  // tuple side and generated side, left side first
+  const u32 at_t = a.swap_sides ? 1 + nr + 1 : 1;
+  const u32 at_r = a.swap_sides ? 1 : 1 + nl + 1;
+  out[a.swap_sides ? 1 + nr : 0] = nl;
+  out[a.swap_sides ? 0 : 1 + nl] = nr;
+  for (u32 i = 0; i < nl; ++i) out[at_t + i] = f->L[i];
+  u32 p = 0, q = 0;
+  for (u32 i = 0; i < nr; ++i) {
+    u32 cq = 0xFFFFFFFFu;  // ch[q], or none
+#pragma unroll
+    for (int s2 = 0; s2 < NM; ++s2)
+      if ((u32)s2 == q && q < nch) cq = ch[s2];
+    u32 v;
+    if (q >= nch || (p < f->nfree && fr[p] < cq)) v = fr[p++];
+    else { v = cq; ++q; }
+    out[at_r + i] = v;
+  }
 }
 
 // Emits the maximal (L, R) of family f with chosen vertices ch for every lane
@@ -422,33 +501,9 @@ __device__ __forceinline__ bool limit_reached(const ThinArgs& a) {
 template <int K>
 __device__ __forceinline__ bool emit_wave(const ThinArgs& a, bool want, const thin::Family* f, u32 (&ch)[2 * K * K],
                                           u32 nch, u32 stripe, thin::Cost& cost) {
-  constexpr int NM = 2 * K * K;
   const int lane = threadIdx.x & 63;
   u64 fpv = 0;
-  if (want) {
-    const u32 nr0 = f->nfree + nch;
-    const u32* fr0 = f->freep;
-    u64 h = 0;
-    for (u32 i = 0; i < f->j; ++i) h += mix64(f->L[i]);
-    for (u32 i = 0; i < f->nfree; ++i) h += mix64(fr0[i]);
-#pragma unroll
-    for (int i = 0; i < NM; ++i)
-      if ((u32)i < nch) h += mix64(ch[i]);
-    cost.bytes += 4ull * f->nfree;
-    const u64 nlr = a.swap_sides ? nr0 : f->j, nrr = a.swap_sides ? f->j : nr0;
-    fpv = fmix64(h + nlr * FP_A + nrr * FP_B);
-    if (a.dedup.on) {
-      // the table's check word: an independent hash of the same record
-      u64 h2 = 0;
-      for (u32 i = 0; i < f->j; ++i) h2 += mix64b(f->L[i]);
-      for (u32 i = 0; i < f->nfree; ++i) h2 += mix64b(fr0[i]);
-#pragma unroll
-      for (int i = 0; i < NM; ++i)
-        if ((u32)i < nch) h2 += mix64b(ch[i]);
-      cost.bytes += 4ull * f->nfree;
-      want = dedup_insert(a.dedup, fpv, fmix64(h2 + nlr * FP_B + nrr * FP_A), a.flags, stripe);
-    }
-  }
+  if (want) want = lane_fingerprint<K>(a, f, ch, nch, stripe, fpv, cost);
   const u64 mask = __ballot(want);
   if (!mask) return false;
   const int leader = __ffsll((unsigned long long)mask) - 1;
@@ -485,32 +540,105 @@ __device__ __forceinline__ bool emit_wave(const ThinArgs& a, bool want, const th
     atomicOr(a.flags, FLAG_RES_OVERFLOW);
     return true;
   }
-  // sort the chosen vertices (few) for the merge with the free list:
-  // odd-even transposition over the first nch slots
-#pragma unroll
-  for (int pass = 0; pass < NM; ++pass)
-#pragma unroll
-    for (int q = pass & 1; q + 1 < NM; q += 2)
-      if ((u32)(q + 1) < nch && ch[q] > ch[q + 1]) { const u32 tmp = ch[q]; ch[q] = ch[q + 1]; ch[q + 1] = tmp; }
-  const u32* fr = f->freep;
-  // tuple side and generated side, left side first
-  const u64 at_t = a.swap_sides ? pos + 1 + nr + 1 : pos + 1;
-  const u64 at_r = a.swap_sides ? pos + 1 : pos + 1 + nl + 1;
-  a.res[a.swap_sides ? pos + 1 + nr : pos] = nl;
-  a.res[a.swap_sides ? pos : pos + 1 + nl] = nr;
-  for (u32 i = 0; i < nl; ++i) a.res[at_t + i] = f->L[i];
-  u32 p = 0, q = 0;
-  for (u32 i = 0; i < nr; ++i) {
-    u32 cq = 0xFFFFFFFFu;  // ch[q], or none
-#pragma unroll
-    for (int s2 = 0; s2 < NM; ++s2)
-      if ((u32)s2 == q && q < nch) cq = ch[s2];
-    u32 v;
-    if (q >= nch || (p < f->nfree && fr[p] < cq)) v = fr[p++];
-    else { v = cq; ++q; }
-    a.res[at_r + i] = v;
-  }
+  write_record<K>(a.res + pos, a, f, ch, nch);
   return true;
+}
+
+// Record words a workgroup of the lane-per-rank check stages in LDS before
+// writing them out with coalesced stores (more go straight to device memory).
+constexpr u32 STAGE_WORDS = 4096;
+
+// emit_wave for a whole workgroup of 4 waves (every thread calls it; used by
+// the lane-per-rank check, whose workgroups reach it together): result slots,
+// record space, fingerprint sums and the check statistics are taken with one
+// atomic each per workgroup, and the workgroup's records, which fill one
+// contiguous range, are assembled in LDS and written with coalesced stores.
+template <int K>
+__device__ __forceinline__ bool emit_block(const ThinArgs& a, bool want, const thin::Family* f, u32 (&ch)[2 * K * K],
+                                           u32 nch, u64 n_tested, thin::Cost& cost) {
+  __shared__ u64 s_n[4], s_w[4], s_fs[4], s_fx[4], s_st[4][3];
+  __shared__ u64 s_base, s_wbase;
+  __shared__ u32 s_stage[STAGE_WORDS];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const u32 stripe = blockIdx.x;
+  u64 fpv = 0;
+  if (want) want = lane_fingerprint<K>(a, f, ch, nch, stripe, fpv, cost);
+  const u64 mask = __ballot(want);
+  if (lane == 0) s_n[wv] = (u64)__popcll(mask);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const u64 tot = s_n[0] + s_n[1] + s_n[2] + s_n[3];
+    u64 base = 0;
+    if (tot && a.unlimited) {
+      // every result of the launch fits under the limit: the count is added
+      // without waiting for the old value
+      atomicAdd((unsigned long long*)a.res_count, (unsigned long long)tot);
+    } else if (tot) {
+      base = atomicAdd((unsigned long long*)a.res_count, (unsigned long long)tot);
+      if (base + tot >= a.limit) atomicOr(a.flags, FLAG_LIMIT);
+    }
+    s_base = base;
+  }
+  __syncthreads();
+  u64 before = 0;
+#pragma unroll
+  for (int w = 0; w < 4; ++w) before += w < wv ? s_n[w] : 0ull;
+  const u64 slot = s_base + before + (u64)__popcll(mask & ((1ull << lane) - 1));
+  if (want && !a.unlimited && slot >= a.limit) want = false;
+  const u32 nl = want ? f->j : 0, nr = want ? f->nfree + nch : 0;
+  const u64 len = want && a.keep ? 2ull + nl + nr : 0;
+  cost.bytes += 4ull * len;
+  const u64 off = wave_excl_scan_u64(len);
+  const u64 wtot = wave_bcast_u64(off + len, 63);
+  const u64 fsum = wave_sum_u64(want ? fpv : 0);
+  u64 fxor = want ? fpv : 0;
+  for (int m = 32; m > 0; m >>= 1) fxor ^= shfl_xor_u64(fxor, m);
+  const u64 se = wave_sum_u64(want ? 1 : 0), stt = wave_sum_u64(n_tested), sb = wave_sum_u64(cost.bytes);
+  if (lane == 0) {
+    s_w[wv] = wtot;
+    s_fs[wv] = fsum;
+    s_fx[wv] = fxor;
+    s_st[wv][0] = stt;
+    s_st[wv][1] = se;
+    s_st[wv][2] = sb;
+  }
+  __syncthreads();
+  const u64 btot = s_w[0] + s_w[1] + s_w[2] + s_w[3];
+  if (threadIdx.x == 0) {
+    const u64 fs = s_fs[0] + s_fs[1] + s_fs[2] + s_fs[3], fx = s_fx[0] ^ s_fx[1] ^ s_fx[2] ^ s_fx[3];
+    if (fs | fx) {
+      u64* fp = a.fp + (stripe % NSTRIPE) * 2;
+      atomicAdd((unsigned long long*)&fp[0], (unsigned long long)fs);
+      atomicXor((unsigned long long*)&fp[1], (unsigned long long)fx);
+    }
+    const u64 tt = s_st[0][0] + s_st[1][0] + s_st[2][0] + s_st[3][0];
+    if (tt) {
+      u64* sp = a.stat + (stripe % NSTRIPE) * NSTAT;
+      atomicAdd((unsigned long long*)&sp[1], (unsigned long long)tt);
+      atomicAdd((unsigned long long*)&sp[2], (unsigned long long)(s_st[0][1] + s_st[1][1] + s_st[2][1] + s_st[3][1]));
+      atomicAdd((unsigned long long*)&sp[5], (unsigned long long)(s_st[0][2] + s_st[1][2] + s_st[2][2] + s_st[3][2]));
+    }
+    u64 wb = 0;
+    if (btot) {
+      wb = atomicAdd((unsigned long long*)a.res_words, (unsigned long long)btot);
+      if (wb + btot > a.res_cap) atomicOr(a.flags, FLAG_RES_OVERFLOW);
+    }
+    s_wbase = wb;
+  }
+  __syncthreads();
+  const u64 wb = s_wbase;
+  if (!btot || wb + btot > a.res_cap) return want;  // the same in every thread
+  u64 boff = off;
+#pragma unroll
+  for (int w = 0; w < 4; ++w) boff += w < wv ? s_w[w] : 0ull;
+  if (btot <= STAGE_WORDS) {
+    if (want) write_record<K>(s_stage + boff, a, f, ch, nch);
+    __syncthreads();
+    for (u32 i = threadIdx.x; i < (u32)btot; i += 256) a.res[wb + i] = s_stage[i];
+  } else if (want) {
+    write_record<K>(a.res + wb + boff, a, f, ch, nch);
+  }
+  return want;
 }
 
 // Checks rank r (one lane): unrank its combination into f and ch, test
@@ -525,8 +653,7 @@ __device__ __forceinline__ bool check_rank(const ThinArgs& a, u64 r, const thin:
   u32 ctsz[NM];
   f = unrank_rank<K>(a, r, ch, ctsz, nch, cost);
   if (!f) return false;
-  const int m = thin::side_a_maximal<NM>(a.g, f->L, f->j, f->freep, f->nfree, f->fpick, f->nfpick, ch, ctsz, nch,
-                                         a.k, cost, a.budget);
+  const int m = thin::side_a_maximal<NM>(a.g, fam_view(a, *f), ch, ctsz, nch, a.k, cost, a.budget);
   if (m < 0) {
     const u32 i = atomicAdd(a.defer_n, 1u);
     a.defer[i] = r;  // the list holds a launch's ranks
@@ -561,7 +688,8 @@ __global__ __launch_bounds__(256) void thin_check_deferred(ThinArgs a) {
     const thin::Family* f = unrank_rank<K>(a, r, ch, ctsz, nch, cost);
     if (!f) continue;  // the same rank in every lane
     thin::CandSegs cs;
-    thin::candidate_segments<NM>(a.g, f->j, f->nfree, f->fpick, f->nfpick, ch, ctsz, nch, a.k, cs, cost);
+    const thin::FamRef fr = fam_view(a, *f);
+    thin::candidate_segments<NM>(a.g, fr, ch, ctsz, nch, a.k, cs, cost);
     u64 total = 0;
 #pragma unroll
     for (int q = 0; q < 4; ++q) total += (u32)q < cs.n ? cs.len[q] : 0u;
@@ -582,8 +710,7 @@ __global__ __launch_bounds__(256) void thin_check_deferred(ThinArgs a) {
           }
         }
         cost.bytes += 4;
-        joins = thin::candidate_joins<NM>(a.g, f->L, f->j, f->freep, f->nfree, ch, ctsz, nch, a.k, cs, known,
-                                          lst[o], cost);
+        joins = thin::candidate_joins<NM>(a.g, fr, ch, ctsz, nch, a.k, cs, known, lst[o], cost);
       }
       if (__ballot(joins)) break;
     }
@@ -596,23 +723,21 @@ __global__ __launch_bounds__(256) void thin_check_deferred(ThinArgs a) {
   flush_check_stats<K>(a, cost, n_emit, n_tested);
 }
 
-// One lane per rank over the whole grid.
+// One lane per rank over the whole grid; a workgroup emits its results
+// together (emit_block), so every thread reaches that call.
 template <int K>
 __global__ __launch_bounds__(256) void thin_check(ThinArgs a) {
   constexpr int NM = 2 * K * K;
   const u64 r = a.r0 + (u64)blockIdx.x * blockDim.x + threadIdx.x;
-  u64 n_emit = 0, n_tested = 0;
+  u64 n_tested = 0;
   thin::Cost cost;
-  if (!limit_reached(a)) {
-    const u64 t0 = a.item_log ? (u64)wall_clock64() : 0;
-    const thin::Family* f = nullptr;
-    u32 ch[NM], nch = 0;
-    const bool want = check_rank<K>(a, r, f, ch, nch, cost, n_tested);
-    const bool em = emit_wave<K>(a, want, f, ch, nch, blockIdx.x * 4 + (threadIdx.x >> 6), cost);
-    if (em) ++n_emit;
-    log_item(a, f, em, t0);
-  }
-  flush_check_stats<K>(a, cost, n_emit, n_tested);
+  const thin::Family* f = nullptr;
+  u32 ch[NM], nch = 0;
+  bool want = false;
+  const u64 t0 = a.item_log ? (u64)wall_clock64() : 0;
+  if (!limit_reached(a)) want = check_rank<K>(a, r, f, ch, nch, cost, n_tested);
+  const bool em = emit_block<K>(a, want, f, ch, nch, n_tested, cost);
+  log_item(a, f, em, t0);
 }
 
 // Chunked form: a fixed grid of waves takes chunks of 64 consecutive ranks from
@@ -668,6 +793,81 @@ hipError_t launch_thin_build(const ThinArgs& a, hipStream_t s) {
       default: return hipErrorInvalidValue;  // j <= k <= 3 here
     }
   }
+  return hipGetLastError();
+}
+
+// Guard tables (thin.h): one wave per family of the launch.  The lanes split
+// the candidate lists of the plan, count each candidate's misses on the free
+// list and insert the qualifying ones by compare-and-swap into a table taken
+// from the arena; lane 0 then sets the family's guard mode (or drops a
+// dominated family by setting its total to 0).  A family whose plan is over
+// the size bounds, whose scan passes the work bound or whose table finds the
+// arena full keeps the plain check.
+struct DevGuardIns {
+  u32* t;
+  u32 lg;
+  __device__ void operator()(u32 w, u32 m) {
+    const u32 mask = (1u << lg) - 1u, word = w | (m << 30);
+    for (u32 s = thin::nh_slot(w, lg);; s = (s + 1) & mask) {
+      const u32 old = atomicCAS(&t[s], thin::GUARD_EMPTY, word);
+      if (old == thin::GUARD_EMPTY || (old & thin::GUARD_VMASK) == w) return;
+    }
+  }
+};
+
+__device__ void guard_family(const ThinArgs& a, thin::Family* f, thin::Cost& cost) {
+  const int lane = threadIdx.x & 63;
+  if (f->total == 0) return;
+  thin::GuardPlan p;
+  thin::guard_plan(a.g, *f, a.k, p, cost);
+  if (!p.ok) return;
+  const u32 lg = thin::guard_log2(p.ctot);
+  u64 at = 0;
+  if (p.ctot) {
+    if (lane == 0) {
+      DevArena ar{a.arena, a.arena_cap, a.arena_used};
+      at = ar.alloc(1ull << lg);
+    }
+    at = wave_bcast_u64(at, 0);
+  }
+  if (at == thin::SAT) return;
+  u32* t = p.ctot ? a.arena + at : nullptr;
+  for (u32 s = lane; p.ctot && s < (1u << lg); s += 64) t[s] = thin::GUARD_EMPTY;
+  __threadfence();
+  DevGuardIns ins{t, lg};
+  bool over = false;
+  const u32 gm = thin::guard_scan(a.g, *f, a.k, p, (u32)lane, 64u, ins, over, cost);
+  const u32 gmin_t = (u32)wave_min_u64((u64)gm);
+  if (__ballot(over) != 0) return;
+  if (lane == 0) {
+    bool dominated = false;
+    u32 gmin = 0;
+    const u32 mode = thin::guard_finish(a.g, f->j, f->nfree, a.k, p, gmin_t, dominated, gmin);
+    if (dominated) f->total = 0;
+    f->gmin = gmin;
+    f->s1cap = thin::guard_s1cap(f->nfree, a.k, p);
+    f->glog = lg;
+    f->gtab = (u64)t;
+    f->gmode = mode;
+  }
+}
+
+// A fixed grid of waves takes the families of the launch (their number is read
+// on the device, so the guard build follows the construction kernels with no
+// host round trip).
+__global__ __launch_bounds__(256) void thin_guard(ThinArgs a) {
+  const int lane = threadIdx.x & 63;
+  const u32 wv = blockIdx.x * 4 + (threadIdx.x >> 6), nwv = gridDim.x * 4;
+  u32 nf = __hip_atomic_load(a.fam_n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (nf > a.fam_cap) nf = a.fam_cap;
+  thin::Cost cost;
+  for (u32 i = wv; i < nf; i += nwv) guard_family(a, a.fams + i, cost);
+  const u64 b = wave_sum_u64(cost.bytes);
+  if (lane == 0 && b) atomicAdd((unsigned long long*)&a.stat[(wv % NSTRIPE) * NSTAT + 4], (unsigned long long)b);
+}
+
+hipError_t launch_thin_guard(const ThinArgs& a, hipStream_t s, u32 blocks) {
+  hipLaunchKernelGGL(thin_guard, dim3(blocks), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 

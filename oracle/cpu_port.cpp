@@ -3,8 +3,9 @@
 // leg), next to the reference BPPivot on one core.  TEST AND BENCHMARK
 // INFRASTRUCTURE ONLY: nothing in the product library links or calls it.
 //
-// It runs the same family construction and maximality test as the HIP
-// kernels (gmkbe_amd/csrc/thin.h, thin_build.h, compiled here for the host):
+// It runs the same family construction, guard tables and maximality test as
+// the HIP kernels (gmkbe_amd/csrc/thin.h, thin_build.h, compiled here for the
+// host):
 // one-vertex tuples of the left side in ID order (the heuristic order of the
 // GPU's batched mode), each tuple's families in order, and every family's
 // combinations in rank order split into chunks over the threads.  A run stops
@@ -57,6 +58,7 @@ extern "C" long long cpu_thin_port(unsigned n, unsigned n_left, const unsigned* 
     HostFams fo;
     Cost cost;
     build_families(g, L, 1, k, 1, 0, ar, fo, cost);
+    for (Family& f : fo.f) guard_build(g, f, k, ar, cost);
     for (const Family& f : fo.f) {
       if (count.load() >= lim) break;
       const u64 nchunks = (f.total + CHUNK - 1) / CHUNK;
@@ -82,7 +84,7 @@ extern "C" long long cpu_thin_port(unsigned n, unsigned n_left, const unsigned* 
             }
           }
           ++done;
-          if (side_a_maximal<18>(g, f.L, f.j, f.freep, f.nfree, f.fpick, f.nfpick, ch, ct, nch, k, cc) == 1) ++found;
+          if (side_a_maximal<18>(g, fam_ref(f), ch, ct, nch, k, cc) == 1) ++found;
         }
         tested += done;
         count += found;

@@ -51,7 +51,19 @@ struct HostHashes {
   }
 };
 u32 g_hash_min = 0;  // 0: the model uses sorted-list searches only
+bool g_guards = true;  // guard tables for the checks, as the device builds them
+unsigned long long g_modes[4] = {0, 0, 0, 0};  // families per guard mode, and dropped as dominated
 }  // namespace
+
+// Families of the thin_model calls since the last read, by guard mode
+// (GUARD_NONE, GUARD_TABLE, GUARD_ALLMAX) and dropped as dominated (index 3).
+extern "C" void thin_model_guard_modes(unsigned long long* out) {
+  for (int i = 0; i < 4; ++i) { out[i] = g_modes[i]; g_modes[i] = 0; }
+}
+
+// Later thin_model calls build the families' guard tables (1, the default) or
+// check every rank with the plain candidate lists (0).
+extern "C" void thin_model_set_guards(int on) { g_guards = on != 0; }
 
 // Later thin_model calls use neighbour hashes for vertices of degree >= v (0: none).
 extern "C" void thin_model_set_hash_min(unsigned v) { g_hash_min = v; }
@@ -104,6 +116,13 @@ extern "C" long long thin_model(unsigned n, unsigned n_left, const unsigned* off
         HostFams fo;
         Cost cost;
         build_families(g, L, j, k, min_b, 0, ar, fo, cost);
+        if (g_guards)
+          for (Family& f : fo.f) {
+            const bool live = f.total > 0;
+            guard_build(g, f, k, ar, cost);
+            if (live && f.total == 0) ++g_modes[3];
+            else if (live) ++g_modes[f.gmode];
+          }
         for (const Family& f : fo.f) {
           for (u64 r = 0; r < f.total; ++r) {
             u64 rr = r;
@@ -120,7 +139,7 @@ extern "C" long long thin_model(unsigned n, unsigned n_left, const unsigned* off
                 ++nch;
               }
             }
-            if (side_a_maximal<MAXJ * 3>(g, f.L, f.j, f.freep, f.nfree, f.fpick, f.nfpick, ch, ct, nch, k, cost) != 1)
+            if (side_a_maximal<MAXJ * 3>(g, fam_ref(f), ch, ct, nch, k, cost) != 1)
               continue;
             std::vector<u32> R(f.freep, f.freep + f.nfree);
             R.insert(R.end(), ch, ch + nch);
@@ -182,4 +201,13 @@ extern "C" void thin_scan_totals_model(const unsigned long long* totals, unsigne
     part.swap(next);
   }
   for (u32 t = 0; t < TSCAN; ++t) scan_phase_write(n, t, part.data(), total, (u64*)prefix);
+}
+
+// The device's combination arithmetic on the host: C(n, c) saturating at
+// 2^62, and the colex unrank of rank r of C(n, c) into out[0 .. c-1].
+extern "C" unsigned long long thin_model_binom(unsigned long long n, unsigned c) { return binom(n, c); }
+extern "C" void thin_model_unrank(unsigned long long r, unsigned c, unsigned long long n, unsigned* out) {
+  u32 idx[3] = {0, 0, 0};
+  unrank(r, c, n, idx);
+  for (u32 i = 0; i < c && i < 3; ++i) out[i] = idx[i];
 }

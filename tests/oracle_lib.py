@@ -168,6 +168,14 @@ def _model_init():
         u64p = P(ctypes.c_uint64)
         _model.thin_scan_totals_model.restype = None
         _model.thin_scan_totals_model.argtypes = [u64p, ctypes.c_uint32, u64p]
+        _model.thin_model_set_guards.restype = None
+        _model.thin_model_set_guards.argtypes = [ctypes.c_int]
+        _model.thin_model_guard_modes.restype = None
+        _model.thin_model_guard_modes.argtypes = [u64p]
+        _model.thin_model_binom.restype = ctypes.c_uint64
+        _model.thin_model_binom.argtypes = [ctypes.c_uint64, ctypes.c_uint32]
+        _model.thin_model_unrank.restype = None
+        _model.thin_model_unrank.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint64, u32p]
 
 
 def scan_totals_model(totals):
@@ -179,6 +187,20 @@ def scan_totals_model(totals):
     P = ctypes.POINTER(ctypes.c_uint64)
     _model.thin_scan_totals_model(t.ctypes.data_as(P), len(t), out.ctypes.data_as(P))
     return out
+
+
+def model_binom(n, c):
+    """C(n, c) as the kernels compute it (saturating at 2^62)."""
+    _model_init()
+    return int(_model.thin_model_binom(n, c))
+
+
+def model_unrank(r, c, n):
+    """The kernels' colex unrank of rank r among the c-subsets of range(n)."""
+    _model_init()
+    out = (ctypes.c_uint32 * 3)()
+    _model.thin_model_unrank(r, c, n, out)
+    return [int(out[i]) for i in range(c)]
 
 
 def hash_check(g, nh_min):
@@ -198,21 +220,30 @@ def thin_dominated(g, k, side_a):
     return int(_model.thin_dominated(g.n, g.n_left, _p(off), _p(adj), k, side_a))
 
 
-def thin_model(g, k, theta, side_a, min_b, hash_min=0):
+def thin_model(g, k, theta, side_a, min_b, hash_min=0, guards=True, modes=None):
     """CPU model of the thin-result kernels (tests/model/thin_model.cpp);
     hash_min > 0: adjacency tests through neighbour hashes of the vertices of
-    at least that degree."""
+    at least that degree; guards: the families' guard tables (as the device
+    builds them) or plain candidate lists only.  `modes` (a dict) receives the
+    families per guard mode."""
     _model_init()
     off = np.ascontiguousarray(g.off, dtype=np.uint32)
     adj = np.ascontiguousarray(g.adj, dtype=np.uint32)
     wp = ctypes.POINTER(ctypes.c_uint32)()
     nw = ctypes.c_uint64(0)
+    cnt = np.zeros(4, dtype=np.uint64)
+    _model.thin_model_guard_modes(cnt.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)))
     _model.thin_model_set_hash_min(hash_min)
+    _model.thin_model_set_guards(1 if guards else 0)
     try:
         _model.thin_model(g.n, g.n_left, _p(off), _p(adj), k, theta, side_a, min_b, ctypes.byref(wp),
                           ctypes.byref(nw))
     finally:
         _model.thin_model_set_hash_min(0)
+        _model.thin_model_set_guards(1)
+    _model.thin_model_guard_modes(cnt.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)))
+    if modes is not None:
+        modes.update(none=int(cnt[0]), table=int(cnt[1]), allmax=int(cnt[2]), dominated=int(cnt[3]))
     return _take(wp, nw)
 
 

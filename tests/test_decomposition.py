@@ -66,3 +66,43 @@ def test_thin_model_with_neighbour_hashes(c):
         for hm in (1, 3):
             assert np.array_equal(O.thin_model(g, k, th, side_a, min_b, hash_min=hm), base), (side_a, hm)
 
+
+
+@pytest.mark.parametrize("c", [c for c in GC.cases(max_count=600_000, min_k=1) if c["theta"] <= 2 * c["k"]],
+                         ids=GC.case_id)
+def test_guard_tables_keep_results(c):
+    """With the families' guard tables (candidates from the chosen vertices'
+    lists, free-list misses from the table, exact dominance), the thin results
+    are the same, in the same order, as with the plain candidate lists."""
+    g = GC.graph(c["graph"])
+    k, th = c["k"], c["theta"]
+    lb = max(th, 2 * k + 1)
+    for side_a, min_b in ((0, th), (1, lb)):
+        base = O.thin_model(g, k, th, side_a, min_b, guards=False)
+        for hm in (0, 3):
+            got = O.thin_model(g, k, th, side_a, min_b, hash_min=hm)
+            assert np.array_equal(got, base), (side_a, hm)
+
+
+@pytest.mark.parametrize("k", [1, 2, 3])
+@pytest.mark.parametrize("seed", [1, 2, 3, 4])
+def test_guard_modes_on_power_law_graphs(k, seed):
+    """Power-law graphs exercise every guard mode (tables, no candidate at all,
+    families dropped as dominated); the thin results match the plain lists',
+    and with the fat part they give the reference's set."""
+    from gmkbe_amd import gen
+    a, b, m = {1: (60, 45, 260), 2: (40, 30, 150), 3: (24, 20, 80)}[k]  # full sets of about 10^4-10^6
+    g = gen.chung_lu(a, b, m, 2.0, 9100 + seed)
+    lb = 2 * k + 1
+    modes = {}
+    got = [O.thin_model(g, k, 1, 0, 1, modes=modes)]
+    seen = dict(modes)
+    got.append(O.thin_model(g, k, 1, 1, lb, modes=modes))
+    seen = {m: seen[m] + modes[m] for m in seen}
+    assert np.array_equal(got[0], O.thin_model(g, k, 1, 0, 1, guards=False))
+    assert np.array_equal(got[1], O.thin_model(g, k, 1, 1, lb, guards=False))
+    assert seen["table"] > 0 and seen["dominated"] + seen["allmax"] > 0, seen
+    ref, _, _ = O.bppivot(g, k, 1, mode=0)
+    fat, _, _ = O.bppivot(g, k, lb, mode=1)
+    a, b = O.digest(np.concatenate([fat] + got)), O.digest(ref)
+    assert (a.count, a.canon_lo, a.canon_hi) == (b.count, b.canon_lo, b.canon_hi)
