@@ -226,8 +226,14 @@ struct Family {
 // tests only varY (the chosen vertices) per combination
 // (include/for_checkmaximality.cuh:2587-2650).
 enum : u32 { GUARD_NONE = 0, GUARD_TABLE = 1, GUARD_ALLMAX = 2 };
-constexpr u32 GUARD_CAND_MAX = 1u << 14;   // candidate list entries a family's table build may scan
-constexpr u32 GUARD_WORK_MAX = 1u << 14;   // adjacency tests per scanning lane before the build gives up
+// A family's table build is a short serial chain per lane (it adds to the
+// latency of the launch that built the family): it scans at most
+// GUARD_CAND_MAX list entries and makes at most GUARD_WORK_MAX adjacency tests
+// per lane, and is skipped for families of fewer than GUARD_MIN_RANKS
+// combinations, which gain little from it.
+constexpr u32 GUARD_CAND_MAX = 1024;
+constexpr u32 GUARD_WORK_MAX = 64;
+constexpr u64 GUARD_MIN_RANKS = 1024;
 constexpr u32 GUARD_ABSENT = 0xFFFFFFFFu;
 constexpr u32 GUARD_EMPTY = 0xFFFFFFFFu;
 constexpr u32 GUARD_VMASK = (1u << 30) - 1u;  // vertex part of a table word (vertex IDs < 2^30)
@@ -396,9 +402,13 @@ struct GuardPlan {
   u32 ok;       // 1 when the build may run (ctot <= GUARD_CAND_MAX, IDs fit the words)
 };
 
-GMK_HD void guard_plan(const Graph& g, const Family& f, u32 k, GuardPlan& p, Cost& cost) {
+GMK_HD void guard_plan(const Graph& g, const Family& f, u32 k, u64 min_ranks, GuardPlan& p, Cost& cost) {
   p.nch = 0;
   p.sat = 0;
+  p.np = 0;
+  p.ctot = 0;
+  p.ok = 0;
+  if (f.total < min_ranks) return;
   for (u32 t = 0; t < f.ntypes; ++t) {
     p.nch += f.cnt[t];
     if ((u32)popc(f.mask[t]) >= k) p.sat = 1;
@@ -504,14 +514,14 @@ GMK_HD u32 guard_s1cap(u32 nfree, u32 k, const GuardPlan& p) { return nfree >= k
 
 // Sequential guard build of one family (host model): table from the arena.
 template <class Arena>
-GMK_HD void guard_build(const Graph& g, Family& f, u32 k, Arena& ar, Cost& cost) {
+GMK_HD void guard_build(const Graph& g, Family& f, u32 k, Arena& ar, Cost& cost, u64 min_ranks = GUARD_MIN_RANKS) {
   f.gmode = GUARD_NONE;
   f.gtab = 0;
   f.glog = 0;
   f.gmin = 0;
   if (f.total == 0) return;
   GuardPlan p;
-  guard_plan(g, f, k, p, cost);
+  guard_plan(g, f, k, min_ranks, p, cost);
   if (!p.ok) return;
   const u32 lg = guard_log2(p.ctot);
   u32* t = nullptr;

@@ -34,6 +34,7 @@ struct ThinArgs {
   u64 r0, r1;           // rank range of this launch
   int unlimited;        // 1 when the launch cannot reach the result limit (no slot check)
   u32 guard_lists;      // 1: ranks of families with guard tables may take the guard lists
+  u64 guard_min_ranks;  // families with fewer combinations get no guard table
   u64 rank_base;        // offset of the launch's ranks inside every family (rank slices)
   int keep;
   int swap_sides;       // 1 when side A is the right side (records list L first)

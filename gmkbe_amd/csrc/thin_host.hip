@@ -202,6 +202,10 @@ int thin_setup(gmk_ctx* c, const gmk_params* prm, RunState& rs, int side_a, uint
   // GMK_GUARD_LISTS=0 keeps the guard tables for whole-family decisions only
   a.guard_lists = 1;
   if (const char* e = getenv("GMK_GUARD_LISTS")) a.guard_lists = atoi(e) != 0;
+  // GMK_GUARD_MIN_RANKS overrides the family size below which no table is
+  // built (0: every family), so small test graphs exercise the tables
+  a.guard_min_ranks = thin::GUARD_MIN_RANKS;
+  if (const char* e = getenv("GMK_GUARD_MIN_RANKS")) a.guard_min_ranks = strtoull(e, nullptr, 10);
   a.defer = (u64*)c->tdefer.p;
   a.ridx = (const u32*)c->ridx.p;
   a.tcount = (u32*)c->tcount.p;

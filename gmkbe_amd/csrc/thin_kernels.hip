@@ -819,7 +819,7 @@ __device__ void guard_family(const ThinArgs& a, thin::Family* f, thin::Cost& cos
   const int lane = threadIdx.x & 63;
   if (f->total == 0) return;
   thin::GuardPlan p;
-  thin::guard_plan(a.g, *f, a.k, p, cost);
+  thin::guard_plan(a.g, *f, a.k, a.guard_min_ranks, p, cost);
   if (!p.ok) return;
   const u32 lg = thin::guard_log2(p.ctot);
   u64 at = 0;

@@ -21,12 +21,16 @@ CASES = [("er12x10", 3, 1), ("er12x10", 2, 2), ("southern_women", 2, 3), ("er40x
 
 @pytest.fixture
 def guards():
-    old = os.environ.get("GMK_GUARDS")
+    """Sets GMK_GUARDS; tables are built for every family (GMK_GUARD_MIN_RANKS=0)
+    so the small graphs exercise them."""
+    old = {v: os.environ.get(v) for v in ("GMK_GUARDS", "GMK_GUARD_MIN_RANKS")}
+    os.environ["GMK_GUARD_MIN_RANKS"] = "0"
     yield lambda v: os.environ.__setitem__("GMK_GUARDS", str(v))
-    if old is None:
-        os.environ.pop("GMK_GUARDS", None)
-    else:
-        os.environ["GMK_GUARDS"] = old
+    for v, x in old.items():
+        if x is None:
+            os.environ.pop(v, None)
+        else:
+            os.environ[v] = x
 
 
 @pytest.mark.parametrize("on", [0, 1], ids=["plain", "guards"])
