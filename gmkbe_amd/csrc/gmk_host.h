@@ -92,9 +92,14 @@ using gmk::NSTRIPE;
 
 // Device counters of one run, read back after every batch.
 struct RunCounters {
-  u32 flags, pad;
-  u64 res_words, res_count;
-  u32 lane[8][4];         // per bitset width 2^l: children, requeued parents, persistent work counter, unused
+  // the run flags, the record-space counter and the result counter sit on
+  // cache lines of their own: every emitting workgroup takes record space and
+  // result slots with one atomic each, and every wave reads the flags
+  alignas(128) u32 flags;
+  u32 pad;
+  alignas(128) u64 res_words;
+  alignas(128) u64 res_count;
+  alignas(128) u32 lane[8][4];         // per bitset width 2^l: children, requeued parents, persistent work counter, unused
   u64 fp[NSTRIPE * 2];
   u64 stat[NSTRIPE * 8];  // 0 states, 1 intersections, 2 emitted leaves, 3-6 bytes per kernel
   u64 dstat[NSTRIPE * 4]; // dedup: inserts, probes, duplicates

@@ -26,6 +26,7 @@ constexpr u32 TCH = 1u << 16;          // tuples per family-construction launch
 constexpr u64 ARENA = 1ull << 27;      // arena words (512 MiB)
 constexpr u32 FCAP = 1u << 18;         // families per launch
 constexpr u64 RCH = 1ull << 22;        // ranks per check launch
+constexpr u64 SHRINK_MIN = 1ull << 16;  // remaining results below which a launch takes the slot check
 constexpr u32 DEFER_BUDGET = 64;       // candidates one lane tests before a rank goes to a whole wave
 constexpr u32 LEARNED_FRONT = 4096;    // about twice this many one-vertex items are ordered by the learned schedule (the rest follow in ID order)
 constexpr u64 LEARNED_SLICE = 1u << 16; // ranks of each family per slice in learned runs
@@ -45,6 +46,8 @@ struct ThinRun {
   std::vector<u64>* probe_res = nullptr;
   std::vector<u64>* probe_ticks = nullptr;
   bool guards = true;  // guard tables for the checks (GMK_GUARDS=0: plain checks only)
+  u64 count0 = 0;          // result count when the run's thin part started
+  u64 ranks_launched = 0;  // combinations launched since then
 };
 
 // Checks ranks [0, total) of the current family table (its prefix sums are
@@ -54,14 +57,28 @@ int check_ranks(ThinRun& T, u64 total) {
   hipStream_t s = c->stream;
   ThinArgs& a = T.a;
   RunCounters* dc = (RunCounters*)c->counters.p;
-  const u64 nr = (total + RCH - 1) / RCH;
+  const u64 limit = T.rs->limit;
   // GMK_DEDUP_REPLAY: every rank range a second time (its results repeat)
-  for (u64 q = 0, nq = T.rs->replay ? 2 : 1; q < nq * nr; ++q) {
-    a.r0 = (q % nr) * RCH;
-    a.r1 = std::min(total, a.r0 + RCH);
-    // a launch yields at most one result per rank: below the limit, the
-    // kernels skip the per-result slot check
-    a.unlimited = T.rs->limit == ~0ull || c->h_cnt->res_count + (a.r1 - a.r0) < T.rs->limit;
+  const u64 passes = T.rs->replay ? 2 : 1;
+  for (u64 pass = 0, r0 = 0; pass < passes; r0 = 0, ++pass)
+  for (; r0 < total; r0 = a.r1) {
+    // A launch yields at most one result per rank, so one of at most
+    // limit - count ranks cannot pass the limit and skips the per-result slot
+    // check (unlimited).  Near the limit, while the run's launches have been
+    // yielding at least half a result per rank, launches shrink to what
+    // remains (each then covers at least half of it); below SHRINK_MIN
+    // remaining results, or at a lower yield, a launch takes the slot check.
+    const u64 cnt = c->h_cnt->res_count;
+    u64 n = std::min(RCH, total - r0);
+    if (limit != ~0ull) {
+      const u64 rem = limit > cnt ? limit - cnt : 0;
+      const bool high_yield = T.ranks_launched && 2 * (cnt - T.count0) >= T.ranks_launched;
+      if (rem < n && rem >= SHRINK_MIN && high_yield) n = rem;
+    }
+    a.r0 = r0;
+    a.r1 = r0 + n;
+    a.unlimited = limit == ~0ull || cnt + n <= limit;
+    T.ranks_launched += n;
     int rr;
     if ((rr = dedup_reserve(c, *T.rs, c->h_cnt->res_count + (a.r1 - a.r0)))) return rr;
     a.dedup = T.rs->dedup;
@@ -85,6 +102,16 @@ int check_ranks(ThinRun& T, u64 total) {
                  std::min<u64>(c->h_cnt->res_count, T.rs->limit), 0);
     if (c->h_cnt->flags & FLAG_RES_OVERFLOW) { c->err = "result buffer overflow (raise result_capacity_words)"; return GMK_EOVERFLOW; }
     if (c->h_cnt->flags & FLAG_LIMIT) { T.stop = true; T.rs->hit_limit = true; return GMK_OK; }
+    if (c->h_cnt->res_count >= limit) {
+      // an unlimited launch ended exactly at the limit: raise the flag the
+      // kernels of later phases read
+      c->h_cnt->flags |= FLAG_LIMIT;
+      HIPCHK(c, hipMemcpyAsync(&dc->flags, &c->h_cnt->flags, sizeof(u32), hipMemcpyHostToDevice, s));
+      HIPCHK(c, hipStreamSynchronize(s));
+      T.stop = true;
+      T.rs->hit_limit = true;
+      return GMK_OK;
+    }
   }
   return GMK_OK;
 }
@@ -167,6 +194,7 @@ int thin_setup(gmk_ctx* c, const gmk_params* prm, RunState& rs, int side_a, uint
   if ((r = ensure(c, c->ridx, ((RCH >> RIDX_LOG) + 2) * 4))) return r;
   if ((r = ensure(c, c->tcount, (size_t)TCH * 8 * 4))) return r;
   T.c = c; T.prm = prm; T.rs = &rs;
+  T.count0 = c->h_cnt->res_count;
   ThinArgs& a = T.a;
   a.g.off = c->d_off;
   a.g.adj = c->d_adj;

@@ -74,3 +74,30 @@ def test_asymmetric_csr_is_refused():
         print("[gpu] asymmetric CSR", flush=True)
         with pytest.raises(GmkError, match="invalid argument"):
             e.set_graph(Graph(g.n_left, g.n, off2, adj2))
+
+
+def test_limits_around_the_full_count():
+    """Limits just below, at and above the full result count of a seeded run
+    (launches shrink to the results still allowed, and one may end exactly at
+    the limit): exactly min(limit, N) valid, distinct results, the full set
+    when the limit is not reached."""
+    import numpy as np
+    import golden_cases as GC
+    c = [c for c in GC.cases() if c["graph"] == "er200x300" and c["k"] == 1 and c["theta"] == 1][0]
+    g = GC.graph("er200x300")
+    with Engine(0) as e:
+        e.set_graph(g)
+        full = e.enumerate(k=1, theta=1, mode="batched", partition="seeded")
+        n = full.count
+        assert GC.digest_matches(c, O.digest(full.words))
+        for lim in (n - 70_000, n - 1, n, n + 1):
+            print(f"[gpu] er200x300 k=1 seeded limit {lim} (full {n})", flush=True)
+            res = e.enumerate(k=1, theta=1, limit=lim, mode="batched", partition="seeded")
+            assert res.count == min(lim, n), (lim, res.count)
+            assert res.stats["hit_limit"] == (lim <= n)
+            f = O.fingerprints(res.words)
+            assert np.unique(f).size == res.count
+            if lim >= n:
+                assert GC.digest_matches(c, O.digest(res.words))
+            else:
+                assert O.validate(g, 1, 1, O.sample_records(res.words, 5000, seed=lim)) == 0
