@@ -18,17 +18,31 @@
 // adjacency rows, which is what the wave does in parallel: lane i owns vertices
 // i, i+64, ...  Ties in every argmax/argmin go to the smallest local ID.
 //
+// Widths W up to 128 words (8192 vertices per side) keep the wave's working
+// set (sets, per-vertex counters) in LDS; wider partitions, up to 1024 words
+// (65536 per side), keep it in an HBM slice of their own per wave
+// (bb_expand_wide), with the same code.
+//
 // Children of a state are written to a contiguous block claimed with one
 // atomicAdd; when the block would cross the buffer's end, the claimed slots
 // inside the buffer are marked as holes (skipped when read) and the parent is
 // written to the retry list unchanged, to be expanded again in a later batch.
 #include "gmk_device.h"
 
+#include <type_traits>
+
 namespace gmk {
 
 struct Stats {
   u64 states, intersections, bytes, leaves;
 };
+
+// Per-vertex degree counters: 16 bits up to 512 words (32768 vertices per side).
+template <int W>
+using deg_t = typename std::conditional<(W > 512), u32, uint16_t>::type;
+
+// Widest partition whose working set lives in LDS.
+constexpr int W_LDS_MAX = 128;
 
 template <int W>
 struct __align__(16) Smem {
@@ -40,9 +54,9 @@ struct __align__(16) Smem {
   u64 sat[W];             // S members saturated by the include being built
   u64 andsat[W];          // AND of the rows of sat
   u64 prefix[W];           // pivot candidates already branched on
-  uint16_t deg[2][64 * W];
+  deg_t<W> deg[2][64 * W];
   uint8_t ms[2][64 * W];
-  uint8_t widx[W];         // words of the opposite side that the current pass reads
+  uint16_t widx[W];        // words of the opposite side that the current pass reads
   u64 rm[2][W];            // candidates removed by the degree pruning of this state
   u32 c1[64 * W];          // child bound: neighbours of the include missing only this S member
 };
@@ -52,13 +66,26 @@ struct __align__(16) Smem {
 template <int W>
 __device__ __forceinline__ u32 word_list(Smem<W>& sm, const u64* ma, const u64* mb, u32 wb) {
   const int lane = threadIdx.x & 63;
-  const bool nz = (u32)lane < wb && ((ma[lane] | (mb ? mb[lane] : 0)) != 0);
-  const u64 bal = __ballot(nz);
-  if (nz) sm.widx[__popcll(bal & ((1ull << lane) - 1ull))] = (uint8_t)lane;
+  u32 n = 0;
+  for (u32 base = 0; base < wb; base += 64) {
+    const u32 w = base + lane;
+    const bool nz = w < wb && ((ma[w] | (mb ? mb[w] : 0)) != 0);
+    const u64 bal = __ballot(nz);
+    if (nz) sm.widx[n + __popcll(bal & ((1ull << lane) - 1ull))] = (uint16_t)w;
+    n += (u32)__popcll(bal);
+  }
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-  return (u32)__popcll(bal);
+  return n;
+}
+
+// End of a counter pass: LDS working sets need only the wave barrier; the
+// HBM ones of wide partitions also wait for the wave's stores.
+template <int W>
+__device__ __forceinline__ void pass_barrier() {
+  if constexpr (W > W_LDS_MAX) wave_barrier();
+  else __builtin_amdgcn_wave_barrier();
 }
 
 __device__ __forceinline__ u64* setp(u64* base, int side, int which, int W) {
@@ -96,12 +123,12 @@ __device__ void side_stats(Smem<W>& sm, const PartDesc& P, const u64* rows, int 
       }
       ++tests;
     }
-    sm.deg[a][v] = (uint16_t)d;
+    sm.deg[a][v] = (deg_t<W>)d;
     sm.ms[a][v] = (uint8_t)(s > 255 ? 255 : s);
   }
   st.intersections += tests;
   st.bytes += (u64)tests * nw * 8;
-  __builtin_amdgcn_wave_barrier();  // widx is rewritten by the next pass
+  pass_barrier<W>();  // widx is rewritten by the next pass
 }
 
 // deg of the members of side a after the vertices rm_b left C_b:
@@ -131,13 +158,13 @@ __device__ void side_deg_sub(Smem<W>& sm, const PartDesc& P, const u64* rows, in
         const u32 w = sm.widx[q];
         d += __popcll(row[w] & Rb[w]);
       }
-      sm.deg[a][v] = (uint16_t)(sm.deg[a][v] - d);
+      sm.deg[a][v] = (deg_t<W>)(sm.deg[a][v] - d);
       ++tests;
     }
   }
   st.intersections += tests;
   st.bytes += (u64)tests * nw * 8;
-  __builtin_amdgcn_wave_barrier();
+  pass_barrier<W>();
 }
 
 template <int W>
@@ -243,15 +270,15 @@ __device__ bool write_child(Smem<W>& sm, const PartDesc& P, const u64* rows, u32
     rowU = rows + (us ? P.rows_r : P.rows_l) + (u64)inc * (us ? P.wl : P.wr);
     ufull = sm.ms[us][inc] >= k;
     // saturated S members of the other side: ms == k-1 and not adjacent to inc
-    if (lane < W) {
+    for (u32 w = lane; w < (u32)W; w += 64) {
       u64 s = 0;
-      if ((u32)lane < wos && k > 0) s = setp(sm.set, os, 0, W)[lane] & ~rowU[lane] & sm.k1[os][lane];
-      sm.sat[lane] = s;
+      if (w < wos && k > 0) s = setp(sm.set, os, 0, W)[w] & ~rowU[w] & sm.k1[os][w];
+      sm.sat[w] = s;
     }
     wave_barrier();
     // AND of the saturated members' rows (over side us)
     const u32 wus = us ? P.wr : P.wl;
-    if (lane < W) {
+    for (u32 w = lane; w < (u32)W; w += 64) {
       u64 acc = ~0ull;
       for (u32 i = 0; i < wos; ++i) {
         u64 s = sm.sat[i];
@@ -259,10 +286,10 @@ __device__ bool write_child(Smem<W>& sm, const PartDesc& P, const u64* rows, u32
           const u32 b = __ffsll((long long)s) - 1;
           s &= s - 1;
           const u32 v = i * 64 + b;
-          if ((u32)lane < wus) acc &= rows[(os ? P.rows_r : P.rows_l) + (u64)v * wus + lane];
+          if (w < wus) acc &= rows[(os ? P.rows_r : P.rows_l) + (u64)v * wus + w];
         }
       }
-      sm.andsat[lane] = acc;
+      sm.andsat[w] = acc;
     }
     wave_barrier();
   }
@@ -386,7 +413,7 @@ template <int W>
 __device__ void expand_one(Smem<W>& sm, const ExpandArgs& a, u32 sidx, Stats& st) {
   const int lane = threadIdx.x & 63;
   constexpr int STRIDE = 1 + 6 * W;
-  const u64* in = a.in + (u64)sidx * STRIDE;
+  const u64* in = a.in + ((a.ctl ? a.ctl->in_off : 0ull) + sidx) * STRIDE;
   const u32 pid = (u32)in[0];
   if (pid == INVALID_PID) return;  // hole left by a child block that did not fit
   const PartDesc P = a.parts[pid];
@@ -419,7 +446,7 @@ __device__ void expand_one(Smem<W>& sm, const ExpandArgs& a, u32 sidx, Stats& st
   // degrees before any removal; removal only lowers degrees, so every removed
   // vertex is one the sequential pass would also be allowed to remove.
   bool pruned_any = false;
-  if (lane < W) { sm.rm[0][lane] = 0; sm.rm[1][lane] = 0; }
+  for (int w = lane; w < W; w += 64) { sm.rm[0][w] = 0; sm.rm[1][w] = 0; }
   wave_barrier();
   for (int a_ = 0; a_ < 2; ++a_) {
     const u32 na = a_ ? P.nr : P.nl, wa = a_ ? P.wr : P.wl;
@@ -578,7 +605,7 @@ __device__ void expand_one(Smem<W>& sm, const ExpandArgs& a, u32 sidx, Stats& st
         const u32 w_os = os ? P.wr : P.wl;
         const u64* rowP = a.rows + (pside ? P.rows_r : P.rows_l) + (u64)pivot * w_os;
         const u64* Cos = setp(sm.set, os, 1, W);
-        if (lane < W) sm.cand[lane] = (u32)lane < w_os ? (Cos[lane] & ~rowP[lane]) : 0;
+        for (u32 w = lane; w < (u32)W; w += 64) sm.cand[w] = w < w_os ? (Cos[w] & ~rowP[w]) : 0;
         wave_barrier();
         kind = 2;
         nchild = popc_set<W>(sm.cand, w_os) + (pivot_in_c ? 1 : 0);
@@ -592,10 +619,11 @@ finish:
     u32 base = 0;
     if (lane == 0) base = atomicAdd(a.out_n, nchild);
     base = (u32)wave_bcast_u64(base, 0);
-    if ((u64)base + nchild > a.out_cap) {
+    const u32 out_cap = a.ctl ? a.ctl->out_cap : a.out_cap;
+    if ((u64)base + nchild > out_cap) {
       // no room: mark the claimed slots below the cap as holes and requeue the
       // parent unchanged
-      for (u32 s2 = base + lane; s2 < a.out_cap && s2 < base + nchild; s2 += 64)
+      for (u32 s2 = base + lane; s2 < out_cap && s2 < base + nchild; s2 += 64)
         a.out[(u64)s2 * STRIDE] = INVALID_PID;
       // no room: requeue the parent unchanged
       u32 r = 0;
@@ -615,7 +643,7 @@ finish:
       const u32 w_os = os ? P.wr : P.wl;
       // running prefix of candidates already branched on (moved to X)
       u64* prefix = sm.prefix;
-      if (lane < W) prefix[lane] = 0;
+      for (int w = lane; w < W; w += 64) prefix[w] = 0;
       wave_barrier();
       u32 c = 0;
       for (u32 i = 0; i < w_os; ++i) {
@@ -656,7 +684,7 @@ __device__ void flush_stats(const ExpandArgs& a, const Stats& st, u32 stripe) {
 template <int W>
 __global__ __launch_bounds__(64) void bb_expand(ExpandArgs a) {
   __shared__ Smem<W> sm;
-  if (blockIdx.x >= a.n_in) return;
+  if (blockIdx.x >= (a.ctl ? a.ctl->n_in : a.n_in)) return;
   if (__hip_atomic_load(a.flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & FLAG_LIMIT) return;
   Stats st = {0, 0, 0, 0};
   expand_one<W>(sm, a, blockIdx.x, st);
@@ -671,26 +699,143 @@ __global__ __launch_bounds__(64) void bb_expand_persistent(ExpandArgs a) {
   __shared__ Smem<W> sm;
   const int lane = threadIdx.x & 63;
   Stats st = {0, 0, 0, 0};
+  const u32 n_in = a.ctl ? a.ctl->n_in : a.n_in;
   for (;;) {
     u32 idx = 0;
     if (lane == 0) idx = atomicAdd(a.work_next, 1u);
     idx = (u32)wave_bcast_u64(idx, 0);
-    if (idx >= a.n_in) break;
+    if (idx >= n_in) break;
     if (__hip_atomic_load(a.flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & FLAG_LIMIT) break;
     expand_one<W>(sm, a, idx, st);
   }
   flush_stats<W>(a, st, blockIdx.x);
 }
 
+// Wide partitions (W > W_LDS_MAX): the persistent form with each wave's
+// working set in its own slice of a.scratch (HBM) instead of LDS.
+template <int W>
+__global__ __launch_bounds__(64) void bb_expand_wide(ExpandArgs a) {
+  Smem<W>& sm = ((Smem<W>*)a.scratch)[blockIdx.x];
+  const int lane = threadIdx.x & 63;
+  Stats st = {0, 0, 0, 0};
+  const u32 n_in = a.ctl ? a.ctl->n_in : a.n_in;
+  for (;;) {
+    u32 idx = 0;
+    if (lane == 0) idx = atomicAdd(a.work_next, 1u);
+    idx = (u32)wave_bcast_u64(idx, 0);
+    if (idx >= n_in) break;
+    if (__hip_atomic_load(a.flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & FLAG_LIMIT) break;
+    expand_one<W>(sm, a, idx, st);
+  }
+  flush_stats<W>(a, st, blockIdx.x);
+}
+
+// Device-planned steps.  bb_plan runs one lane per width.  It first
+// completes the width's last batch, whose retry list and children bb_commit
+// has already copied back over the batch: the new top, the smoothed fan-out,
+// and the check that some parent's children fitted.  With `plan` it then sets
+// up the next batch as the host loop would (a batch of at most bmax states,
+// fewer when the free room at the last fan-out is short; children may take
+// half the stack) and clears the width's batch counters; a width whose stack
+// is too full to expand safely halts (the host spills it), as does every width
+// once the run hit its limit or overflowed its record buffer.
+__global__ __launch_bounds__(64) void bb_plan(LaneCtl* ctl, u32* lane_cnt, const u32* flags, u32 active, int plan) {
+  const u32 l = threadIdx.x;
+  if (l >= 32 || !((active >> l) & 1u)) return;
+  LaneCtl c = ctl[l];
+  u32* cnt = lane_cnt + 4 * l;
+  if (c.n_in) {
+    const u64 nout = cnt[0] < c.out_cap ? cnt[0] : c.out_cap, nret = cnt[1];
+    if (c.n_in > nret) c.fanout = 0.5f * c.fanout + 0.5f * (float)cnt[0] / (float)(c.n_in - nret);
+    c.top = c.in_off + nret + nout;
+    c.steps += 1;
+    c.states += c.n_in;
+    if (nret == c.n_in && cnt[0] >= c.out_cap) c.halt = 2;
+    c.n_in = 0;
+  }
+  if (plan) {
+    cnt[0] = cnt[1] = cnt[2] = 0;
+    const u64 W = 1ull << l;
+    const u64 room = c.cap - c.top;
+    const u64 need = c.cap / 4 > 2 * (2 * 64 * W + 2) ? c.cap / 4 : 2 * (2 * 64 * W + 2);
+    const u32 fl = __hip_atomic_load(flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (!c.halt && c.top > 0 && !(fl & (FLAG_LIMIT | FLAG_RES_OVERFLOW | FLAG_TABLE_FULL))) {
+      if (c.top > 1 && room < need) {
+        c.halt = 1;
+      } else {
+        const float f = c.fanout > 1.f ? c.fanout : 1.f;
+        const u64 by_room = (u64)((double)room / (2.0 * (double)f));
+        u64 B = c.top < c.bmax ? c.top : c.bmax;
+        B = B < by_room ? B : by_room;
+        B = B > 1 ? B : 1;
+        const u64 oc = c.cap / 2 < room ? c.cap / 2 : room;
+        if (oc == 0) {
+          c.halt = 2;
+        } else {
+          c.n_in = (u32)B;
+          c.in_off = c.top - B;
+          c.out_cap = (u32)oc;
+        }
+      }
+    }
+  }
+  ctl[l] = c;
+}
+
+// Copies a width's retry list and then its children (at most out_cap) over
+// its batch on the stack, so the search stays depth-first across batches.
+__global__ __launch_bounds__(256) void bb_commit(const LaneCtl* ctl, const u32* cnt, u64* stack, const u64* next,
+                                                 const u64* retry, u32 stride) {
+  const u32 n_in = ctl->n_in;
+  if (n_in == 0) return;
+  const u64 nret = cnt[1], nout = cnt[0] < ctl->out_cap ? cnt[0] : ctl->out_cap;
+  const u64 wr = nret * stride, total = (nret + nout) * stride;
+  u64* dst = stack + ctl->in_off * stride;
+  for (u64 w = (u64)blockIdx.x * blockDim.x + threadIdx.x; w < total; w += (u64)gridDim.x * blockDim.x)
+    dst[w] = w < wr ? retry[w] : next[w - wr];
+}
 
 }  // namespace gmk
 
 namespace gmk {
+hipError_t launch_bb_plan(LaneCtl* ctl, u32* lane_cnt, const u32* flags, u32 active, int plan, hipStream_t s) {
+  hipLaunchKernelGGL(bb_plan, dim3(1), dim3(64), 0, s, ctl, lane_cnt, flags, active, plan);
+  return hipGetLastError();
+}
+
+hipError_t launch_bb_commit(const LaneCtl* ctl, const u32* cnt, u64* stack, const u64* next, const u64* retry,
+                            u32 stride, hipStream_t s) {
+  hipLaunchKernelGGL(bb_commit, dim3(512), dim3(256), 0, s, ctl, cnt, stack, next, retry, stride);
+  return hipGetLastError();
+}
+
+size_t bb_wide_scratch_bytes(int W) {
+  switch (W) {
+    case 256: return sizeof(Smem<256>);
+    case 512: return sizeof(Smem<512>);
+    case 1024: return sizeof(Smem<1024>);
+    default: return 0;
+  }
+}
+
 // Host-side launcher for the expansion kernel: one wave per state, or, with
-// persistent_waves > 0, that many persistent waves draining the batch.
+// persistent_waves > 0, that many persistent waves draining the batch.  Widths
+// above W_LDS_MAX always run persistent, with persistent_waves slices of
+// a.scratch (bb_wide_scratch_bytes(W) each).
 hipError_t launch_bb_expand(int W, u32 n_states, hipStream_t s, const ExpandArgs& a, u32 persistent_waves) {
   if (n_states == 0) return hipSuccess;
   dim3 block(64);
+  if (W > W_LDS_MAX) {
+    if (!persistent_waves || !a.scratch) return hipErrorInvalidValue;
+    dim3 grid(n_states < persistent_waves ? n_states : persistent_waves);
+    switch (W) {
+      case 256: hipLaunchKernelGGL(bb_expand_wide<256>, grid, block, 0, s, a); break;
+      case 512: hipLaunchKernelGGL(bb_expand_wide<512>, grid, block, 0, s, a); break;
+      case 1024: hipLaunchKernelGGL(bb_expand_wide<1024>, grid, block, 0, s, a); break;
+      default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+  }
   if (persistent_waves) {
     dim3 grid(n_states < persistent_waves ? n_states : persistent_waves);
     switch (W) {
@@ -701,6 +846,7 @@ hipError_t launch_bb_expand(int W, u32 n_states, hipStream_t s, const ExpandArgs
       case 16: hipLaunchKernelGGL(bb_expand_persistent<16>, grid, block, 0, s, a); break;
       case 32: hipLaunchKernelGGL(bb_expand_persistent<32>, grid, block, 0, s, a); break;
       case 64: hipLaunchKernelGGL(bb_expand_persistent<64>, grid, block, 0, s, a); break;
+      case 128: hipLaunchKernelGGL(bb_expand_persistent<128>, grid, block, 0, s, a); break;
       default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -714,6 +860,7 @@ hipError_t launch_bb_expand(int W, u32 n_states, hipStream_t s, const ExpandArgs
     case 16: hipLaunchKernelGGL(bb_expand<16>, grid, block, 0, s, a); break;
     case 32: hipLaunchKernelGGL(bb_expand<32>, grid, block, 0, s, a); break;
     case 64: hipLaunchKernelGGL(bb_expand<64>, grid, block, 0, s, a); break;
+    case 128: hipLaunchKernelGGL(bb_expand<128>, grid, block, 0, s, a); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

@@ -26,6 +26,10 @@ using namespace gmk;
 
 namespace gmk {
 hipError_t launch_bb_expand(int W, u32 n_states, hipStream_t s, const ExpandArgs& a, u32 persistent_waves);
+hipError_t launch_bb_plan(LaneCtl* ctl, u32* lane_cnt, const u32* flags, u32 active, int plan, hipStream_t s);
+hipError_t launch_bb_commit(const LaneCtl* ctl, const u32* cnt, u64* stack, const u64* next, const u64* retry,
+                            u32 stride, hipStream_t s);
+size_t bb_wide_scratch_bytes(int W);
 struct RowBuildArgs {
   const PartDesc* parts;
   const u32* pids;
@@ -37,20 +41,42 @@ struct RowBuildArgs {
   const u64* adj_base;
   u64* rows;
   u64* bytes;
+  u32 max_words;        // widest row of the partitions built
 };
 hipError_t launch_build_part_rows(const RowBuildArgs& a, hipStream_t s);
 
 namespace {
-constexpr int NLANE = 7;  // widths 1, 2, 4, 8, 16, 32, 64 words
 int lane_of(int W) {
   int l = 0;
   while ((1 << l) < W) ++l;
   return l;
 }
+// States per launch of widths without a tuning entry (above 64 words)
+constexpr u32 kWideBatch = 1024;
+// Persistent waves per CU of widths above 128 words (each holds an HBM working set)
+constexpr u32 kWideWavesPerCU = 2;
 }  // namespace
 
+u32 BBEngine::batch_max(int l) const {
+  return l < LANE_TUNED ? std::min<u32>(kBatchMax, c_->tune.batch_states[l]) : kWideBatch;
+}
+
 BBEngine::BBEngine(gmk_ctx* c, const gmk_params* prm, RunState& rs, bool persistent)
-    : c_(c), prm_(prm), rs_(rs), persistent_(persistent) {}
+    : c_(c), prm_(prm), rs_(rs), persistent_(persistent) {
+  // GMK_STEPS_PER_SYNC: batches per width between two host synchronisations
+  // (1 to 64); GMK_GRAPHS=0: launch them directly instead of from a hipGraph
+  if (const char* e = getenv("GMK_STEPS_PER_SYNC")) steps_ = std::clamp(atoi(e), 1, 64);
+  if (const char* e = getenv("GMK_GRAPHS")) graphs_ = atoi(e) != 0;
+}
+
+// Tables too small for one partition on their own grow, when no state
+// references them.
+int BBEngine::make_room(const BuiltPart& b) {
+  if (fits(b)) return GMK_OK;
+  if (live() != 0 || rows_used_ != 0) { c_->err = "partition tables full"; return GMK_EOVERFLOW; }
+  return reserve(std::max(rows_cap_, (size_t)b.row_words()), std::max(maps_cap_, b.maps.size()),
+                 std::max<size_t>(parts_cap_, 1));
+}
 
 int BBEngine::reserve(size_t rows_words, size_t maps_words, size_t nparts) {
   int r;
@@ -85,6 +111,7 @@ void BBEngine::add(BuiltPart& b) {
   if (b.device_rows) {
     // the row range is reserved here and filled on the device before any state runs
     rowbuild_.push_back(pid);
+    rowbuild_words_ = std::max({rowbuild_words_, P.wl, P.wr});
   } else {
     // the stage covers [rows_dev_, rows_used_): zeros over device-built ranges
     // (the build kernel runs after the copy)
@@ -196,10 +223,12 @@ int BBEngine::lane_alloc(int l) {
   const int W = 1 << l;
   const u64 state_bytes = (1 + 6ull * W) * 8;
   // 4 GiB of states for widths up to 4 words, 8 GiB up to 16, 16 GiB above
-  // (HBM3E holds 288 GB: wide searches branch on up to 4096 candidates); the
+  // (HBM3E holds 288 GB: wide searches branch on up to 64 W candidates); the
   // children buffer is half of it
   const u64 bytes = W <= 4 ? (4ull << 30) : W <= 16 ? (8ull << 30) : (16ull << 30);
-  u64 cap = std::max<u64>(1 << 17, bytes / state_bytes);
+  // at least 2^17 states, and eight times the most children one state can
+  // have (64 W + 1), so a spill always leaves room for the top state's children
+  u64 cap = std::max<u64>({1ull << 17, bytes / state_bytes, 8 * (64ull * W + 1)});
   // GMK_STACK_STATES caps every stack (at least 4096 states, and four times
   // the most children one state can have): small stacks exercise the spill
   // path on small graphs
@@ -210,7 +239,7 @@ int BBEngine::lane_alloc(int l) {
   int r;
   if ((r = ensure(c_, c_->lane_stack[l], cap * state_bytes))) return r;
   if ((r = ensure(c_, c_->lane_next[l], cap / 2 * state_bytes))) return r;
-  if ((r = ensure(c_, c_->lane_retry[l], (u64)kBatchMax * state_bytes))) return r;
+  if ((r = ensure(c_, c_->lane_retry[l], (u64)batch_max(l) * state_bytes))) return r;
   cap_[l] = cap;
   return GMK_OK;
 }
@@ -242,7 +271,7 @@ int BBEngine::upload() {
     RunCounters* dc = (RunCounters*)c_->counters.p;
     RowBuildArgs ra{(const PartDesc*)c_->parts.p, (const u32*)c_->rowpids.p, (u32)rowbuild_.size(),
                     (const u32*)c_->maps.p, src_off_, src_adj_, src_off_base_, src_adj_base_, (u64*)c_->rows.p,
-                    &dc->stat[3 + 3]};
+                    &dc->stat[3 + 3], rowbuild_words_};
     HIPCHK(c_, hipEventRecord(c_->ev2, s));
     HIPCHK(c_, launch_build_part_rows(ra, s));
     HIPCHK(c_, hipEventRecord(c_->ev3, s));
@@ -272,17 +301,53 @@ int BBEngine::upload() {
 // Host staging buffers are released after the stream has synchronised.
 void BBEngine::clear_stage() {
   rowbuild_.clear();
+  rowbuild_words_ = 0;
   rows_stage_.clear();
   maps_stage_.clear();
   parts_stage_.clear();
   for (int l = 0; l < NLANE; ++l) { roots_stage_[l].clear(); rev_stage_[l].clear(); }
 }
 
+// One synchronisation of the search: the host first spills nearly full
+// stacks and brings spilled states back into empty ones, then the device runs
+// steps_ batches per non-empty width, each planned on the device from the
+// last one's outcome (bb_plan), expanded (bb_expand) and copied back onto the
+// stack (bb_commit), with no host round trip in between.  The sequence is
+// captured once into a hipGraph and replayed while its launch parameters stay
+// the same (GMK_GRAPHS=0 launches the kernels directly).  Counters are
+// cleared by bb_plan, so the graph holds no memset node.
 int BBEngine::step() {
   int r;
   if ((r = upload())) return r;
   hipStream_t s = c_->stream;
   RunCounters* dc = (RunCounters*)c_->counters.p;
+  u32 active = 0;
+  for (int l = 0; l < NLANE; ++l) {
+    if (!top_[l] && spill_n_[l] && (r = unspill(l))) return r;
+    // Spill when less than a quarter of the stack (and less than twice the most
+    // children one state can have) is free: the stack is then at least half
+    // free, which the stack's minimum size keeps above the most children of
+    // one state, so every batch expands at least its top state.
+    if (top_[l] > 1 && cap_[l] - top_[l] < std::max<u64>(cap_[l] / 4, 2 * (2ull * 64 * (1u << l) + 2)) &&
+        (r = spill(l)))
+      return r;
+    if (top_[l]) active |= 1u << l;
+  }
+  if (!active) return GMK_OK;
+  const int S = steps_;
+  // each state emits at most one result
+  if ((r = dedup_reserve(c_, rs_, c_->h_cnt->res_count + (u64)S * NLANE * kBatchMax))) return r;
+  if ((r = ensure(c_, c_->lane_ctl, sizeof(LaneCtl) * NLANE))) return r;
+  LaneCtl* dctl = (LaneCtl*)c_->lane_ctl.p;
+  LaneCtl* hctl = c_->h_ctl;
+  for (int l = 0; l < NLANE; ++l) {
+    hctl[l] = LaneCtl{};
+    hctl[l].top = top_[l];
+    hctl[l].cap = cap_[l];
+    hctl[l].bmax = batch_max(l);
+    hctl[l].fanout = (float)fanout_[l];
+  }
+  HIPCHK(c_, hipMemcpyAsync(dctl, hctl, sizeof(LaneCtl) * NLANE, hipMemcpyHostToDevice, s));
   ExpandArgs a{};
   a.parts = (const PartDesc*)c_->parts.p;
   a.rows = (const u64*)c_->rows.p;
@@ -297,56 +362,41 @@ int BBEngine::step() {
   a.fp = dc->fp;
   a.stat = dc->stat;
   a.flags = &dc->flags;
-  // each state emits at most one result
-  if ((r = dedup_reserve(c_, rs_, c_->h_cnt->res_count + (u64)NLANE * kBatchMax))) return r;
   a.dedup = rs_.dedup;
   a.gcount = rs_.gcount;
   a.gfp = rs_.gfp;
   a.gcap = rs_.gcap;
-  u32 B[NLANE] = {0};
-  u32 out_cap[NLANE] = {0};
-  HIPCHK(c_, hipMemsetAsync(dc->lane, 0, sizeof(dc->lane), s));
-  HIPCHK(c_, hipEventRecord(c_->ev0, s));
-  int launched = 0;
+  Launch L[NLANE];
+  int nl = 0;
   for (int l = 0; l < NLANE; ++l) {
-    if (!top_[l] && spill_n_[l] && (r = unspill(l))) return r;
-    // Spill when less than a quarter of the stack (and less than twice the most
-    // children one state can have) is free: the stack is then at least half
-    // free, which the override's minimum keeps above the most children of one
-    // state, so every step expands at least its top state.
-    if (top_[l] > 1 && cap_[l] - top_[l] < std::max<u64>(cap_[l] / 4, 2 * (2ull * 64 * (1u << l) + 2)) &&
-        (r = spill(l)))
-      return r;
-    if (!top_[l]) continue;
+    if (!((active >> l) & 1u)) continue;
+    Launch& q = L[nl++];
+    q.l = l;
+    q.a = a;
+    q.a.in = (const u64*)c_->lane_stack[l].p;
+    q.a.out = (u64*)c_->lane_next[l].p;
+    q.a.retry = (u64*)c_->lane_retry[l].p;
+    q.a.out_n = &dc->lane[l][0];
+    q.a.retry_n = &dc->lane[l][1];
+    q.a.work_next = &dc->lane[l][2];
+    q.a.ctl = dctl + l;
     const int W = 1 << l;
-    const u64 stride = 1 + 6ull * W;
-    const u64 room = cap_[l] - top_[l];
-    // Fan-out of the last batch (children per parent): the next batch takes no
-    // more parents than the free stack can hold at that fan-out, so a search
-    // that branches widely drifts toward depth-first.
-    const u64 by_room = (u64)((double)room / (2.0 * std::max(fanout_[l], 1.0)));
-    const u64 bmax = std::min<u64>(kBatchMax, c_->tune.batch_states[l]);
-    B[l] = (u32)std::max<u64>(1, std::min<u64>({top_[l], bmax, by_room}));
-    out_cap[l] = (u32)std::min<u64>(cap_[l] / 2, room);
-    // A parent whose children do not fit is requeued; with the stack nearly full
-    // only the top (deepest) states run, and the step fails below when none of
-    // them fits.
-    if (out_cap[l] == 0) { c_->err = "state stack full"; return GMK_EOVERFLOW; }
-    u64* stack = (u64*)c_->lane_stack[l].p;
-    a.in = stack + (top_[l] - B[l]) * stride;
-    a.n_in = B[l];
-    a.out = (u64*)c_->lane_next[l].p;
-    a.out_cap = out_cap[l];
-    a.retry = (u64*)c_->lane_retry[l].p;
-    a.out_n = &dc->lane[l][0];
-    a.retry_n = &dc->lane[l][1];
-    a.work_next = &dc->lane[l][2];
-    const u32 waves = persistent_ ? c_->n_cu * c_->tune.waves_per_cu[l] : 0u;
-    HIPCHK(c_, launch_bb_expand(W, B[l], s, a, waves));
-    ++launched;
+    if (l < LANE_TUNED) {
+      q.waves = persistent_ ? c_->n_cu * c_->tune.waves_per_cu[l] : 0u;
+    } else if (l > LANE_LDS_MAX) {
+      // every wave of a wide width works in its own HBM slice
+      q.waves = c_->n_cu * kWideWavesPerCU;
+      if ((r = ensure(c_, c_->wide_scratch, (size_t)q.waves * bb_wide_scratch_bytes(W)))) return r;
+      q.a.scratch = c_->wide_scratch.p;
+    }
+    // a workgroup per state of the largest batch, or the persistent grid
+    q.grid = q.waves ? q.waves : batch_max(l);
   }
+  HIPCHK(c_, hipEventRecord(c_->ev0, s));
+  if ((r = launch_sequence(L, nl, active, S))) return r;
   HIPCHK(c_, hipEventRecord(c_->ev1, s));
   HIPCHK(c_, hipMemcpyAsync(c_->h_cnt, dc, offsetof(RunCounters, fp), hipMemcpyDeviceToHost, s));
+  HIPCHK(c_, hipMemcpyAsync(hctl, dctl, sizeof(LaneCtl) * NLANE, hipMemcpyDeviceToHost, s));
   HIPCHK(c_, hipStreamSynchronize(s));
   clear_stage();
   float ms = 0;
@@ -360,32 +410,20 @@ int BBEngine::step() {
   HIPCHK(c_, hipEventElapsedTime(&ms, c_->ev0, c_->ev1));
   rs_.ms_expand += ms;
   rs_.ms_kernel[0] += ms;
-  rs_.launches += launched;
-  rs_.launches_kernel[0] += launched;
+  rs_.launches += (u64)S * nl;
+  rs_.launches_kernel[0] += (u64)S * nl;
   const RunCounters& h = *c_->h_cnt;
   rs_.progress(std::min<u64>(h.res_count, rs_.limit));
   if (h.flags & FLAG_RES_OVERFLOW) {
     c_->err = "result buffer overflow (raise result_capacity_words)";
     return GMK_EOVERFLOW;
   }
-  for (int l = 0; l < NLANE; ++l) {
-    if (!B[l]) continue;
-    const u64 stride = 1 + 6ull * (1u << l);
-    const u64 state_bytes = stride * 8;
-    const u64 nout = std::min<u64>(h.lane[l][0], out_cap[l]);
-    const u64 nret = h.lane[l][1];
-    if (B[l] > nret) fanout_[l] = 0.5 * fanout_[l] + 0.5 * (double)h.lane[l][0] / (double)(B[l] - nret);
-    u64* stack = (u64*)c_->lane_stack[l].p;
-    u64 dst = top_[l] - B[l];
-    if (nret)
-      HIPCHK(c_, hipMemcpyAsync(stack + dst * stride, c_->lane_retry[l].p, nret * state_bytes, hipMemcpyDeviceToDevice,
-                                s));
-    dst += nret;
-    if (nout)
-      HIPCHK(c_, hipMemcpyAsync(stack + dst * stride, c_->lane_next[l].p, nout * state_bytes, hipMemcpyDeviceToDevice,
-                                s));
-    top_[l] = dst + nout;
-    if (nret == B[l] && h.lane[l][0] >= out_cap[l]) {
+  for (int i = 0; i < nl; ++i) {
+    const int l = L[i].l;
+    top_[l] = hctl[l].top;
+    fanout_[l] = hctl[l].fanout;
+    rs_.batches += hctl[l].steps;
+    if (hctl[l].halt == 2) {
       // every parent was requeued: the free stack is smaller than the children
       // of each state that ran
       c_->err = "state stack full (no parent's children fit)";
@@ -395,6 +433,73 @@ int BBEngine::step() {
   if (h.flags & FLAG_LIMIT) rs_.hit_limit = true;
   rs_.report("branch-and-bound", std::min<u64>(h.res_count, rs_.limit), live());
   return GMK_OK;
+}
+
+// The kernels of `steps` batches of the widths in L, from a hipGraph when
+// graphs are on (re-captured when any launch parameter changed).
+int BBEngine::launch_sequence(const Launch* L, int nl, u32 active, int steps) {
+  hipStream_t s = c_->stream;
+  RunCounters* dc = (RunCounters*)c_->counters.p;
+  LaneCtl* dctl = (LaneCtl*)c_->lane_ctl.p;
+  auto enqueue = [&]() -> hipError_t {
+    hipError_t e = launch_bb_plan(dctl, &dc->lane[0][0], &dc->flags, active, 1, s);
+    for (int i = 0; i < steps && e == hipSuccess; ++i) {
+      for (int q = 0; q < nl && e == hipSuccess; ++q)
+        e = launch_bb_expand(1 << L[q].l, L[q].grid, s, L[q].a, L[q].waves);
+      for (int q = 0; q < nl && e == hipSuccess; ++q) {
+        const int l = L[q].l;
+        e = launch_bb_commit(dctl + l, &dc->lane[l][0], (u64*)c_->lane_stack[l].p, (const u64*)c_->lane_next[l].p,
+                             (const u64*)c_->lane_retry[l].p, (u32)(1 + 6 * (1u << l)), s);
+      }
+      if (e == hipSuccess) e = launch_bb_plan(dctl, &dc->lane[0][0], &dc->flags, active, i + 1 < steps, s);
+    }
+    return e;
+  };
+  if (!graphs_) {
+    HIPCHK(c_, enqueue());
+    return GMK_OK;
+  }
+  // the graph's key: every launch parameter
+  std::vector<char> key;
+  auto put = [&](const void* p, size_t n) { key.insert(key.end(), (const char*)p, (const char*)p + n); };
+  put(&active, sizeof(active));
+  put(&steps, sizeof(steps));
+  put(&dctl, sizeof(dctl));
+  put(&dc, sizeof(dc));
+  for (int q = 0; q < nl; ++q) {
+    put(&L[q].a, sizeof(ExpandArgs));
+    put(&L[q].grid, sizeof(u32));
+    put(&L[q].waves, sizeof(u32));
+    const void* bufs[3] = {c_->lane_stack[L[q].l].p, c_->lane_next[L[q].l].p, c_->lane_retry[L[q].l].p};
+    put(bufs, sizeof(bufs));
+  }
+  if (!graph_exec_ || key != graph_key_) {
+    if (graph_exec_) { (void)hipGraphExecDestroy(graph_exec_); graph_exec_ = nullptr; }
+    hipGraph_t g = nullptr;
+    HIPCHK(c_, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    const hipError_t e = enqueue();
+    const hipError_t e2 = hipStreamEndCapture(s, &g);
+    if (e != hipSuccess || e2 != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      c_->err = std::string("graph capture: ") + hipGetErrorString(e != hipSuccess ? e : e2);
+      return GMK_EDEVICE;
+    }
+    const hipError_t e3 = hipGraphInstantiate(&graph_exec_, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e3 != hipSuccess) {
+      graph_exec_ = nullptr;
+      c_->err = std::string("graph instantiate: ") + hipGetErrorString(e3);
+      return GMK_EDEVICE;
+    }
+    graph_key_ = std::move(key);
+    ++rs_.graphs;
+  }
+  HIPCHK(c_, hipGraphLaunch(graph_exec_, s));
+  return GMK_OK;
+}
+
+BBEngine::~BBEngine() {
+  if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
 }
 
 int BBEngine::drain() {
@@ -413,7 +518,7 @@ static void build_host_rows(const HostGraph& g, const SeedPartition& sp, std::ve
 
 // Maps and root state of a seed partition (and its rows unless the device
 // builds them), with offsets relative to the part's own vectors.  Returns
-// false when a side exceeds 4096 vertices.
+// false when a side exceeds 65536 vertices (1024 words).
 static bool make_built(const SeedJob& job, const SeedPartition& sp, std::vector<u32>& loc, BuiltPart& b,
                        bool device_rows) {
   const HostGraph& g = job.g;
@@ -485,7 +590,7 @@ struct BuildTask {
 };
 struct BuildOut {
   bool ok = false;       // a partition was built
-  bool too_wide = false; // a side exceeds 4096 vertices
+  bool too_wide = false; // a side exceeds 65536 vertices
   BuiltPart part;
 };
 struct Worker {
@@ -584,7 +689,7 @@ int run_seed_jobs(gmk_ctx* c, std::vector<SeedJob>& jobs, const gmk_params* prm,
   std::vector<BuildTask> tasks;
   std::vector<BuildOut> outs;
   auto too_wide = [&](u32 seed) {
-    c->err = "seed partition of vertex " + std::to_string(seed) + " has a side above 4096 vertices";
+    c->err = "seed partition of vertex " + std::to_string(seed) + " has a side above 65536 vertices";
     return GMK_EUNSUPPORTED;
   };
 
@@ -598,6 +703,8 @@ int run_seed_jobs(gmk_ctx* c, std::vector<SeedJob>& jobs, const gmk_params* prm,
         if (outs[0].too_wide) return too_wide(tasks[0].seed);
         if (!outs[0].ok) continue;
         ++st->partitions;
+        st->widest_side = std::max<u64>({st->widest_side, outs[0].part.P.nl, outs[0].part.P.nr});
+        if ((r = eng.make_room(outs[0].part))) return r;
         eng.add(outs[0].part);
         if ((r = eng.drain())) return r;
         eng.reset_tables();
@@ -642,13 +749,17 @@ int run_seed_jobs(gmk_ctx* c, std::vector<SeedJob>& jobs, const gmk_params* prm,
         if (o.too_wide) return too_wide(tasks[i].seed);
         if (!o.ok) continue;
         ++st->partitions;
+        st->widest_side = std::max<u64>({st->widest_side, o.part.P.nl, o.part.P.nr});
         if (carry.empty() && eng.fits(o.part)) eng.add(o.part);
         else carry.push_back(std::move(o.part));
       }
     }
     if (eng.live() == 0) {
       eng.reset_tables();  // nothing references the tables any more
-      if (!carry.empty()) continue;
+      if (!carry.empty()) {
+        if ((r = eng.make_room(carry.front()))) return r;
+        continue;
+      }
       bool more = false;
       for (size_t ji = cur; ji < jobs.size() && !more; ++ji) more = !done(jobs[ji]);
       if (!more) break;
@@ -661,7 +772,7 @@ int run_seed_jobs(gmk_ctx* c, std::vector<SeedJob>& jobs, const gmk_params* prm,
 
 // Whole partition set, all roots at once.
 int gmk_run_bb(gmk_ctx* c, const PartitionSet& ps, const gmk_params* prm, gmk_stats* st, RunState& rs) {
-  (void)st;
+  for (const PartDesc& P : ps.parts) st->widest_side = std::max<u64>({st->widest_side, P.nl, P.nr});
   BBEngine eng(c, prm, rs, prm->mode != GMK_MODE_PLAIN);
   int r;
   if ((r = eng.reserve(ps.rows.size(), ps.maps.size(), ps.parts.size()))) return r;

@@ -106,8 +106,24 @@ __device__ __forceinline__ bool dedup_insert(const DedupTable& t, u64 fp, u64 fp
   return state != 2;
 }
 
+// Batch control of one state stack (one bitset width) in device memory, for
+// steps planned on the device (bb_plan): the planned batch is the top n_in
+// states, [in_off, in_off + n_in), and its children may take out_cap slots.
+struct LaneCtl {
+  u64 top;        // states on the stack
+  u64 cap;        // its capacity in states
+  u64 in_off;     // first state of the planned batch
+  u32 n_in;       // states in the planned batch (0: the width is idle)
+  u32 out_cap;    // children the batch may write
+  u32 bmax;       // states per batch at most
+  u32 halt;       // 0 running, 1 the stack wants a spill (host), 2 no parent's children fit
+  float fanout;   // children per parent of the last batch, smoothed
+  u32 steps;      // batches run since the host last read this record
+  u64 states;     // parents taken by those batches
+};
+
 struct ExpandArgs {
-  const u64* in;        // input states (stride 1 + 6W words)
+  const u64* in;        // input states (stride 1 + 6W words); with ctl: the stack, batch at ctl->in_off
   u32 n_in;
   u32* work_next;       // persistent form: next state of the batch to take
   u64* out;             // children
@@ -133,6 +149,8 @@ struct ExpandArgs {
   unsigned long long* gcount;  // results per graph
   u64* gfp;                    // [graph][2] fingerprint sum / xor
   u64 gcap;                    // results kept per graph
+  void* scratch;               // wide partitions: one working-set slice per persistent wave
+  const LaneCtl* ctl;          // device-planned steps: batch size, position and child room (null: n_in, out_cap)
 };
 
 __device__ __forceinline__ void wave_barrier() {

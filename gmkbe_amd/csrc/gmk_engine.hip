@@ -30,6 +30,7 @@ extern "C" int gmk_create(int device, gmk_ctx** out) {
   }
   if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
       hipHostMalloc((void**)&c->h_cnt, sizeof(RunCounters), hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void**)&c->h_ctl, sizeof(LaneCtl) * NLANE, hipHostMallocDefault) != hipSuccess ||
       hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
       hipEventCreate(&c->ev2) != hipSuccess || hipEventCreate(&c->ev3) != hipSuccess) {
     delete c;
@@ -78,14 +79,16 @@ extern "C" void gmk_destroy(gmk_ctx* c) {
   for (DeviceVec* v : {&c->rows, &c->maps, &c->parts, &c->res, &c->counters, &c->tuples, &c->arena, &c->fams,
                         &c->prefix, &c->tcnt, &c->tdefer, &c->ridx, &c->tcount, &c->nh_off, &c->nh, &c->nh_verts, &c->dedup, &c->gcnt,
                         &c->bcsr_off, &c->bcsr_adj, &c->bcsr_base, &c->rowpids, &c->sched_p, &c->sched_hist,
-                        &c->sched_sel, &c->core_deg, &c->core_dead, &c->core_front, &c->core_off, &c->core_adj})
+                        &c->sched_sel, &c->core_deg, &c->core_dead, &c->core_front, &c->core_off, &c->core_adj,
+                        &c->wide_scratch, &c->lane_ctl})
     if (v->p) (void)hipFree(v->p);
-  for (int l = 0; l < 7; ++l)
+  for (int l = 0; l < NLANE; ++l)
     for (DeviceVec* v : {&c->lane_stack[l], &c->lane_next[l], &c->lane_retry[l], &c->lane_spill[l]})
       if (v->p) (void)hipFree(v->p);
   if (c->d_off) (void)hipFree(c->d_off);
   if (c->d_adj) (void)hipFree(c->d_adj);
   if (c->h_cnt) (void)hipHostFree(c->h_cnt);
+  if (c->h_ctl) (void)hipHostFree(c->h_ctl);
   if (c->h_stage) (void)hipHostFree(c->h_stage);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
@@ -301,7 +304,7 @@ int gmk::dedup_reserve(gmk_ctx* c, RunState& rs, u64 bound) {
 // Branch-and-bound over a set of partitions.
 // ---------------------------------------------------------------------------
 int gmk_pick_w(u32 words) {
-  for (int w : {1, 2, 4, 8, 16, 32, 64})
+  for (int w : {1, 2, 4, 8, 16, 32, 64, 128, 256, 512, 1024})
     if ((u32)w >= words) return w;
   return 0;
 }
@@ -448,7 +451,7 @@ extern "C" int gmk_enumerate_batch(gmk_ctx* c, const gmk_graph_desc* graphs, uin
     W[i] = gmk_pick_w(std::max((g.n_left + 63) / 64, (nr + 63) / 64));
     // graphs with theta <= 2k run as one global partition, bounded by the widest state
     if (!W[i] && g.theta <= 2 * g.k) {
-      c->err = "graph " + std::to_string(i) + " has a side above 4096 vertices and theta <= 2k";
+      c->err = "graph " + std::to_string(i) + " has a side above 65536 vertices and theta <= 2k";
       return GMK_EUNSUPPORTED;
     }
   }

@@ -36,6 +36,8 @@ struct RunState {
   u64 res_cap = 0;
   double ms_expand = 0;
   u64 launches = 0;
+  u64 batches = 0;          // branch-and-bound batches planned on the device
+  u64 graphs = 0;           // hipGraphs captured for them
   bool hit_limit = false;
   u64 fp_sum_extra = 0, fp_xor_extra = 0;
   u32 spills = 0;
@@ -90,6 +92,13 @@ using gmk::NSTRIPE;
     }                                                                                       \
   } while (0)
 
+// State stacks per bitset width 2^l words, l = 0 .. NLANE - 1 (1 .. 1024
+// words: partitions of up to 65536 vertices per side).  Widths up to
+// LANE_LDS_MAX keep a state's working set in LDS, wider ones in HBM.
+constexpr int NLANE = 11;
+constexpr int LANE_TUNED = 7;    // widths 1 .. 64: launch forms and batch sizes from the tuning table
+constexpr int LANE_LDS_MAX = 7;  // width 128: one workgroup per state with a 100 KB LDS working set
+
 // Device counters of one run, read back after every batch.
 struct RunCounters {
   // the run flags, the record-space counter and the result counter sit on
@@ -99,7 +108,7 @@ struct RunCounters {
   u32 pad;
   alignas(128) u64 res_words;
   alignas(128) u64 res_count;
-  alignas(128) u32 lane[8][4];         // per bitset width 2^l: children, requeued parents, persistent work counter, unused
+  alignas(128) u32 lane[NLANE + 1][4]; // per bitset width 2^l: children, requeued parents, persistent work counter, unused
   u64 fp[NSTRIPE * 2];
   u64 stat[NSTRIPE * 8];  // 0 states, 1 intersections, 2 emitted leaves, 3-6 bytes per kernel
   u64 dstat[NSTRIPE * 4]; // dedup: inserts, probes, duplicates
@@ -124,8 +133,11 @@ struct gmk_ctx {
   u32* d_off = nullptr;
   u32* d_adj = nullptr;
   // per-run buffers, grown on demand and reused
-  DeviceVec lane_stack[7], lane_next[7], lane_retry[7];  // state stacks per bitset width
-  DeviceVec lane_spill[7];  // shallow states moved out of a nearly full stack
+  DeviceVec lane_stack[NLANE], lane_next[NLANE], lane_retry[NLANE];  // state stacks per bitset width
+  DeviceVec lane_spill[NLANE];  // shallow states moved out of a nearly full stack
+  DeviceVec wide_scratch;       // working sets of the persistent waves of widths above 128 words
+  DeviceVec lane_ctl;           // LaneCtl per width (device-planned steps)
+  gmk::LaneCtl* h_ctl = nullptr;  // pinned host copy
   DeviceVec rows, maps, parts, res, counters;
   DeviceVec tuples, arena, fams, prefix, tcnt, tdefer, ridx, tcount;  // thin-result kernels
   DeviceVec nh_off, nh, nh_verts;  // neighbour hashes of the vertices of degree >= nh_min
@@ -176,17 +188,16 @@ void mlp_pack(const gmk_mlp& m, float* P);
 // The device address of the static scratch array of gmk_item_predict_static on
 // the current device, and the array's size in floats.
 int static_scratch(float** ptr, size_t* floats);
-// Learned schedule of single-graph runs: about the 2 `want` one-vertex work
+// Learned schedule of single-graph runs: about 1.5 `want` one-vertex work
 // items of side A ([a_lo, a_hi)) with the highest predicted results per unit
-// of device time under the item cost model, unsorted, as (score key << 32 |
-// ~(item - a_lo)) entries (a sampled threshold decides which are gathered).
+// of device time under the item cost model, sorted best first (ties: lower ID
+// first), as (score key << 32 | ~(item - a_lo)) entries (a sampled threshold
+// decides which are gathered; the device sorts them).
 int learned_front(gmk_ctx* c, u32 a_lo, u32 a_hi, u32 k, u32 want, std::vector<u64>& front);
-// Hands out gathered items best first (ties: lower ID first), sorting only as
-// far as they are taken.
+// Hands out the front's items in order.
 struct LearnedOrder {
-  static constexpr size_t STEP = 256;
   std::vector<u64> cand;
-  size_t pos = 0, sorted = 0;
+  size_t pos = 0;
   bool done() const { return pos >= cand.size(); }
   u32 next(u32 a_lo);
 };
@@ -222,7 +233,11 @@ struct BuiltPart {
 class BBEngine {
  public:
   BBEngine(gmk_ctx* c, const gmk_params* prm, RunState& rs, bool persistent);
+  ~BBEngine();
+  BBEngine(const BBEngine&) = delete;
+  BBEngine& operator=(const BBEngine&) = delete;
   int reserve(size_t rows_words, size_t maps_words, size_t nparts);
+  int make_room(const BuiltPart& b);  // grows empty tables that cannot hold b
   bool fits(const BuiltPart& b) const;
   void add(BuiltPart& b);
   int add_set(const PartitionSet& ps);
@@ -236,6 +251,17 @@ class BBEngine {
 
  private:
   static constexpr u32 kBatchMax = 1u << 16;  // largest states per launch (retry buffer size)
+  u32 batch_max(int l) const;                 // states per launch of width 2^l at most
+  struct Launch {                             // expansion launch of one width in a step sequence
+    int l = 0;
+    ExpandArgs a{};
+    u32 grid = 0, waves = 0;
+  };
+  int launch_sequence(const Launch* L, int nl, u32 active, int steps);
+  int steps_ = 8;                             // batches per width between host synchronisations
+  bool graphs_ = true;                        // replay the step sequence from a hipGraph
+  hipGraphExec_t graph_exec_ = nullptr;
+  std::vector<char> graph_key_;               // launch parameters of graph_exec_
   int lane_alloc(int l);
   int upload();
   void clear_stage();
@@ -249,17 +275,18 @@ class BBEngine {
   std::vector<u64> rows_stage_;
   std::vector<u32> maps_stage_;
   std::vector<PartDesc> parts_stage_;
-  std::vector<u64> roots_stage_[7], rev_stage_[7];
+  std::vector<u64> roots_stage_[NLANE], rev_stage_[NLANE];
   std::vector<u32> rowbuild_;  // staged partitions whose rows the device builds
+  u32 rowbuild_words_ = 0;     // widest row among them
   bool rows_built_ = false;    // a row build was launched in this step
   const u32 *src_off_ = nullptr, *src_adj_ = nullptr;
   const u64 *src_off_base_ = nullptr, *src_adj_base_ = nullptr;
   u64 staged_roots_ = 0;
-  u64 top_[7] = {0, 0, 0, 0, 0, 0, 0};
-  u64 cap_[7] = {0, 0, 0, 0, 0, 0, 0};
-  double fanout_[7] = {2, 2, 2, 2, 2, 2, 2};  // children per parent in the last batch, smoothed
-  u64 spill_n_[7] = {0, 0, 0, 0, 0, 0, 0};    // states in the spill buffer per width
-  u64 spill_cap_[7] = {0, 0, 0, 0, 0, 0, 0};  // its capacity in states
+  u64 top_[NLANE] = {};
+  u64 cap_[NLANE] = {};
+  double fanout_[NLANE] = {2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2};  // children per parent in the last batch, smoothed
+  u64 spill_n_[NLANE] = {};    // states in the spill buffer per width
+  u64 spill_cap_[NLANE] = {};  // its capacity in states
   int spill(int l);
   int unspill(int l);
 };

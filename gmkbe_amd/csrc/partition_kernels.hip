@@ -8,7 +8,9 @@
 // from the graph's CSR: one workgroup of four waves per partition, one wave per
 // row at a time; lanes split the row's adjacency list, find each neighbour's
 // local ID by binary search in the other side's map, and set its bit in an LDS
-// copy of the row, which the wave then writes out with coalesced stores.
+// copy of the row, which the wave then writes out with coalesced stores.  Rows
+// of up to 64 words use 2 KB of LDS per workgroup; a launch with wider
+// partitions (up to 1024 words) uses the 32 KB form.
 #include "gmk_device.h"
 
 namespace gmk {
@@ -24,10 +26,12 @@ struct RowBuildArgs {
   const u64* adj_base;  // per graph: start of its lists in adj
   u64* rows;
   u64* bytes;           // algorithmic bytes (adjacency + probes + rows)
+  u32 max_words;        // widest row of the partitions built
 };
 
+template <int MAXW>
 __global__ __launch_bounds__(256) void build_part_rows(RowBuildArgs a) {
-  __shared__ u64 row[4][64];
+  __shared__ u64 row[4][MAXW];
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   if (blockIdx.x >= a.n) return;
   const PartDesc P = a.parts[a.pids[blockIdx.x]];
@@ -43,7 +47,7 @@ __global__ __launch_bounds__(256) void build_part_rows(RowBuildArgs a) {
     const u32 n_other = left ? P.nr : P.nl;
     const u32 w_row = left ? P.wr : P.wl;
     u64* out = a.rows + (left ? P.rows_l + (u64)i * P.wr : P.rows_r + (u64)i * P.wl);
-    row[wv][lane] = 0;
+    for (u32 w = lane; w < w_row; w += 64) row[wv][w] = 0;
     __builtin_amdgcn_wave_barrier();
     const u32 e0 = off[v], e1 = off[v + 1];
     for (u32 e = e0 + lane; e < e1; e += 64) {
@@ -60,8 +64,10 @@ __global__ __launch_bounds__(256) void build_part_rows(RowBuildArgs a) {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-    if ((u32)lane < w_row) out[lane] = row[wv][lane];
-    bytes += (u32)lane < w_row ? 8 : 0;
+    for (u32 w = lane; w < w_row; w += 64) {
+      out[w] = row[wv][w];
+      bytes += 8;
+    }
     __builtin_amdgcn_wave_barrier();
   }
   const u64 b = wave_sum_u64(bytes);
@@ -70,7 +76,9 @@ __global__ __launch_bounds__(256) void build_part_rows(RowBuildArgs a) {
 
 hipError_t launch_build_part_rows(const RowBuildArgs& a, hipStream_t s) {
   if (a.n == 0) return hipSuccess;
-  hipLaunchKernelGGL(build_part_rows, dim3(a.n), dim3(256), 0, s, a);
+  if (a.max_words > 1024) return hipErrorInvalidValue;
+  if (a.max_words <= 64) hipLaunchKernelGGL(build_part_rows<64>, dim3(a.n), dim3(256), 0, s, a);
+  else hipLaunchKernelGGL(build_part_rows<1024>, dim3(a.n), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 

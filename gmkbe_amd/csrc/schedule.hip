@@ -7,10 +7,12 @@
 // first.  The model only reorders work: every item is still enumerated
 // exactly, and a run without a model keeps the degree order of the IDs.
 //
-// Device work: one lane per item computes its features and the forward pass
-// (item_score), every 64th score is sampled to set the threshold of the best
-// `want` items, and the items above it are gathered (score_select).  The host
-// sorts only the gathered items.
+// Device work, on the run's stream with one synchronisation at the end: one
+// lane per item computes its features and the forward pass (item_score), every
+// 64th score is sampled and a radix select over the samples sets the threshold
+// of about 1.5 `want` items (select_threshold), the items above it are
+// gathered (score_select) and sorted best first by rank (sort_front).  The
+// host receives the sorted front.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -24,9 +26,11 @@ using namespace gmk;
 namespace {
 
 constexpr int IN = GMK_MLP_IN, H = GMK_MLP_HID;
-constexpr u32 SAMPLE = 32;           // neighbours sampled per item for the degree moments
-constexpr u32 SSTEP = 64;           // every SSTEP-th item's key is sampled for the selection threshold
-constexpr u32 SEL_MAX = 1u << 17;    // gathered items, at most
+constexpr u32 LOG2_SAMPLE = 4;
+constexpr u32 SAMPLE = 1u << LOG2_SAMPLE;  // neighbours sampled per item for the degree moments
+constexpr u32 SSTEP = 64;                  // every SSTEP-th item's key is sampled for the selection threshold
+constexpr u32 FRONT_CAP = 8192;            // gathered items, at most (sorted in LDS: 64 KiB)
+constexpr u32 SELECT_ITEMS = 2048;         // items per workgroup of score_select
 
 struct ItemArgs {
   const u32* off;
@@ -59,7 +63,8 @@ __device__ __forceinline__ void item_features(const ItemArgs& a, u32 u, float (&
   u32 mn = 0xFFFFFFFFu;
   double hop = 0;
   for (u32 i = 0; i < ns; ++i) {
-    const u32 e = ns == d ? i : (u32)(((u64)i * d) / ns);
+    // i * d / ns with ns == SAMPLE a power of two when d > SAMPLE
+    const u32 e = ns == d ? i : (u32)(((u64)i * d) >> LOG2_SAMPLE);
     const u32 v = a.adj[o + e];
     const u32 dv = a.off[v + 1] - a.off[v];
     const float l = log1pf((float)dv);
@@ -81,48 +86,117 @@ __device__ __forceinline__ void item_features(const ItemArgs& a, u32 u, float (&
   x[7] = a.k;
 }
 
-// One lane per item: features, then (with a model) the forward pass with the
-// hidden layer in registers and the weights in LDS (w2 transposed so a lane
-// reads one output's weights contiguously).
+// bf16 bits of x rounded to nearest even (x finite), and back.
+__device__ __forceinline__ short bf16_rne(float x) {
+  u32 u = __float_as_uint(x);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (short)(u >> 16);
+}
+__device__ __forceinline__ float bf16_val(short h) { return __uint_as_float((u32)(uint16_t)h << 16); }
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+// One lane per item: features, then (with a model) the forward pass, 64 items
+// per wave.  The hidden-to-hidden layer (64 x 64 per item, 88% of the
+// multiply-adds) runs on the matrix cores: H2^T = W2^T H1^T with
+// v_mfma_f32_32x32x8_bf16 over 2 x 2 tiles of 32 (output units x items) and 8
+// steps of 8 inputs, each operand split into bf16 high and low parts
+// (a b ~ ah bh + ah bl + al bh, three products per step, FP32 accumulation:
+// about 2^-15 relative error per product).  Operand layout of the instruction
+// (lane l, j = 0..3): A[i = l % 32][k = 4 (l / 32) + j], B[k = 4 (l / 32) + j]
+// [n = l % 32]; result D[8 (r / 4) + 4 (l / 32) + r % 4][l % 32], r = 0..15.
+// So lane l feeds item n = 32 t + l % 32 of tile t (its own item or that of
+// lane l ^ 32, whose features it takes by a shuffle) and computes that item's
+// first-layer units k = 8 s + 4 (l / 32) + j of step s itself, in FP32.  W2^T
+// sits in LDS split into its two bf16 parts.  The output layer sums each
+// item's 64 units over the lane pair (l, l ^ 32).
 __global__ __launch_bounds__(256) void item_score(ItemArgs a) {
-  __shared__ float w1[IN * H], b1[H], w2t[H * H], b2[H], w3[H], mu[IN], sd[IN];
+  __shared__ float w1[IN * H], b1[H], b2[H], w3[H], mu[IN], sd[IN];
   __shared__ float b3;
+  __shared__ __align__(16) short w2h[H][H + 4], w2l[H][H + 4];  // W2^T: [output unit][input unit]
   if (a.P) {
     const float* P = a.P;
     constexpr int OW1 = 0, OB1 = OW1 + IN * H, OW2 = OB1 + H, OB2 = OW2 + H * H, OW3 = OB2 + H, OB3 = OW3 + H;
     for (int i = threadIdx.x; i < IN * H; i += blockDim.x) w1[i] = P[OW1 + i];
-    for (int i = threadIdx.x; i < H * H; i += blockDim.x) w2t[(i % H) * H + i / H] = P[OW2 + i];
+    for (int i = threadIdx.x; i < H * H; i += blockDim.x) {
+      const float w = P[OW2 + i];  // w2[input i / H][output i % H]
+      const short hi = bf16_rne(w);
+      w2h[i % H][i / H] = hi;
+      w2l[i % H][i / H] = bf16_rne(w - bf16_val(hi));
+    }
     for (int i = threadIdx.x; i < H; i += blockDim.x) { b1[i] = P[OB1 + i]; b2[i] = P[OB2 + i]; w3[i] = P[OW3 + i]; }
     if (threadIdx.x < IN) { mu[threadIdx.x] = P[OB3 + 1 + threadIdx.x]; sd[threadIdx.x] = P[OB3 + 1 + IN + threadIdx.x]; }
     if (threadIdx.x == 0) b3 = P[OB3];
     __syncthreads();
   }
   const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= a.n) return;
+  const bool live = i < a.n;
   float x[IN];
-  item_features(a, a.first + i, x);
-  if (a.feat) {
+  if (live) item_features(a, a.first + i, x);
+  if (live && a.feat) {
 #pragma unroll
     for (int q = 0; q < IN; ++q) a.feat[(u64)i * IN + q] = x[q];
   }
   if (!a.P) return;
+  // every lane of the wave takes part in the matrix products from here
 #pragma unroll
-  for (int q = 0; q < IN; ++q) x[q] = (x[q] - mu[q]) / sd[q];
-  float h1[H];
+  for (int q = 0; q < IN; ++q) x[q] = live ? (x[q] - mu[q]) / sd[q] : 0.f;
+  const int lane = threadIdx.x & 63, half = lane >> 5, col = lane & 31;
+  float xp[IN];  // features of lane l ^ 32's item
 #pragma unroll
-  for (int j = 0; j < H; ++j) {
-    float z = b1[j];
+  for (int q = 0; q < IN; ++q) xp[q] = __shfl_xor(x[q], 32, 64);
+  f32x16 acc[2][2];
 #pragma unroll
-    for (int q = 0; q < IN; ++q) z = fmaf(x[q], w1[q * H + j], z);
-    h1[j] = z > 0.f ? z : 0.f;
+  for (int m = 0; m < 2; ++m)
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[m][t][r] = 0.f;
+#pragma unroll 2
+  for (int s = 0; s < H / 8; ++s) {
+    const int k0 = 8 * s + 4 * half;
+    bf16x4 bh[2], bl[2];
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        float z = b1[k0 + j];
+#pragma unroll
+        for (int q = 0; q < IN; ++q) z = fmaf(t == half ? x[q] : xp[q], w1[q * H + k0 + j], z);
+        z = z > 0.f ? z : 0.f;
+        const short hi = bf16_rne(z);
+        bh[t][j] = hi;
+        bl[t][j] = bf16_rne(z - bf16_val(hi));
+      }
+    }
+#pragma unroll
+    for (int m = 0; m < 2; ++m) {
+      const bf16x4 ah = *(const bf16x4*)&w2h[32 * m + col][k0];
+      const bf16x4 al = *(const bf16x4*)&w2l[32 * m + col][k0];
+#pragma unroll
+      for (int t = 0; t < 2; ++t) {
+        acc[m][t] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(ah, bh[t], acc[m][t], 0, 0, 0);
+        acc[m][t] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(ah, bl[t], acc[m][t], 0, 0, 0);
+        acc[m][t] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(al, bh[t], acc[m][t], 0, 0, 0);
+      }
+    }
   }
-  float y = b3;
-  for (int j = 0; j < H; ++j) {
-    float z = b2[j];
+  float part[2] = {0.f, 0.f};
 #pragma unroll
-    for (int q = 0; q < H; ++q) z = fmaf(h1[q], w2t[j * H + q], z);
-    y = fmaf(z > 0.f ? z : 0.f, w3[j], y);
-  }
+  for (int m = 0; m < 2; ++m)
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int u = 32 * m + 8 * (r / 4) + 4 * half + (r % 4);
+        const float z = acc[m][t][r] + b2[u];
+        part[t] = fmaf(z > 0.f ? z : 0.f, w3[u], part[t]);
+      }
+#pragma unroll
+  for (int t = 0; t < 2; ++t) part[t] += __shfl_xor(part[t], 32, 64);
+  if (!live) return;
+  const float y = b3 + part[half];
   if (a.score) a.score[i] = y;
   if (!a.key) return;
   const u32 kk = score_key(y);
@@ -130,22 +204,107 @@ __global__ __launch_bounds__(256) void item_score(ItemArgs a) {
   if (a.sample && i % SSTEP == 0) a.sample[i / SSTEP] = kk;
 }
 
-// Items with key >= thr, gathered with one counter atomic per wave (a wave's
-// passing lanes take consecutive slots).  Entries: key << 32 | ~item, so a
-// descending sort puts ties in ascending item order.
-__global__ __launch_bounds__(256) void score_select(const u32* key, u32 n, u32 thr, u32 cap, u32* cnt, u64* out) {
-  const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int lane = threadIdx.x & 63;
-  const u32 kk = i < n ? key[i] : 0u;
-  const bool pass = i < n && kk >= thr;
-  const u64 m = __ballot(pass);
-  if (!m) return;
-  const int leader = __ffsll((unsigned long long)m) - 1;
-  u32 base = 0;
-  if (lane == leader) base = atomicAdd(cnt, (u32)__popcll(m));
-  base = (u32)__shfl((int)base, leader, 64);
-  const u32 at = base + (u32)__popcll(m & ((1ull << lane) - 1ull));
-  if (pass && at < cap) out[at] = ((u64)kk << 32) | (u64)(~i);
+__global__ void fill_u32(u32* p, u32 n, u32 v) {
+  for (u32 i = threadIdx.x; i < n; i += blockDim.x) p[i] = v;
+}
+
+// Threshold of the selection (one workgroup): the key of rank `at` (0 = the
+// largest) among the n sampled keys, by a most-significant-digit-first radix
+// select over eight 4-bit digits.  Each lane counts its samples per digit in
+// registers (no atomics: the keys cluster, and atomics on one bin would
+// serialise), the waves reduce their counts and one lane picks the digit.
+// 0 when every item is to be gathered.
+__global__ __launch_bounds__(1024) void select_threshold(const u32* smp, u32 n, u32 at, int all, u32* thr) {
+  __shared__ u32 wcnt[16][16];  // [wave][digit]
+  __shared__ u32 prefix_s, at_s;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (all) {
+    if (threadIdx.x == 0) *thr = 0;
+    return;
+  }
+  if (threadIdx.x == 0) { prefix_s = 0; at_s = at; }
+  __syncthreads();
+  for (int shift = 28; shift >= 0; shift -= 4) {
+    const u32 prefix = prefix_s;
+    const u32 hmask = shift == 28 ? 0u : (0xFFFFFFFFu << (shift + 4));
+    u32 cnt[16];
+#pragma unroll
+    for (int d = 0; d < 16; ++d) cnt[d] = 0;
+    for (u32 i = threadIdx.x; i < n; i += blockDim.x) {
+      const u32 v = smp[i];
+      const bool in = (v & hmask) == prefix;
+      const u32 dg = (v >> shift) & 15u;
+#pragma unroll
+      for (int d = 0; d < 16; ++d) cnt[d] += (in && dg == (u32)d) ? 1u : 0u;
+    }
+#pragma unroll
+    for (int d = 0; d < 16; ++d) {
+      u32 c = cnt[d];
+      for (int m = 32; m > 0; m >>= 1) c += (u32)__shfl_xor((int)c, m, 64);
+      if (lane == 0) wcnt[wv][d] = c;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const int nw = (int)((blockDim.x + 63) / 64);
+      u32 r = at_s, d = 15;
+      for (;; --d) {
+        u32 h = 0;
+        for (int w = 0; w < nw; ++w) h += wcnt[w][d];
+        if (r < h || d == 0) break;
+        r -= h;
+      }
+      prefix_s = prefix | (d << shift);
+      at_s = r;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *thr = prefix_s;
+}
+
+// Items with key >= *thr, gathered as entries key << 32 | ~item (a descending
+// sort puts ties in ascending item order).  A workgroup collects the passing
+// items of its SELECT_ITEMS items in LDS and claims their output slots with
+// one atomic; slots beyond cap are dropped (cnt still counts them).
+__global__ __launch_bounds__(256) void score_select(const u32* key, u32 n, const u32* thr_p, u32 cap, u32* cnt,
+                                                    u64* out) {
+  __shared__ u64 list[SELECT_ITEMS];
+  __shared__ u32 nl, base;
+  if (threadIdx.x == 0) nl = 0;
+  __syncthreads();
+  const u32 thr = *thr_p;
+  const u32 i0 = blockIdx.x * SELECT_ITEMS;
+  for (u32 q = threadIdx.x; q < SELECT_ITEMS; q += blockDim.x) {
+    const u32 i = i0 + q;
+    if (i >= n) break;
+    const u32 kk = key[i];
+    if (kk >= thr) list[atomicAdd(&nl, 1u)] = ((u64)kk << 32) | (u64)(~i);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) base = nl ? atomicAdd(cnt, nl) : 0u;
+  __syncthreads();
+  for (u32 q = threadIdx.x; q < nl; q += blockDim.x)
+    if (base + q < cap) out[base + q] = list[q];
+}
+
+// Sorts the gathered entries best first by rank: the entries are distinct
+// (each holds its item), so an entry's position is the number of entries
+// above it.  Every workgroup stages all entries in LDS and ranks its share
+// (256 entries) against them, with broadcast LDS reads.  out[0] = entries
+// gathered (may exceed the cap), out[1 ..] = the sorted entries.
+constexpr u32 RANK_BLOCK = 256;
+__global__ __launch_bounds__(RANK_BLOCK) void sort_front(const u32* cnt, const u64* sel, u64* out) {
+  __shared__ u64 s[FRONT_CAP];
+  const u32 m = *cnt < FRONT_CAP ? *cnt : FRONT_CAP;
+  if (blockIdx.x == 0 && threadIdx.x == 0) out[0] = *cnt;
+  const u32 i = blockIdx.x * RANK_BLOCK + threadIdx.x;
+  if (blockIdx.x * RANK_BLOCK >= m) return;
+  for (u32 q = threadIdx.x; q < m; q += RANK_BLOCK) s[q] = sel[q];
+  __syncthreads();
+  if (i >= m) return;
+  const u64 e = s[i];
+  u32 rank = 0;
+  for (u32 q = 0; q < m; ++q) rank += s[q] > e ? 1u : 0u;
+  out[1 + rank] = e;
 }
 
 }  // namespace
@@ -154,60 +313,55 @@ int gmk::learned_front(gmk_ctx* c, u32 a_lo, u32 a_hi, u32 k, u32 want, std::vec
   front.clear();
   const u32 n = a_hi - a_lo;
   if (!c->has_item_model || n == 0 || want == 0) return GMK_OK;
-  want = std::min(want, std::min(n, SEL_MAX / 4));
+  want = std::min(want, std::min(n, FRONT_CAP / 2));
   hipStream_t s = c->stream;
   int r;
   const u32 ns = (n + SSTEP - 1) / SSTEP;
+  // device: parameters, then (u32) count, threshold, keys, samples; entries;
+  // sorted front.  The parameters and the sorted front pass through the
+  // context's pinned staging.
   if ((r = ensure(c, c->sched_p, (MLP_NPARAM + 2 * IN) * 4))) return r;
   if ((r = ensure(c, c->sched_hist, 16 + 4 * ((size_t)n + ns)))) return r;
-  if ((r = ensure(c, c->sched_sel, (size_t)SEL_MAX * 8))) return r;
-  std::vector<float> P(MLP_NPARAM + 2 * IN);
-  mlp_pack(c->item_model, P.data());
+  if ((r = ensure(c, c->sched_sel, (size_t)2 * (FRONT_CAP + 1) * 8))) return r;
+  static_assert((size_t)(MLP_NPARAM + 2 * IN) * 4 + (FRONT_CAP + 1) * 8 <=
+                    ((size_t)MLP_NPARAM + 2 * IN + GMK_STATIC_ITEMS) * 4,
+                "the pinned staging holds the parameters and the sorted front");
+  float* P = c->h_stage;
+  u64* hfront = (u64*)(c->h_stage + (MLP_NPARAM + 2 * IN + 1) / 2 * 2);
+  mlp_pack(c->item_model, P);
   for (int q = 0; q < IN; ++q) { P[MLP_NPARAM + q] = c->item_model.mu[q]; P[MLP_NPARAM + IN + q] = c->item_model.sd[q]; }
   u32* cnt = (u32*)c->sched_hist.p;
+  u32* thr = cnt + 1;
   u32* key = cnt + 4;
   u32* smp = key + n;
-  HIPCHK(c, hipMemcpyAsync(c->sched_p.p, P.data(), P.size() * 4, hipMemcpyHostToDevice, s));
-  HIPCHK(c, hipMemsetAsync(cnt, 0, 16, s));
+  u64* sel = (u64*)c->sched_sel.p;
+  u64* sorted = sel + FRONT_CAP + 1;
+  HIPCHK(c, hipMemcpyAsync(c->sched_p.p, P, (MLP_NPARAM + 2 * IN) * 4, hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(fill_u32, dim3(1), dim3(64), 0, s, cnt, 4u, 0u);
   ItemArgs a{c->d_off, c->d_adj, a_lo, n, (a_lo == 0 ? c->n - c->n_left : c->n_left), (float)k,
              (const float*)c->sched_p.p, nullptr, key, smp, nullptr};
-  const dim3 g((n + 255) / 256), b(256);
-  hipLaunchKernelGGL(item_score, g, b, 0, s, a);
+  hipLaunchKernelGGL(item_score, dim3((n + 255) / 256), dim3(256), 0, s, a);
+  // Threshold: the sampled key ranked at about 1.5 `want` items, so that the
+  // gathered set holds the best `want` items with room for the sample's error
+  // and stays under FRONT_CAP; every item when there are at most 1.5 `want`.
+  const u64 span = (u64)want * 3 / 2;
+  const u32 at = (u32)std::min<u64>(ns - 1, (span + SSTEP - 1) / SSTEP);
+  hipLaunchKernelGGL(select_threshold, dim3(1), dim3(1024), 0, s, (const u32*)smp, ns, at, n <= span ? 1 : 0, thr);
+  hipLaunchKernelGGL(score_select, dim3((n + SELECT_ITEMS - 1) / SELECT_ITEMS), dim3(256), 0, s, (const u32*)key, n,
+                     (const u32*)thr, FRONT_CAP, cnt, sel);
+  hipLaunchKernelGGL(sort_front, dim3(FRONT_CAP / RANK_BLOCK), dim3(RANK_BLOCK), 0, s, (const u32*)cnt,
+                     (const u64*)sel, sorted);
   HIPCHK(c, hipGetLastError());
-  // Threshold from the sampled keys: the key of the sample ranked at about
-  // twice `want` items, so the gathered set holds the best `want` items with
-  // room for the sample's error (it is gathered whole up to SEL_MAX).
-  std::vector<u32> h(ns);
-  HIPCHK(c, hipMemcpyAsync(h.data(), smp, (size_t)ns * 4, hipMemcpyDeviceToHost, s));
+  HIPCHK(c, hipMemcpyAsync(hfront, sorted, (size_t)(FRONT_CAP + 1) * 8, hipMemcpyDeviceToHost, s));
   HIPCHK(c, hipStreamSynchronize(s));
-  const size_t at = std::min<size_t>(ns - 1, (2ull * want + SSTEP - 1) / SSTEP);
-  std::nth_element(h.begin(), h.begin() + at, h.end(), std::greater<u32>());
-  const u32 thr = n <= 2ull * want ? 0u : h[at];
-  hipLaunchKernelGGL(score_select, g, b, 0, s, (const u32*)key, n, thr, SEL_MAX, cnt, (u64*)c->sched_sel.p);
-  HIPCHK(c, hipGetLastError());
-  u32 m = 0;
-  HIPCHK(c, hipMemcpyAsync(&m, cnt, 4, hipMemcpyDeviceToHost, s));
-  HIPCHK(c, hipStreamSynchronize(s));
-  m = std::min(m, SEL_MAX);
-  std::vector<u64> sel(m);
-  if (m) {
-    HIPCHK(c, hipMemcpyAsync(sel.data(), c->sched_sel.p, (size_t)m * 8, hipMemcpyDeviceToHost, s));
-    HIPCHK(c, hipStreamSynchronize(s));
-  }
-  front.swap(sel);
+  // entries gathered beyond FRONT_CAP (a rare sampling error) are left out of
+  // the front; their items follow in ID order like every item outside it
+  const u64 m = std::min<u64>(hfront[0], FRONT_CAP);
+  front.assign(hfront + 1, hfront + 1 + m);
   return GMK_OK;
 }
 
-u32 gmk::LearnedOrder::next(u32 a_lo) {
-  if (pos == sorted) {
-    // bring the next best STEP entries to the front of the rest, sorted
-    const size_t end = std::min(cand.size(), pos + STEP);
-    std::nth_element(cand.begin() + pos, cand.begin() + (end - 1), cand.end(), std::greater<u64>());
-    std::sort(cand.begin() + pos, cand.begin() + end, std::greater<u64>());
-    sorted = end;
-  }
-  return a_lo + ~(u32)(cand[pos++] & 0xFFFFFFFFu);
-}
+u32 gmk::LearnedOrder::next(u32 a_lo) { return a_lo + ~(u32)(cand[pos++] & 0xFFFFFFFFu); }
 
 // Features (predict = false) or predictions of the item model (predict = true)
 // of items first .. first + count - 1, by the scheduling kernel itself.

@@ -28,7 +28,7 @@ constexpr u32 FCAP = 1u << 18;         // families per launch
 constexpr u64 RCH = 1ull << 22;        // ranks per check launch
 constexpr u64 SHRINK_MIN = 1ull << 16;  // remaining results below which a launch takes the slot check
 constexpr u32 DEFER_BUDGET = 64;       // candidates one lane tests before a rank goes to a whole wave
-constexpr u32 LEARNED_FRONT = 4096;    // about twice this many one-vertex items are ordered by the learned schedule (the rest follow in ID order)
+constexpr u32 LEARNED_FRONT = 4096;    // about 1.5 times this many one-vertex items are ordered by the learned schedule (the rest follow in ID order)
 constexpr u64 LEARNED_SLICE = 1u << 16; // ranks of each family per slice in learned runs
 
 struct ThinRun {
@@ -318,9 +318,7 @@ int gmk_run_thin(gmk_ctx* c, const gmk_params* prm, gmk_stats* st, RunState& rs,
     TupleFeed feed(T, 1, chunk);
     std::vector<u32> taken;
     while (!order.done()) {
-      const auto t1 = std::chrono::steady_clock::now();
       u32 u = order.next(a_lo);
-      rs.ms_schedule += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t1).count();
       taken.push_back(u);
       if ((r = feed.push(&u)) || T.stop) return r;
     }

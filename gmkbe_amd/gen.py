@@ -14,6 +14,8 @@ Workloads (sizes are the ones the benchmarks and tests use):
   * chung_lu(2_000_000, 1_000_000, 20_000_000, 2.0)
   * planted(500_000, 500_000, 5_000_000, 2.2, 200 near-bicliques of sides
     uniform in [8, 40], at most 3 missing edges per vertex)
+  * hub(n_u, n_v, ...)                      a hub adjacent to every left vertex:
+    seed partitions with 4,500 to 40,000 vertices on a side (wide bitsets)
 """
 from __future__ import annotations
 
@@ -154,6 +156,29 @@ def planted_edges(n_u: int, n_v: int, m: int, alpha: float, n_cores: int, side_l
         keep = ~np.isin(uu * n_v + vv, rk)
         uu, vv = uu[keep], vv[keep]
     return uu.astype(np.uint32), (vv + n_u).astype(np.uint32)
+
+
+def hub_edges(n_u: int, n_v: int, blocks: int, block_side: int, seed: int) -> tuple[np.ndarray, np.ndarray]:
+    """A graph whose seed partitions are wide: right vertex 0 (the hub) is
+    adjacent to every left vertex, so the 2-hop set of any left seed is the
+    whole left side; every left vertex also gets one random right neighbour,
+    and `blocks` planted block_side x block_side bicliques (on random vertex
+    sets) each lose about 10% of their edges."""
+    rng = _rng(seed)
+    us = [np.arange(n_u, dtype=np.int64), np.arange(n_u, dtype=np.int64)]
+    vs = [np.zeros(n_u, dtype=np.int64), rng.integers(1, n_v, n_u)]
+    for _ in range(blocks):
+        A = rng.choice(n_u, block_side, replace=False)
+        B = rng.choice(np.arange(1, n_v), block_side, replace=False)
+        a, b = np.meshgrid(A, B, indexing="ij")
+        keep = rng.random(a.size) > 0.1
+        us.append(a.ravel()[keep]); vs.append(b.ravel()[keep])
+    e = np.unique(np.stack([np.concatenate(us), np.concatenate(vs)], 1), axis=0)
+    return e[:, 0].astype(np.uint32), (e[:, 1] + n_u).astype(np.uint32)
+
+
+def hub(n_u: int, n_v: int, blocks: int, block_side: int, seed: int) -> Graph:
+    return convert_edges(*hub_edges(n_u, n_v, blocks, block_side, seed))
 
 
 def er(n_u: int, n_v: int, p: float, seed: int) -> Graph:

@@ -34,10 +34,12 @@ enum {
  *                    one-candidate-at-a-time structure, src/main.cu:1078-1506)
  *   GMK_MODE_BATCHED work items fill each launch: thin tuples by the tens of
  *                    thousands, search states of all partitions in one device
- *                    stack per bitset width (one workgroup per state, or a
- *                    persistent grid taking states from a device counter, as
- *                    the tuning table says); items in the degree order of the
- *                    vertex IDs (the heuristic order)
+ *                    stack per bitset width, expanded in batches that the
+ *                    device plans from the last batch's outcome (one
+ *                    workgroup per state, or a persistent grid taking states
+ *                    from a device counter, as the tuning table says); items
+ *                    in the degree order of the vertex IDs (the heuristic
+ *                    order)
  *   GMK_MODE_LEARNED as BATCHED, with the one-vertex items ordered by the item
  *                    cost model set with gmk_set_item_model: best predicted
  *                    results per unit of device time first (single-graph
@@ -101,6 +103,9 @@ typedef struct {
                                 before the seed partitions */
   uint64_t bytes_reduce;     /* its algorithmic bytes */
   uint64_t reduce_removed;   /* vertices it removed */
+  uint64_t widest_side;      /* most vertices on one side of any branch-and-bound partition
+                                (up to 8192 per side a state's working set is in LDS, up
+                                to 65536 in HBM) */
 } gmk_stats;
 
 /* Create a context on HIP device `device` (-d, src/main.cu:1765-1766). */
@@ -165,12 +170,13 @@ typedef struct {
 } gmk_graph_stats;
 
 /* Enumerate many graphs in one call (the reference runs one graph per process).
- * Each graph is one global partition (both sides <= 4096).  params->limit is a
+ * Each graph with theta <= 2k is one global partition (both sides <= 65536),
+ * each other graph runs as seed partitions.  params->limit is a
  * per-graph cap (0 = none); params->k / theta are ignored (per-graph values).
  *   GMK_MODE_PLAIN    graphs one after another, one block per state
- *   GMK_MODE_BATCHED  all graphs' states in one device stack, drained by the
- *                     persistent expansion kernel, graphs taken in `order`
- *                     (null = input order)
+ *   GMK_MODE_BATCHED  all graphs' states in one device stack per bitset width,
+ *                     in batches planned on the device, graphs taken in
+ *                     `order` (null = input order)
  *   GMK_MODE_LEARNED  as BATCHED, with the order given by the cost model set
  *                     with gmk_set_cost_model (most expensive first), or by
  *                     the size heuristic (edges x vertices) when none is set.
@@ -224,7 +230,7 @@ int gmk_set_cost_model(gmk_ctx* ctx, const gmk_mlp* model);
  * Features of items first .. first + count - 1 (global IDs, one side), for k:
  * log(1+d(u)), log(1+|other side| - d(u)), mean, max and standard deviation of
  * log(1+d(v)) over N(u), log(1+two-hop paths from u), log(1+min d(v)), k
- * (moments over at most 32 neighbours spread evenly over N(u)).  out is
+ * (moments over at most 16 neighbours spread evenly over N(u)).  out is
  * count x GMK_MLP_IN. */
 int gmk_item_features(gmk_ctx* ctx, uint32_t first, uint32_t count, uint32_t k, float* out);
 /* Predictions of the item model set with gmk_set_item_model for items first ..

@@ -13,6 +13,10 @@ Outputs (data only):
                             the reference BPPivot's printed result set
   c3_slice.json             (--c3) per graph of the first 40 of the many-graphs
                             workload's draw: theta and the reference's digest
+  wide.json                 (--wide) graphs with seed partitions of 4,500 to
+                            40,000 vertices a side (gen.hub): the reference's
+                            count and digests; for k = 0 also those of the
+                            maximal bicliques by intersection closure
   sets/<case>.u32           the full record stream for small cases
 """
 from __future__ import annotations
@@ -195,8 +199,64 @@ def main_c3() -> None:
                    "cases": cases}, f, indent=1)
 
 
+# Graphs with wide seed partitions (gen.hub: a hub adjacent to every left
+# vertex): (name, n_u, n_v, blocks, block side, seed, k, theta).  Their left
+# seed partitions hold the whole left side: 4,500 to 40,000 vertices.
+WIDE = [
+    ("hub4500x1000", 4500, 1000, 3, 6, 3, 1, 3),
+    ("hub9000x1000", 9000, 1000, 3, 6, 6, 1, 3),
+    ("hub5000x2000", 5000, 2000, 5, 6, 1, 0, 2),
+    ("hub9000x2000", 9000, 2000, 5, 6, 2, 0, 2),
+    ("hub20000x3000", 20000, 3000, 5, 6, 4, 0, 2),
+    ("hub40000x3000", 40000, 3000, 5, 6, 5, 0, 2),
+]
+
+
+def closure_digest(g, theta: int) -> dict:
+    """Maximal bicliques by intersection closure (oracle_lib.closure_bicliques):
+    the k = 0 check of these graphs, where the reference BPPivot's output
+    misses some maximal bicliques (each closure record is checked maximal by
+    the oracle's validator)."""
+    words = oracle_lib.closure_bicliques(g, theta)
+    if oracle_lib.validate(g, 0, theta, words) != 0:
+        raise SystemExit("closure records fail the validator")
+    dg = oracle_lib.digest(words)
+    return {"count": int(dg.count), "fp_sum": str(dg.fp_sum), "fp_xor": str(dg.fp_xor),
+            "canon_lo": str(dg.canon_lo), "canon_hi": str(dg.canon_hi)}
+
+
+def main_wide() -> None:
+    wpath = os.path.join(HERE, "wide.json")
+    have = {c["name"]: c for c in json.load(open(wpath))["cases"]} if os.path.exists(wpath) else {}
+    cases = []
+    for name, a, b, blocks, side, seed, k, theta in WIDE:
+        if name in have and "--force" not in sys.argv:
+            c = have[name]
+            if k == 0 and "closure" not in c:
+                c["closure"] = closure_digest(gen.hub(a, b, blocks, side, seed), theta)
+            cases.append(c)
+            continue
+        g = gen.hub(a, b, blocks, side, seed)
+        with tempfile.TemporaryDirectory() as td:
+            path = os.path.join(td, "g.graph.bin")
+            graphio.write_graph_bin(path, g)
+            words = run_bppivot(path, k, theta, timeout=600)
+        dg = oracle_lib.digest(words)
+        c = {"name": name, "n_u": a, "n_v": b, "blocks": blocks, "side": side, "seed": seed, "k": k,
+             "theta": theta, "edges": int(g.n_edges), "count": int(dg.count), "fp_sum": str(dg.fp_sum),
+             "fp_xor": str(dg.fp_xor), "canon_lo": str(dg.canon_lo), "canon_hi": str(dg.canon_hi)}
+        if k == 0:
+            c["closure"] = closure_digest(g, theta)
+        cases.append(c)
+        print(name, k, theta, dg.count, flush=True)
+    with open(os.path.join(HERE, "wide.json"), "w") as f:
+        json.dump({"cases": cases}, f, indent=1)
+
+
 if __name__ == "__main__":
     if "--c3" in sys.argv:
         main_c3()
+    elif "--wide" in sys.argv:
+        main_wide()
     else:
         main()

@@ -268,3 +268,39 @@ def cpu_thin_port(g, k, limit=0, threads=0):
     rk = ctypes.c_uint64(0)
     n = _port.cpu_thin_port(g.n, g.n_left, _p(off), _p(adj), k, limit, threads, ctypes.byref(sec), ctypes.byref(rk))
     return int(n), float(sec.value), int(rk.value)
+
+
+def closure_bicliques(g, theta):
+    """Maximal bicliques (k = 0) with both sides >= theta, as a record stream,
+    by closure under intersection: R is the right side of a maximal biclique
+    iff R = the intersection of N(u) over its common neighbours u, and each such
+    R is reached from some N(u) by intersecting with further N(u') while its
+    size stays >= theta.  Independent of the branch-and-bound (a textbook
+    characterisation of maximal bicliques); pinned against brute force in
+    tests/test_oracle.py."""
+    nl = int(g.n_left)
+    off = np.asarray(g.off, dtype=np.int64)
+    adj = np.asarray(g.adj, dtype=np.int64)
+    nbr = lambda v: adj[off[v]:off[v + 1]]
+    base = {tuple(nbr(u)) for u in range(nl) if off[u + 1] - off[u] >= theta}
+    found = set(base)
+    frontier = list(base)
+    cover = {}
+    while frontier:
+        new = []
+        for s in frontier:
+            # cnt[u] = |s & N(u)| over the left vertices
+            cnt = np.bincount(np.concatenate([nbr(x) for x in s]), minlength=nl)
+            cover[s] = np.nonzero(cnt == len(s))[0]
+            for u in np.nonzero((cnt >= theta) & (cnt < len(s)))[0]:
+                t = tuple(np.intersect1d(np.asarray(s, dtype=np.int64), nbr(u)))
+                if t not in found:
+                    found.add(t)
+                    new.append(t)
+        frontier = new
+    parts = []
+    for s in sorted(found):
+        L = cover[s]
+        if len(L) >= theta:
+            parts.append(np.concatenate([[len(L)], L, [len(s)], np.asarray(s)]).astype(np.uint32))
+    return np.concatenate(parts) if parts else np.zeros(0, dtype=np.uint32)

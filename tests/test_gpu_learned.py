@@ -22,14 +22,14 @@ def eng():
 
 def _features_numpy(g, k, first, count):
     """The item features (gmk_item_features) restated in numpy: degree moments
-    over at most 32 neighbours spread evenly over N(u)."""
+    over at most 16 neighbours spread evenly over N(u)."""
     deg = g.degrees().astype(np.int64)
     nb = g.n - g.n_left if first < g.n_left else g.n_left
     out = np.zeros((count, 8), dtype=np.float64)
     for i in range(count):
         u = first + i
         o, d = int(g.off[u]), int(deg[u])
-        ns = min(d, 32)
+        ns = min(d, 16)
         idx = np.arange(ns) if ns == d else (np.arange(ns, dtype=np.int64) * d) // ns
         dv = deg[g.adj[o + idx].astype(np.int64)] if ns else np.zeros(0, dtype=np.int64)
         lg = np.log1p(dv.astype(np.float64))

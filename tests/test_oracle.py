@@ -6,7 +6,7 @@ import pytest
 
 import golden_cases as GC
 import oracle_lib as O
-from gmkbe_amd import graphio
+from gmkbe_amd import gen, graphio
 
 
 # BPPivot is exact for k >= 1 only: at k = 0 its seeded mode keeps 3-hop
@@ -26,6 +26,31 @@ def test_reference_bppivot_incomplete_at_k0():
     ref = GC.reference_set(c)
     sb, sr = set(O.records(wb)), set(O.records(ref))
     assert sr < sb  # every reference result is correct, some are missing
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_closure_bicliques_match_brute(seed):
+    """The intersection-closure enumeration of maximal bicliques (the k = 0
+    check of the wide-partition graphs) equals brute force."""
+    rng = np.random.default_rng(700 + seed)
+    a, b = int(rng.integers(3, 11)), int(rng.integers(3, 11))
+    u, v = gen.er_edges(a, b, float(rng.uniform(0.2, 0.8)), 700 + seed)
+    g = graphio.from_edges(a, b, u, v - a)
+    for theta in (1, 2, 3):
+        wb, _ = O.brute(g, 0, theta)
+        got, want = O.digest(O.closure_bicliques(g, theta)), O.digest(wb)
+        assert (got.count, got.canon_lo, got.canon_hi) == (want.count, want.canon_lo, want.canon_hi)
+
+
+def test_bppivot_incomplete_at_k0_on_hub_graph():
+    """On the 5000 x 2000 hub graph (k = 0, theta = 2) BPPivot's branching
+    misses maximal bicliques that the closure finds and the validator accepts."""
+    g = gen.hub(5000, 2000, 5, 6, 1)
+    closure = O.closure_bicliques(g, 2)
+    assert O.validate(g, 0, 2, closure) == 0
+    words, _, _ = O.bppivot(g, 0, 2)
+    sc, so = set(O.records(closure)), set(O.records(words))
+    assert so < sc and len(sc) == 1473
 
 
 @pytest.mark.parametrize("c", [c for c in GC.cases(min_k=1) if GC.graph(c["graph"]).n <= 24], ids=GC.case_id)
