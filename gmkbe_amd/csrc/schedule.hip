@@ -28,7 +28,8 @@ namespace {
 constexpr int IN = GMK_MLP_IN, H = GMK_MLP_HID;
 constexpr u32 LOG2_SAMPLE = 4;
 constexpr u32 SAMPLE = 1u << LOG2_SAMPLE;  // neighbours sampled per item for the degree moments
-constexpr u32 SSTEP = 64;                  // every SSTEP-th item's key is sampled for the selection threshold
+constexpr u32 SSTEP_MIN = 64;              // every sstep-th item's key is sampled for the selection threshold
+constexpr u32 SEL_PER_LANE = 32;           // samples per lane of select_threshold (at most 32768 samples)
 constexpr u32 FRONT_CAP = 8192;            // gathered items, at most (sorted in LDS: 64 KiB)
 constexpr u32 SELECT_ITEMS = 2048;         // items per workgroup of score_select
 
@@ -41,8 +42,9 @@ struct ItemArgs {
   const float* P;   // model parameters (gmk::mlp_pack order), then mu[IN], sd[IN]; null: features only
   float* feat;      // n x IN features, or null
   u32* key;         // order-preserving score keys, or null
-  u32* sample;      // key of every SSTEP-th item, or null
+  u32* sample;      // key of every sstep-th item, or null
   float* score;     // predictions, or null
+  u32 sstep;        // sample stride, a power of two (with sample)
 };
 
 // Order-preserving map float -> u32 (larger float, larger key; NaN lowest).
@@ -58,22 +60,31 @@ __device__ __forceinline__ u32 score_key(float y) {
 __device__ __forceinline__ void item_features(const ItemArgs& a, u32 u, float (&x)[IN]) {
   const u32 o = a.off[u], d = a.off[u + 1] - o;
   const u32 ns = d < SAMPLE ? d : SAMPLE;
+  // the sampled neighbours' degrees, every load of a round issued together
+  u32 nb[SAMPLE], dv[SAMPLE];
+#pragma unroll
+  for (u32 i = 0; i < SAMPLE; ++i) {
+    // i * d / ns with ns == SAMPLE a power of two when d > SAMPLE
+    const u32 e = ns == d ? i : (u32)(((u64)i * d) >> LOG2_SAMPLE);
+    nb[i] = i < ns ? a.adj[o + e] : 0u;
+  }
+#pragma unroll
+  for (u32 i = 0; i < SAMPLE; ++i) dv[i] = i < ns ? a.off[nb[i] + 1] - a.off[nb[i]] : 0u;
   // mean and variance by Welford's update (no cancellation when the degrees agree)
   float mean = 0.f, m2 = 0.f, mx = 0.f;
   u32 mn = 0xFFFFFFFFu;
   double hop = 0;
-  for (u32 i = 0; i < ns; ++i) {
-    // i * d / ns with ns == SAMPLE a power of two when d > SAMPLE
-    const u32 e = ns == d ? i : (u32)(((u64)i * d) >> LOG2_SAMPLE);
-    const u32 v = a.adj[o + e];
-    const u32 dv = a.off[v + 1] - a.off[v];
-    const float l = log1pf((float)dv);
-    const float delta = l - mean;
-    mean += delta / (float)(i + 1);
-    m2 += delta * (l - mean);
-    mx = l > mx ? l : mx;
-    mn = dv < mn ? dv : mn;
-    hop += (double)(dv - 1);
+#pragma unroll
+  for (u32 i = 0; i < SAMPLE; ++i) {
+    if (i < ns) {
+      const float l = __logf(1.f + (float)dv[i]);
+      const float delta = l - mean;
+      mean += __fdividef(delta, (float)(i + 1));
+      m2 += delta * (l - mean);
+      mx = l > mx ? l : mx;
+      mn = dv[i] < mn ? dv[i] : mn;
+      hop += (double)(dv[i] - 1);
+    }
   }
   const float var = ns ? m2 / (float)ns : 0.f;
   x[0] = log1pf((float)d);
@@ -98,27 +109,34 @@ typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 // One lane per item: features, then (with a model) the forward pass, 64 items
-// per wave.  The hidden-to-hidden layer (64 x 64 per item, 88% of the
-// multiply-adds) runs on the matrix cores: H2^T = W2^T H1^T with
-// v_mfma_f32_32x32x8_bf16 over 2 x 2 tiles of 32 (output units x items) and 8
-// steps of 8 inputs, each operand split into bf16 high and low parts
-// (a b ~ ah bh + ah bl + al bh, three products per step, FP32 accumulation:
-// about 2^-15 relative error per product).  Operand layout of the instruction
-// (lane l, j = 0..3): A[i = l % 32][k = 4 (l / 32) + j], B[k = 4 (l / 32) + j]
-// [n = l % 32]; result D[8 (r / 4) + 4 (l / 32) + r % 4][l % 32], r = 0..15.
-// So lane l feeds item n = 32 t + l % 32 of tile t (its own item or that of
-// lane l ^ 32, whose features it takes by a shuffle) and computes that item's
-// first-layer units k = 8 s + 4 (l / 32) + j of step s itself, in FP32.  W2^T
-// sits in LDS split into its two bf16 parts.  The output layer sums each
-// item's 64 units over the lane pair (l, l ^ 32).
+// per wave, both weight layers on the matrix cores: Z^T = W^T X^T with
+// v_mfma_f32_32x32x8_bf16 over 2 x 2 tiles of 32 (units x items), each operand
+// split into bf16 high and low parts (a b ~ ah bh + ah bl + al bh, three
+// products per step, FP32 accumulation: about 2^-15 relative error per
+// product).  Operand layout of the instruction (lane l, j = 0..3):
+// A[i = l % 32][k = 4 (l / 32) + j], B[k = 4 (l / 32) + j][n = l % 32];
+// result D[8 (r / 4) + 4 (l / 32) + r % 4][l % 32], r = 0..15.  Lane l feeds
+// item n = 32 t + l % 32 of tile t: its own item or that of lane l ^ 32, whose
+// features it takes by a shuffle.  The first layer's result tile (units
+// 32 m + 8 (r / 4) + 4 (l / 32) + r % 4) is, after bias and ReLU, exactly the
+// B operand of the second layer's step s = 4 m + r / 4, so no data moves
+// between the layers.  The weights sit in LDS transposed, split into their two
+// bf16 parts.  The output layer sums each item's 64 units over the lane pair
+// (l, l ^ 32).
 __global__ __launch_bounds__(256) void item_score(ItemArgs a) {
-  __shared__ float w1[IN * H], b1[H], b2[H], w3[H], mu[IN], sd[IN];
+  __shared__ float b1[H], b2[H], w3[H], mu[IN], sd[IN];
   __shared__ float b3;
-  __shared__ __align__(16) short w2h[H][H + 4], w2l[H][H + 4];  // W2^T: [output unit][input unit]
+  __shared__ __align__(16) short w1h[H][IN + 4], w1l[H][IN + 4];  // W1^T: [unit][input]
+  __shared__ __align__(16) short w2h[H][H + 4], w2l[H][H + 4];    // W2^T: [output unit][input unit]
   if (a.P) {
     const float* P = a.P;
     constexpr int OW1 = 0, OB1 = OW1 + IN * H, OW2 = OB1 + H, OB2 = OW2 + H * H, OW3 = OB2 + H, OB3 = OW3 + H;
-    for (int i = threadIdx.x; i < IN * H; i += blockDim.x) w1[i] = P[OW1 + i];
+    for (int i = threadIdx.x; i < IN * H; i += blockDim.x) {
+      const float w = P[OW1 + i];  // w1[input i / H][unit i % H]
+      const short hi = bf16_rne(w);
+      w1h[i % H][i / H] = hi;
+      w1l[i % H][i / H] = bf16_rne(w - bf16_val(hi));
+    }
     for (int i = threadIdx.x; i < H * H; i += blockDim.x) {
       const float w = P[OW2 + i];  // w2[input i / H][output i % H]
       const short hi = bf16_rne(w);
@@ -146,6 +164,32 @@ __global__ __launch_bounds__(256) void item_score(ItemArgs a) {
   float xp[IN];  // features of lane l ^ 32's item
 #pragma unroll
   for (int q = 0; q < IN; ++q) xp[q] = __shfl_xor(x[q], 32, 64);
+  // first layer: B operand of tile t = features 4 half + j of item 32 t + l % 32
+  bf16x4 xh[2], xl[2];
+#pragma unroll
+  for (int t = 0; t < 2; ++t)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const float v = t == half ? (half ? x[4 + j] : x[j]) : (half ? xp[4 + j] : xp[j]);
+      const short hi = bf16_rne(v);
+      xh[t][j] = hi;
+      xl[t][j] = bf16_rne(v - bf16_val(hi));
+    }
+  f32x16 z1[2][2];
+#pragma unroll
+  for (int m = 0; m < 2; ++m) {
+    const bf16x4 ah = *(const bf16x4*)&w1h[32 * m + col][4 * half];
+    const bf16x4 al = *(const bf16x4*)&w1l[32 * m + col][4 * half];
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      f32x16 c;
+#pragma unroll
+      for (int r = 0; r < 16; ++r) c[r] = 0.f;
+      c = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(ah, xh[t], c, 0, 0, 0);
+      c = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(ah, xl[t], c, 0, 0, 0);
+      z1[m][t] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(al, xh[t], c, 0, 0, 0);
+    }
+  }
   f32x16 acc[2][2];
 #pragma unroll
   for (int m = 0; m < 2; ++m)
@@ -153,23 +197,20 @@ __global__ __launch_bounds__(256) void item_score(ItemArgs a) {
     for (int t = 0; t < 2; ++t)
 #pragma unroll
       for (int r = 0; r < 16; ++r) acc[m][t][r] = 0.f;
-#pragma unroll 2
+#pragma unroll
   for (int s = 0; s < H / 8; ++s) {
-    const int k0 = 8 * s + 4 * half;
+    const int k0 = 8 * s + 4 * half;  // this lane's input units of step s
     bf16x4 bh[2], bl[2];
 #pragma unroll
-    for (int t = 0; t < 2; ++t) {
+    for (int t = 0; t < 2; ++t)
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        float z = b1[k0 + j];
-#pragma unroll
-        for (int q = 0; q < IN; ++q) z = fmaf(t == half ? x[q] : xp[q], w1[q * H + k0 + j], z);
+        float z = z1[s / 4][t][4 * (s % 4) + j] + b1[k0 + j];
         z = z > 0.f ? z : 0.f;
         const short hi = bf16_rne(z);
         bh[t][j] = hi;
         bl[t][j] = bf16_rne(z - bf16_val(hi));
       }
-    }
 #pragma unroll
     for (int m = 0; m < 2; ++m) {
       const bf16x4 ah = *(const bf16x4*)&w2h[32 * m + col][k0];
@@ -201,7 +242,7 @@ __global__ __launch_bounds__(256) void item_score(ItemArgs a) {
   if (!a.key) return;
   const u32 kk = score_key(y);
   a.key[i] = kk;
-  if (a.sample && i % SSTEP == 0) a.sample[i / SSTEP] = kk;
+  if (a.sample && (i & (a.sstep - 1)) == 0) a.sample[i / a.sstep] = kk;
 }
 
 __global__ void fill_u32(u32* p, u32 n, u32 v) {
@@ -209,49 +250,55 @@ __global__ void fill_u32(u32* p, u32 n, u32 v) {
 }
 
 // Threshold of the selection (one workgroup): the key of rank `at` (0 = the
-// largest) among the n sampled keys, by a most-significant-digit-first radix
-// select over eight 4-bit digits.  Each lane counts its samples per digit in
-// registers (no atomics: the keys cluster, and atomics on one bin would
-// serialise), the waves reduce their counts and one lane picks the digit.
-// 0 when every item is to be gathered.
+// largest) among the n <= 1024 SEL_PER_LANE sampled keys, by a
+// most-significant-digit-first radix select over four 8-bit digits.  Each lane
+// loads its samples once, into registers.  The keys cluster (items alike in
+// their features score alike), so a wave adds its lanes' digits to the LDS
+// histogram one distinct digit at a time (one atomic per digit, for all lanes
+// holding it) instead of one atomic per lane.  0 when every item is to be
+// gathered.
 __global__ __launch_bounds__(1024) void select_threshold(const u32* smp, u32 n, u32 at, int all, u32* thr) {
-  __shared__ u32 wcnt[16][16];  // [wave][digit]
+  __shared__ u32 hist[256];
   __shared__ u32 prefix_s, at_s;
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
   if (all) {
     if (threadIdx.x == 0) *thr = 0;
     return;
   }
+  u32 v[SEL_PER_LANE];
+#pragma unroll
+  for (u32 rd = 0; rd < SEL_PER_LANE; ++rd) {
+    const u32 q = rd * blockDim.x + threadIdx.x;
+    v[rd] = q < n ? smp[q] : 0u;
+  }
   if (threadIdx.x == 0) { prefix_s = 0; at_s = at; }
-  __syncthreads();
-  for (int shift = 28; shift >= 0; shift -= 4) {
+  for (int shift = 24; shift >= 0; shift -= 8) {
+    for (u32 q = threadIdx.x; q < 256; q += blockDim.x) hist[q] = 0;
+    __syncthreads();
     const u32 prefix = prefix_s;
-    const u32 hmask = shift == 28 ? 0u : (0xFFFFFFFFu << (shift + 4));
-    u32 cnt[16];
+    const u32 hmask = shift == 24 ? 0u : (0xFFFFFFFFu << (shift + 8));
 #pragma unroll
-    for (int d = 0; d < 16; ++d) cnt[d] = 0;
-    for (u32 i = threadIdx.x; i < n; i += blockDim.x) {
-      const u32 v = smp[i];
-      const bool in = (v & hmask) == prefix;
-      const u32 dg = (v >> shift) & 15u;
-#pragma unroll
-      for (int d = 0; d < 16; ++d) cnt[d] += (in && dg == (u32)d) ? 1u : 0u;
-    }
-#pragma unroll
-    for (int d = 0; d < 16; ++d) {
-      u32 c = cnt[d];
-      for (int m = 32; m > 0; m >>= 1) c += (u32)__shfl_xor((int)c, m, 64);
-      if (lane == 0) wcnt[wv][d] = c;
+    for (u32 rd = 0; rd < SEL_PER_LANE; ++rd) {
+      const u32 q = rd * blockDim.x + threadIdx.x;
+      bool pend = q < n && (v[rd] & hmask) == prefix;
+      const u32 dg = (v[rd] >> shift) & 255u;
+      for (;;) {
+        const u64 m = __ballot(pend);
+        if (!m) break;
+        const int leader = __ffsll((unsigned long long)m) - 1;
+        const u32 ld = (u32)__shfl((int)dg, leader, 64);
+        const u64 same = __ballot(pend && dg == ld);
+        if (lane == leader) atomicAdd(&hist[ld], (u32)__popcll(same));
+        if (pend && dg == ld) pend = false;
+      }
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-      const int nw = (int)((blockDim.x + 63) / 64);
-      u32 r = at_s, d = 15;
+      // digits from the top: the one whose keys cover rank at_s
+      u32 r = at_s, d = 255;
       for (;; --d) {
-        u32 h = 0;
-        for (int w = 0; w < nw; ++w) h += wcnt[w][d];
-        if (r < h || d == 0) break;
-        r -= h;
+        if (r < hist[d] || d == 0) break;
+        r -= hist[d];
       }
       prefix_s = prefix | (d << shift);
       at_s = r;
@@ -288,23 +335,34 @@ __global__ __launch_bounds__(256) void score_select(const u32* key, u32 n, const
 
 // Sorts the gathered entries best first by rank: the entries are distinct
 // (each holds its item), so an entry's position is the number of entries
-// above it.  Every workgroup stages all entries in LDS and ranks its share
-// (256 entries) against them, with broadcast LDS reads.  out[0] = entries
-// gathered (may exceed the cap), out[1 ..] = the sorted entries.
+// above it.  Every workgroup stages all entries in LDS; RANK_SPLIT lanes rank
+// one entry together, each over a strided share of the entries, and add their
+// counts by shuffles.  out[0] = entries gathered (may exceed the cap),
+// out[1 ..] = the sorted entries.
 constexpr u32 RANK_BLOCK = 256;
+constexpr u32 RANK_SPLIT = 8;
+constexpr u32 RANK_PER_BLOCK = RANK_BLOCK / RANK_SPLIT;
 __global__ __launch_bounds__(RANK_BLOCK) void sort_front(const u32* cnt, const u64* sel, u64* out) {
   __shared__ u64 s[FRONT_CAP];
   const u32 m = *cnt < FRONT_CAP ? *cnt : FRONT_CAP;
   if (blockIdx.x == 0 && threadIdx.x == 0) out[0] = *cnt;
-  const u32 i = blockIdx.x * RANK_BLOCK + threadIdx.x;
-  if (blockIdx.x * RANK_BLOCK >= m) return;
-  for (u32 q = threadIdx.x; q < m; q += RANK_BLOCK) s[q] = sel[q];
+  if (blockIdx.x * RANK_PER_BLOCK >= m) return;
+  // FRONT_CAP / RANK_BLOCK loads per lane, issued together
+#pragma unroll
+  for (u32 r = 0; r < FRONT_CAP / RANK_BLOCK; ++r) {
+    const u32 q = r * RANK_BLOCK + threadIdx.x;
+    if (q < m) s[q] = sel[q];
+  }
   __syncthreads();
-  if (i >= m) return;
-  const u64 e = s[i];
+  const u32 e_idx = blockIdx.x * RANK_PER_BLOCK + threadIdx.x / RANK_SPLIT;
+  const u32 part = threadIdx.x % RANK_SPLIT;
+  const u64 e = e_idx < m ? s[e_idx] : ~0ull;
   u32 rank = 0;
-  for (u32 q = 0; q < m; ++q) rank += s[q] > e ? 1u : 0u;
-  out[1 + rank] = e;
+  for (u32 q = part; q < m; q += RANK_SPLIT) rank += s[q] > e ? 1u : 0u;
+  // the RANK_SPLIT lanes of an entry are consecutive in the wave
+#pragma unroll
+  for (u32 off = RANK_SPLIT / 2; off > 0; off >>= 1) rank += (u32)__shfl_xor((int)rank, (int)off, 64);
+  if (part == 0 && e_idx < m) out[1 + rank] = e;
 }
 
 }  // namespace
@@ -316,7 +374,10 @@ int gmk::learned_front(gmk_ctx* c, u32 a_lo, u32 a_hi, u32 k, u32 want, std::vec
   want = std::min(want, std::min(n, FRONT_CAP / 2));
   hipStream_t s = c->stream;
   int r;
-  const u32 ns = (n + SSTEP - 1) / SSTEP;
+  // sample stride: a power of two, so that at most 1024 SEL_PER_LANE samples
+  u32 sstep = SSTEP_MIN;
+  while ((u64)(n + sstep - 1) / sstep > 1024ull * SEL_PER_LANE) sstep <<= 1;
+  const u32 ns = (n + sstep - 1) / sstep;
   // device: parameters, then (u32) count, threshold, keys, samples; entries;
   // sorted front.  The parameters and the sorted front pass through the
   // context's pinned staging.
@@ -339,17 +400,17 @@ int gmk::learned_front(gmk_ctx* c, u32 a_lo, u32 a_hi, u32 k, u32 want, std::vec
   HIPCHK(c, hipMemcpyAsync(c->sched_p.p, P, (MLP_NPARAM + 2 * IN) * 4, hipMemcpyHostToDevice, s));
   hipLaunchKernelGGL(fill_u32, dim3(1), dim3(64), 0, s, cnt, 4u, 0u);
   ItemArgs a{c->d_off, c->d_adj, a_lo, n, (a_lo == 0 ? c->n - c->n_left : c->n_left), (float)k,
-             (const float*)c->sched_p.p, nullptr, key, smp, nullptr};
+             (const float*)c->sched_p.p, nullptr, key, smp, nullptr, sstep};
   hipLaunchKernelGGL(item_score, dim3((n + 255) / 256), dim3(256), 0, s, a);
   // Threshold: the sampled key ranked at about 1.5 `want` items, so that the
   // gathered set holds the best `want` items with room for the sample's error
   // and stays under FRONT_CAP; every item when there are at most 1.5 `want`.
   const u64 span = (u64)want * 3 / 2;
-  const u32 at = (u32)std::min<u64>(ns - 1, (span + SSTEP - 1) / SSTEP);
+  const u32 at = (u32)std::min<u64>(ns - 1, (span + sstep - 1) / sstep);
   hipLaunchKernelGGL(select_threshold, dim3(1), dim3(1024), 0, s, (const u32*)smp, ns, at, n <= span ? 1 : 0, thr);
   hipLaunchKernelGGL(score_select, dim3((n + SELECT_ITEMS - 1) / SELECT_ITEMS), dim3(256), 0, s, (const u32*)key, n,
                      (const u32*)thr, FRONT_CAP, cnt, sel);
-  hipLaunchKernelGGL(sort_front, dim3(FRONT_CAP / RANK_BLOCK), dim3(RANK_BLOCK), 0, s, (const u32*)cnt,
+  hipLaunchKernelGGL(sort_front, dim3(FRONT_CAP / RANK_PER_BLOCK), dim3(RANK_BLOCK), 0, s, (const u32*)cnt,
                      (const u64*)sel, sorted);
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipMemcpyAsync(hfront, sorted, (size_t)(FRONT_CAP + 1) * 8, hipMemcpyDeviceToHost, s));
