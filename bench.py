@@ -40,7 +40,7 @@ HBM_PEAK_GBS = 8000.0
 # train: (workload, fraction of its tuple-side items) probed to train the item
 # cost model of the learned schedule; None: no learned run (global partition)
 WORKLOADS = {
-    "cl50kx30k": dict(k=1, theta=1, limit=1_000_000, cpu_sample=4000, train=[("cl50kx30k", 0.05)],
+    "cl50kx30k": dict(k=1, theta=1, limit=1_000_000, cpu_sample=4000, train=[("cl50kx30k", 1.0)],
                       desc="Chung-Lu bipartite |U|=50000 |V|=30000 m=500000 alpha=2.1 cap=5000 (seeded), k=1, theta=1, r=1000000"),
     "er200x300": dict(k=1, theta=1, limit=0, cpu_sample=4_397_076, train=None,  # the whole set: about 7 s on one core
                       desc="Erdos-Renyi bipartite 200x300 p=0.05 (seeded), k=1, theta=1, full enumeration"),

@@ -406,6 +406,7 @@ extern "C" int gmk_enumerate(gmk_ctx* c, const gmk_params* prm, gmk_stats* st) {
   st->ms_reduce = rs.ms_reduce;
   st->bytes_reduce = rs.bytes_reduce;
   st->reduce_removed = rs.reduce_removed;
+  st->reduce_edges = rs.reduce_edges;
   st->hit_limit = rs.hit_limit || h.res_count >= rs.limit;
   st->ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   rs.progress(st->count);
@@ -497,7 +498,7 @@ extern "C" int gmk_enumerate_batch(gmk_ctx* c, const gmk_graph_desc* graphs, uin
     // global partitions of every width in one run; the first graph of the
     // schedule is expanded first
     if (!global_graphs.empty()) {
-      std::vector<PartitionSet> sets(7);
+      std::vector<PartitionSet> sets(NLANE);
       size_t rows = 0, maps = 0, parts = 0;
       for (uint32_t i : global_graphs) {
         const gmk_graph_desc& g = graphs[i];

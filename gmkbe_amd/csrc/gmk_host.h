@@ -26,7 +26,8 @@ struct RunState {
   double ms_schedule = 0;   // host time spent ordering items (learned runs)
   double ms_reduce = 0;     // core reduction before the seed partitions: device time,
   u64 bytes_reduce = 0;     //   algorithmic bytes,
-  u64 reduce_removed = 0;   //   vertices removed and peeling rounds
+  u64 reduce_removed = 0;   //   vertices removed and peeling rounds (or passes)
+  u64 reduce_edges = 0;     //   edges removed by the butterfly reduction
   u64 reduce_rounds = 0;
   unsigned long long* gcount = nullptr;  // per-graph caps (multi-graph batches)
   u64* gfp = nullptr;
@@ -69,6 +70,9 @@ struct CoreGraph {
   u32 removed = 0;
 };
 int core_reduce(gmk_ctx* c, u32 thr, RunState& rs, CoreGraph& out);
+// Butterfly reduction of a reduced graph's device CSR (vertex IDs kept): the
+// vertices and edges in no k-biplex with both sides >= q leave.
+int butterfly_reduce(gmk_ctx* c, const CoreGraph& in, u32 q, u32 k, RunState& rs, CoreGraph& out);
 
 // Whole graph as one partition (local IDs = global IDs, right side shifted).
 void build_global_partition(u32 n, u32 n_left, const u32* off, const u32* adj, u32 k, u32 theta, int W, PartitionSet& ps,
@@ -159,6 +163,7 @@ struct gmk_ctx {
   gmk_mlp item_model{};
   DeviceVec sched_p, sched_hist, sched_sel;  // learned schedule: model parameters, score keys, selection
   DeviceVec core_deg, core_dead, core_front, core_off, core_adj;  // core reduction of seeded runs
+  DeviceVec bfly_cnt, bfly_mark, bfly_deg, bfly_off, bfly_adj;    // butterfly reduction of seeded runs
   float* scratch = nullptr;      // device address of the library's static array on this device (gmk_item_predict_static)
   float* h_stage = nullptr;      // pinned host staging of that array's contents
 };

@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <string>
 
+#include <cstdlib>
 #include <cstring>
 
 namespace gmk {
@@ -95,12 +96,20 @@ int gmk_run_seeded(gmk_ctx* c, const gmk_params* prm, gmk_stats* st, RunState& r
   const u32 lb = std::max(theta, 2 * k + 1);
   // the fat results live in the (lb - k)-core (BPPivot's core reduction, applied
   // when lb > k: biplex.cpp:441-450); it is built on the device
-  CoreGraph core;
+  CoreGraph core, bfly;
   if ((r = core_reduce(c, lb - k, rs, core))) return r;
-  HostGraph g{c->n, c->n_left, core.off.data(), core.adj.data()};
+  // then BPPivot's butterfly reduction, applied when lb > 2k (biplex.cpp:469-478),
+  // as always here; GMK_BUTTERFLY=0 leaves it out
+  const CoreGraph* red = &core;
+  const char* bf = getenv("GMK_BUTTERFLY");
+  if (!bf || atoi(bf) != 0) {
+    if ((r = butterfly_reduce(c, core, lb, k, rs, bfly))) return r;
+    red = &bfly;
+  }
+  HostGraph g{c->n, c->n_left, red->off.data(), red->adj.data()};
   std::vector<SeedJob> jobs;
   jobs.push_back(make_seed_job(g, k, lb, 0));
-  jobs[0].d_off = core.d_off;
-  jobs[0].d_adj = core.d_adj;
+  jobs[0].d_off = red->d_off;
+  jobs[0].d_adj = red->d_adj;
   return run_seed_jobs(c, jobs, prm, st, rs, nullptr);
 }

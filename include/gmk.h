@@ -99,13 +99,14 @@ typedef struct {
                                 fingerprint had not yet shown its check word (dedup) */
   double ms_schedule;        /* GMK_MODE_LEARNED: host time of the item ordering (scoring
                                 kernel, selection, sort), inside the window `ms` */
-  double ms_reduce;          /* seeded runs: device time of the (lb-k)-core reduction
-                                before the seed partitions */
+  double ms_reduce;          /* seeded runs: device time of the (lb-k)-core reduction and
+                                the butterfly reduction before the seed partitions */
   uint64_t bytes_reduce;     /* its algorithmic bytes */
-  uint64_t reduce_removed;   /* vertices it removed */
+  uint64_t reduce_removed;   /* vertices the reductions removed */
   uint64_t widest_side;      /* most vertices on one side of any branch-and-bound partition
                                 (up to 8192 per side a state's working set is in LDS, up
                                 to 65536 in HBM) */
+  uint64_t reduce_edges;     /* edges the butterfly reduction removed */
 } gmk_stats;
 
 /* Create a context on HIP device `device` (-d, src/main.cu:1765-1766). */

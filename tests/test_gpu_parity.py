@@ -1,5 +1,7 @@
 """GPU parity: the HIP enumerator against the reference BPPivot's golden sets,
 brute force and the oracle, through the C ABI."""
+import os
+
 import numpy as np
 import pytest
 
@@ -102,7 +104,7 @@ def test_dedup_table_finds_no_repeats(eng, partition):
     assert res.stats["dedup_probes"] >= res.stats["dedup_inserts"]
 
 
-def _core_removed(g, alpha):
+def _core_dead(g, alpha):
     """Vertices outside the (alpha, alpha)-core: repeatedly remove vertices of
     degree < alpha (numpy restatement of BPPivot's coreReduction,
     biplex.cpp:123-174)."""
@@ -117,7 +119,56 @@ def _core_removed(g, alpha):
         new_dead = dead | (live_deg < alpha)
         changed = bool((new_dead != dead).any())
         dead = new_dead
-    return int(dead.sum())
+    return dead
+
+
+def _butterfly(g, dead, q, k):
+    """The device butterfly reduction restated on the host: BPPivot's rule
+    (biplex.cpp:176-225) in passes over one side at a time, every vertex of
+    the side decided against the removals of the previous pass, two rounds
+    over both sides (the second skipped when the first removed nothing).
+    Returns (vertices removed, edges removed between surviving vertices)."""
+    nbr = [[int(w) for w in g.adj[g.off[v]:g.off[v + 1]] if not dead[int(w)]] if not dead[v] else []
+           for v in range(g.n)]
+    deg = [len(x) for x in nbr]
+    rm = [d == 0 for d in deg]
+    cut = set()
+    for _ in range(2):
+        any_change = False
+        for lo, hi in ((0, g.n_left), (g.n_left, g.n)):
+            old, new = list(rm), list(rm)
+            for u in range(lo, hi):
+                if old[u]:
+                    continue
+                cn = {}
+                for v in nbr[u]:
+                    if not old[v]:
+                        for w in nbr[v]:
+                            if not old[w]:
+                                cn[w] = cn.get(w, 0) + 1
+                support = 0
+                for v in nbr[u]:
+                    if old[v]:
+                        continue
+                    c = sum(1 for w in nbr[v] if not old[w] and cn.get(w, 0) >= q - 2 * k)
+                    if c >= q - k:
+                        support += 1
+                    else:
+                        cut.add((min(u, v), max(u, v)))
+                        any_change = True
+                if support < q - k:
+                    new[u] = True
+                    any_change = True
+                    for v in nbr[u]:
+                        deg[v] -= 1
+                        if deg[v] == q - k - 1:
+                            new[v] = True
+            rm = new
+        if not any_change:
+            break
+    removed = sum(1 for v in range(g.n) if rm[v] and nbr[v])
+    edges = sum(1 for u, v in cut if not rm[u] and not rm[v])
+    return removed, edges
 
 
 @pytest.mark.parametrize("name,k,theta", [("cl600x400", 1, 3), ("cl600x400", 2, 9), ("moreno_crime", 1, 3),
@@ -125,13 +176,36 @@ def _core_removed(g, alpha):
 def test_core_reduction_removes_the_core_complement(eng, name, k, theta):
     """Seeded runs peel the (lb - k)-core on the device before seeding: the
     count of removed vertices equals a host peel's, and the set is still the
-    reference's."""
+    reference's (butterfly reduction off)."""
     c = [c for c in GC.cases() if c["graph"] == name and c["k"] == k and c["theta"] == theta][0]
     g = GC.graph(name)
     eng.set_graph(g)
     print(f"[gpu] core reduction {name} k={k} theta={theta}", flush=True)
+    os.environ["GMK_BUTTERFLY"] = "0"
+    try:
+        res = eng.enumerate(k=k, theta=theta, partition="seeded", mode="batched")
+    finally:
+        os.environ.pop("GMK_BUTTERFLY", None)
+    lb = max(theta, 2 * k + 1)
+    assert res.stats["reduce_removed"] == int(_core_dead(g, lb - k).sum())
+    assert res.stats["ms_reduce"] > 0
+    _check_set(g, k, theta, res, expect_case=c)
+
+
+@pytest.mark.parametrize("name,k,theta", [("cl600x400", 1, 3), ("cl600x400", 2, 9), ("moreno_crime", 1, 3),
+                                          ("er200x300", 2, 5), ("er200x300", 1, 3), ("southern_women", 1, 3)])
+def test_butterfly_reduction_matches_host_restatement(eng, name, k, theta):
+    """After the core, seeded runs apply the butterfly reduction on the device:
+    the vertices and edges it removes equal the host restatement's, and the
+    set is still the reference's."""
+    c = [c for c in GC.cases() if c["graph"] == name and c["k"] == k and c["theta"] == theta][0]
+    g = GC.graph(name)
+    eng.set_graph(g)
+    print(f"[gpu] butterfly reduction {name} k={k} theta={theta}", flush=True)
     res = eng.enumerate(k=k, theta=theta, partition="seeded", mode="batched")
     lb = max(theta, 2 * k + 1)
-    assert res.stats["reduce_removed"] == _core_removed(g, lb - k)
-    assert res.stats["ms_reduce"] > 0
+    dead = _core_dead(g, lb - k)
+    removed, edges = _butterfly(g, dead, lb, k)
+    assert res.stats["reduce_removed"] == int(dead.sum()) + removed
+    assert res.stats["reduce_edges"] == edges
     _check_set(g, k, theta, res, expect_case=c)
