@@ -10,7 +10,7 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(PKG, "csrc")
 OUT_DIR = os.path.join(PKG, "_lib")
 LIB = os.path.join(OUT_DIR, "libgmk.so")
-SOURCES = ["bb_kernel.hip", "bb_run.hip", "partition_kernels.hip", "gmk_engine.hip", "partition.hip", "thin_kernels.hip", "thin_host.hip", "seeds.cpp", "mlp.hip", "schedule.hip", "reduce.hip", "edge_parse.cpp"]
+SOURCES = ["bb_kernel.hip", "bb_run.hip", "partition_kernels.hip", "gmk_engine.hip", "partition.hip", "thin_kernels.hip", "thin_host.hip", "seeds.cpp", "mlp.hip", "schedule.hip", "reduce.hip", "seed_kernels.hip", "edge_parse.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # -packed-fp32-ops: keep the compiler from emitting packed FP32 instructions.
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++20", "-fPIC", "-Wall", "-Wno-unused-function",

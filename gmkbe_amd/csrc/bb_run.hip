@@ -44,6 +44,7 @@ struct RowBuildArgs {
   u32 max_words;        // widest row of the partitions built
 };
 hipError_t launch_build_part_rows(const RowBuildArgs& a, hipStream_t s);
+hipError_t launch_seed_sets(const SeedArgs& a, u32 grid, hipStream_t s);
 
 namespace {
 int lane_of(int W) {
@@ -628,6 +629,124 @@ void build_tasks(std::vector<SeedJob>& jobs, const std::vector<BuildTask>& tasks
 }
 }  // namespace
 
+// Seed partitions' vertex sets built on the device (seed_kernels.hip), one
+// workgroup per seed; the host then places each partition in the tables as
+// before.  A seed whose lists do not fit the output arena is built on the host.
+class DeviceSeeds {
+ public:
+  static constexpr u64 kArenaWords = 1ull << 24;
+  static constexpr u32 kSideCap = 65536;
+  DeviceSeeds(gmk_ctx* c) : c_(c) {}
+  // CSR source (as the row builder's) and the jobs' degeneracy orders, laid out
+  // like the offsets (graph g's order at its offsets' base)
+  int setup(const std::vector<SeedJob>& jobs, const u32* off, const u32* adj, const u64* off_base,
+            const u64* adj_base, const std::vector<u64>& base_h, size_t order_words, u32 ngraphs) {
+    off_ = off; adj_ = adj; off_base_ = off_base; adj_base_ = adj_base;
+    int r;
+    n_max_ = 1;
+    std::vector<u32> nv(ngraphs, 0), nl(ngraphs, 0), ord(std::max<size_t>(order_words, 1), 0);
+    for (const SeedJob& j : jobs) {
+      n_max_ = std::max(n_max_, j.g.n);
+      nv[j.graph] = j.g.n;
+      nl[j.graph] = j.g.n_left;
+      std::copy(j.order.begin(), j.order.end(), ord.begin() + base_h[j.graph]);
+    }
+    if (n_max_ >= (1u << 30)) return GMK_EUNSUPPORTED;  // the flag bits of the output
+    if ((r = ensure(c_, c_->seed_order, ord.size() * 4))) return r;
+    if ((r = ensure(c_, c_->seed_meta, (size_t)2 * ngraphs * 4 + 64))) return r;
+    grid_ = (u32)std::max<u64>(1, std::min<u64>(2ull * c_->n_cu, (1ull << 30) / (32ull * n_max_)));
+    if ((r = ensure(c_, c_->seed_scratch, (size_t)grid_ * 8 * n_max_ * 4))) return r;
+    if ((r = ensure(c_, c_->seed_arena, kArenaWords * 4))) return r;
+    hipStream_t s = c_->stream;
+    HIPCHK(c_, hipMemcpyAsync(c_->seed_order.p, ord.data(), ord.size() * 4, hipMemcpyHostToDevice, s));
+    HIPCHK(c_, hipMemcpyAsync(c_->seed_meta.p, nv.data(), (size_t)ngraphs * 4, hipMemcpyHostToDevice, s));
+    HIPCHK(c_, hipMemcpyAsync((u32*)c_->seed_meta.p + ngraphs, nl.data(), (size_t)ngraphs * 4, hipMemcpyHostToDevice, s));
+    HIPCHK(c_, hipMemsetAsync(c_->seed_scratch.p, 0, (size_t)grid_ * 8 * n_max_ * 4, s));
+    ngraphs_ = ngraphs;
+    return GMK_OK;
+  }
+
+  int build(std::vector<SeedJob>& jobs, const std::vector<BuildTask>& tasks, std::vector<BuildOut>& outs,
+            Worker& w, RunState& rs) {
+    outs.clear();
+    outs.resize(tasks.size());
+    if (tasks.empty()) return GMK_OK;
+    hipStream_t s = c_->stream;
+    int r;
+    const size_t n = tasks.size();
+    std::vector<SeedTask> tk(n);
+    for (size_t i = 0; i < n; ++i) {
+      const SeedJob& j = jobs[tasks[i].job];
+      tk[i] = SeedTask{j.graph, tasks[i].seed, j.k, j.lb};
+    }
+    if ((r = ensure(c_, c_->seed_tasks, n * sizeof(SeedTask) + 64))) return r;
+    if ((r = ensure(c_, c_->seed_out, n * sizeof(SeedOut) + 64))) return r;
+    unsigned long long* used = (unsigned long long*)((char*)c_->seed_tasks.p + n * sizeof(SeedTask));
+    used = (unsigned long long*)(((uintptr_t)used + 15) & ~(uintptr_t)15);
+    HIPCHK(c_, hipMemcpyAsync(c_->seed_tasks.p, tk.data(), n * sizeof(SeedTask), hipMemcpyHostToDevice, s));
+    HIPCHK(c_, hipMemsetAsync(used, 0, 16, s));
+    SeedArgs a{(const SeedTask*)c_->seed_tasks.p, (u32)n, off_, adj_, off_base_, adj_base_, (const u32*)c_->seed_order.p,
+               (const u32*)c_->seed_meta.p, (const u32*)c_->seed_meta.p + ngraphs_, n_max_,
+               (u32*)c_->seed_scratch.p, (u32*)c_->seed_arena.p, kArenaWords, used, (SeedOut*)c_->seed_out.p,
+               kSideCap, (unsigned long long*)&((RunCounters*)c_->counters.p)->stat[3 + 3]};
+    HIPCHK(c_, hipEventRecord(c_->ev2, s));
+    HIPCHK(c_, launch_seed_sets(a, (u32)std::min<size_t>(grid_, n), s));
+    HIPCHK(c_, hipEventRecord(c_->ev3, s));
+    std::vector<SeedOut> o(n);
+    unsigned long long u[2] = {0, 0};
+    HIPCHK(c_, hipMemcpyAsync(o.data(), c_->seed_out.p, n * sizeof(SeedOut), hipMemcpyDeviceToHost, s));
+    HIPCHK(c_, hipMemcpyAsync(u, used, 16, hipMemcpyDeviceToHost, s));
+    HIPCHK(c_, hipStreamSynchronize(s));
+    const u64 words = std::min<u64>(u[0], kArenaWords);
+    arena_.resize(std::max<u64>(words, 1));
+    if (words) {
+      HIPCHK(c_, hipMemcpyAsync(arena_.data(), c_->seed_arena.p, words * 4, hipMemcpyDeviceToHost, s));
+      HIPCHK(c_, hipStreamSynchronize(s));
+    }
+    float ms = 0;
+    HIPCHK(c_, hipEventElapsedTime(&ms, c_->ev2, c_->ev3));
+    rs.ms_kernel[3] += ms;  // partition construction
+    rs.launches_kernel[3] += 1;
+    rs.launches += 1;
+    for (size_t i = 0; i < n; ++i) {
+      SeedJob& job = jobs[tasks[i].job];
+      BuildOut& bo = outs[i];
+      if (o[i].status == 0) continue;
+      if (o[i].status == 3) { bo.too_wide = true; continue; }
+      if (o[i].status == 2) {
+        // output arena full: this seed is built on the host
+        if (!build_seed_partition(job.g, job.order, tasks[i].seed, job.k, job.lb, w.sp, w.cnt, w.mark)) continue;
+      } else {
+        SeedPartition& sp = w.sp;
+        sp.seed = tasks[i].seed;
+        sp.us = (int)o[i].us;
+        const u32 ns = o[i].us ? o[i].nr : o[i].nl, no = o[i].us ? o[i].nl : o[i].nr;
+        const u32* same = arena_.data() + o[i].at;
+        const u32* other = same + ns;
+        std::vector<u32>& vs = sp.us ? sp.right : sp.left;
+        std::vector<uint8_t>& fs = sp.us ? sp.rflag : sp.lflag;
+        std::vector<u32>& vo = sp.us ? sp.left : sp.right;
+        std::vector<uint8_t>& fo = sp.us ? sp.lflag : sp.rflag;
+        vs.resize(ns);
+        fs.resize(ns);
+        for (u32 q = 0; q < ns; ++q) { vs[q] = same[q] & 0x3FFFFFFFu; fs[q] = (uint8_t)(same[q] >> 30); }
+        vo.assign(other, other + no);
+        fo.assign(no, 0);
+      }
+      if (make_built(job, w.sp, w.loc, bo.part, true)) bo.ok = true;
+      else bo.too_wide = true;
+    }
+    return GMK_OK;
+  }
+
+ private:
+  gmk_ctx* c_;
+  const u32 *off_ = nullptr, *adj_ = nullptr;
+  const u64 *off_base_ = nullptr, *adj_base_ = nullptr;
+  u32 n_max_ = 1, grid_ = 1, ngraphs_ = 1;
+  std::vector<u32> arena_;
+};
+
 }  // namespace gmk
 
 // Runs the seed partitions of the jobs (seeds of each job in degeneracy order).
@@ -640,6 +759,8 @@ int run_seed_jobs(gmk_ctx* c, std::vector<SeedJob>& jobs, const gmk_params* prm,
                   double* ms_per_job) {
   if (jobs.empty()) return GMK_OK;  // (a batch whose graphs all run as global partitions)
   const bool plain = prm->mode == GMK_MODE_PLAIN;
+  std::vector<BuildTask> tasks;
+  std::vector<BuildOut> outs;
   BBEngine eng(c, prm, rs, !plain);
   int r;
   if ((r = eng.reserve((size_t)1 << 27, (size_t)1 << 25, (size_t)1 << 21))) return r;  // 1 GiB of rows
@@ -683,11 +804,27 @@ int run_seed_jobs(gmk_ctx* c, std::vector<SeedJob>& jobs, const gmk_params* prm,
     eng.set_row_source((const u32*)c->bcsr_off.p, (const u32*)c->bcsr_adj.p, (const u64*)c->bcsr_base.p,
                        (const u64*)c->bcsr_base.p + ngraphs);
   }
+  // vertex sets on the device (GMK_DEVICE_SEEDS=0: on host threads)
+  DeviceSeeds dev(c);
+  bool dev_on = true;
+  if (const char* e = getenv("GMK_DEVICE_SEEDS")) dev_on = atoi(e) != 0;
+  if (dev_on) {
+    const int rr = resident ? dev.setup(jobs, jobs[0].d_off, jobs[0].d_adj, (const u64*)c->bcsr_base.p,
+                                        (const u64*)c->bcsr_base.p + 1, std::vector<u64>(2, 0), jobs[0].g.n + 1, 1)
+                            : dev.setup(jobs, (const u32*)c->bcsr_off.p, (const u32*)c->bcsr_adj.p,
+                                        (const u64*)c->bcsr_base.p, (const u64*)c->bcsr_base.p + ngraphs, base,
+                                        coff.size(), ngraphs);
+    if (rr == GMK_EUNSUPPORTED) dev_on = false;
+    else if (rr) return rr;
+  }
+  auto build = [&]() -> int {
+    if (dev_on) return dev.build(jobs, tasks, outs, workers[0], rs);
+    build_tasks(jobs, tasks, outs, workers, true);
+    return GMK_OK;
+  };
   std::vector<u64> gcnt_h(ngraphs, 0);
   auto capped = [&](const SeedJob& j) { return rs.gcount && gcnt_h[j.graph] >= rs.gcap; };
   auto done = [&](const SeedJob& j) { return j.next >= j.seeds.size() || capped(j); };
-  std::vector<BuildTask> tasks;
-  std::vector<BuildOut> outs;
   auto too_wide = [&](u32 seed) {
     c->err = "seed partition of vertex " + std::to_string(seed) + " has a side above 65536 vertices";
     return GMK_EUNSUPPORTED;
@@ -699,7 +836,9 @@ int run_seed_jobs(gmk_ctx* c, std::vector<SeedJob>& jobs, const gmk_params* prm,
       const double t0 = rs.ms_kernel[0];
       while (!done(j) && !rs.hit_limit) {
         tasks.assign(1, BuildTask{(u32)ji, j.seeds[j.next++]});
-        build_tasks(jobs, tasks, outs, workers, true);
+        const auto tb = std::chrono::steady_clock::now();
+        if ((r = build())) return r;
+        rs.ms_build += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tb).count();
         if (outs[0].too_wide) return too_wide(tasks[0].seed);
         if (!outs[0].ok) continue;
         ++st->partitions;
@@ -743,7 +882,11 @@ int run_seed_jobs(gmk_ctx* c, std::vector<SeedJob>& jobs, const gmk_params* prm,
         for (size_t q = 0; q < take; ++q) tasks.push_back(BuildTask{(u32)ji, j.seeds[j.next++]});
         quota[ji] = std::min<u32>(quota[ji] * 2, 1u << 16);
       }
-      if (!tasks.empty()) build_tasks(jobs, tasks, outs, workers, true);
+      if (!tasks.empty()) {
+        const auto tb = std::chrono::steady_clock::now();
+        if ((r = build())) return r;
+        rs.ms_build += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tb).count();
+      }
       for (size_t i = 0; i < tasks.size(); ++i) {
         BuildOut& o = outs[i];
         if (o.too_wide) return too_wide(tasks[i].seed);

@@ -153,6 +153,40 @@ struct ExpandArgs {
   const LaneCtl* ctl;          // device-planned steps: batch size, position and child room (null: n_in, out_cap)
 };
 
+// Seed partitions' vertex sets on the device (seed_kernels.hip).
+struct SeedTask {
+  u32 graph;   // graph of the batch (index into the bases)
+  u32 seed;    // local vertex ID in that graph
+  u32 k, lb;
+};
+
+struct SeedOut {
+  u32 status;  // 0 dropped, 1 built, 2 output arena full (the host builds it), 3 a side above the cap
+  u32 nl, nr;  // sides (left, right)
+  u32 us;      // side of the seed
+  u64 at;      // offset of its lists in the arena: the seed's side, then the other
+};
+
+struct SeedArgs {
+  const SeedTask* tasks;
+  u32 n_tasks;
+  const u32* off;        // CSRs of all graphs, concatenated
+  const u32* adj;
+  const u64* off_base;   // per graph: start of its offsets and lists
+  const u64* adj_base;
+  const u32* order;      // degeneracy positions of all graphs, at off_base
+  const u32* n_vert;     // per graph: vertices
+  const u32* n_left;     // per graph: left vertices
+  u32 n_max;             // most vertices of any graph (slice stride)
+  u32* scratch;          // gridDim.x slices of 8 n_max words (marks, counts, lists, sort)
+  u32* arena;
+  u64 arena_cap;
+  unsigned long long* arena_used;
+  SeedOut* out;
+  u32 side_cap;          // most vertices per side of a partition (65536)
+  unsigned long long* bytes;
+};
+
 __device__ __forceinline__ void wave_barrier() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
   __builtin_amdgcn_wave_barrier();

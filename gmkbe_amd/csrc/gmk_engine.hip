@@ -80,7 +80,9 @@ extern "C" void gmk_destroy(gmk_ctx* c) {
                         &c->prefix, &c->tcnt, &c->tdefer, &c->ridx, &c->tcount, &c->nh_off, &c->nh, &c->nh_verts, &c->dedup, &c->gcnt,
                         &c->bcsr_off, &c->bcsr_adj, &c->bcsr_base, &c->rowpids, &c->sched_p, &c->sched_hist,
                         &c->sched_sel, &c->core_deg, &c->core_dead, &c->core_front, &c->core_off, &c->core_adj,
-                        &c->wide_scratch, &c->lane_ctl})
+                        &c->wide_scratch, &c->lane_ctl, &c->bfly_cnt, &c->bfly_mark, &c->bfly_deg, &c->bfly_off,
+                        &c->bfly_adj, &c->seed_order, &c->seed_meta, &c->seed_scratch, &c->seed_arena, &c->seed_tasks,
+                        &c->seed_out})
     if (v->p) (void)hipFree(v->p);
   for (int l = 0; l < NLANE; ++l)
     for (DeviceVec* v : {&c->lane_stack[l], &c->lane_next[l], &c->lane_retry[l], &c->lane_spill[l]})
@@ -407,6 +409,7 @@ extern "C" int gmk_enumerate(gmk_ctx* c, const gmk_params* prm, gmk_stats* st) {
   st->bytes_reduce = rs.bytes_reduce;
   st->reduce_removed = rs.reduce_removed;
   st->reduce_edges = rs.reduce_edges;
+  st->ms_build = rs.ms_build;
   st->hit_limit = rs.hit_limit || h.res_count >= rs.limit;
   st->ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   rs.progress(st->count);
@@ -545,6 +548,7 @@ extern "C" int gmk_enumerate_batch(gmk_ctx* c, const gmk_graph_desc* graphs, uin
   st->launches = rs.launches;
   st->spills = rs.spills;
   st->ms_expand = rs.ms_expand;
+  st->ms_build = rs.ms_build;
   st->ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   c->prog_count = std::move(rs.prog_count);
   c->prog_ms = std::move(rs.prog_ms);

@@ -28,6 +28,7 @@ struct RunState {
   u64 bytes_reduce = 0;     //   algorithmic bytes,
   u64 reduce_removed = 0;   //   vertices removed and peeling rounds (or passes)
   u64 reduce_edges = 0;     //   edges removed by the butterfly reduction
+  double ms_build = 0;      // host time building seed partitions' vertex sets
   u64 reduce_rounds = 0;
   unsigned long long* gcount = nullptr;  // per-graph caps (multi-graph batches)
   u64* gfp = nullptr;
@@ -164,6 +165,7 @@ struct gmk_ctx {
   DeviceVec sched_p, sched_hist, sched_sel;  // learned schedule: model parameters, score keys, selection
   DeviceVec core_deg, core_dead, core_front, core_off, core_adj;  // core reduction of seeded runs
   DeviceVec bfly_cnt, bfly_mark, bfly_deg, bfly_off, bfly_adj;    // butterfly reduction of seeded runs
+  DeviceVec seed_order, seed_meta, seed_scratch, seed_arena, seed_tasks, seed_out;  // seed vertex sets on the device
   float* scratch = nullptr;      // device address of the library's static array on this device (gmk_item_predict_static)
   float* h_stage = nullptr;      // pinned host staging of that array's contents
 };

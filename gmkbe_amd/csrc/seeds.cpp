@@ -89,7 +89,11 @@ bool build_seed_partition(const HostGraph& g, const std::vector<uint32_t>& order
       }
       for (uint32_t x : XS)
         if (mark[x] == M_XS && (int64_t)cnt[x] < need_us) { mark[x] = M_DEAD; changed = true; }
-      if (!changed) return true;
+      // the seed needs lb - k neighbours among the surviving C_os (it misses at
+      // most k of a side of lb or more); BPPivot's pruneCX tests this only when
+      // one of them leaves (biplex.cpp:240-258), so a seed that starts short
+      // of it kept a partition that holds no result
+      if (!changed) return (int64_t)cnt[s] >= need_os;
     }
   };
   alive = prune();

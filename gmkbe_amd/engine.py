@@ -43,7 +43,7 @@ class Stats(ctypes.Structure):
                 ("bytes_kernel", ctypes.c_uint64 * 4), ("dedup_undecided", ctypes.c_uint64),
                 ("ms_schedule", ctypes.c_double), ("ms_reduce", ctypes.c_double),
                 ("bytes_reduce", ctypes.c_uint64), ("reduce_removed", ctypes.c_uint64),
-                ("widest_side", ctypes.c_uint64), ("reduce_edges", ctypes.c_uint64)]
+                ("widest_side", ctypes.c_uint64), ("reduce_edges", ctypes.c_uint64), ("ms_build", ctypes.c_double)]
 
     KERNELS = ("bb_expand", "thin_build_families", "thin_check", "partition_build")
 
