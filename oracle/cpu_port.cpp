@@ -36,13 +36,15 @@ struct HostFams {
   Family* next() { f.emplace_back(); return &f.back(); }
 };
 constexpr u64 CHUNK = 4096;  // ranks per work unit
+constexpr u64 GROUP = 1024;  // work units per parallel loop
 }  // namespace
 
 // Returns the results found (at most `limit`, 0 = no limit) with `threads`
-// threads (0: all); *seconds receives the enumeration time and *ranks the
-// combinations tested.
+// threads (0: all), stopping after about `budget` seconds when budget > 0
+// (whole chunks: a bounded sample); *seconds receives the enumeration time
+// and *ranks the combinations tested.
 extern "C" long long cpu_thin_port(unsigned n, unsigned n_left, const unsigned* off, const unsigned* adj, unsigned k,
-                                   unsigned long long limit, int threads, double* seconds,
+                                   unsigned long long limit, int threads, double budget, double* seconds,
                                    unsigned long long* ranks) {
   if (k < 1 || k > 3) return -1;
   Graph g;
@@ -52,19 +54,33 @@ extern "C" long long cpu_thin_port(unsigned n, unsigned n_left, const unsigned* 
   const u64 lim = limit ? limit : ~0ull;
   std::atomic<u64> count{0}, tested{0};
   const auto t0 = std::chrono::steady_clock::now();
-  for (u32 s = g.a_lo; s < g.a_hi && count.load() < lim; ++s) {
+  std::atomic<bool> out_of_time{false};
+  auto over = [&]() {
+    if (budget <= 0) return false;
+    if (out_of_time.load(std::memory_order_relaxed)) return true;
+    if (std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() > budget) out_of_time = true;
+    return out_of_time.load();
+  };
+  for (u32 s = g.a_lo; s < g.a_hi && count.load() < lim && !over(); ++s) {
     const u32 L[1] = {s};
     HostArena ar;
     HostFams fo;
     Cost cost;
     build_families(g, L, 1, k, 1, 0, ar, fo, cost);
-    for (Family& f : fo.f) guard_build(g, f, k, ar, cost);
+    for (Family& f : fo.f) {
+      if (over()) break;
+      guard_build(g, f, k, ar, cost);
+    }
     for (const Family& f : fo.f) {
-      if (count.load() >= lim) break;
+      if (count.load() >= lim || over()) break;
       const u64 nchunks = (f.total + CHUNK - 1) / CHUNK;
+      // chunks in groups, so a family of 10^16 combinations stops at the
+      // limit or the budget instead of skipping through every chunk
+      for (u64 c0 = 0; c0 < nchunks && count.load() < lim && !over(); c0 += GROUP) {
+      const u64 c1 = std::min(nchunks, c0 + GROUP);
 #pragma omp parallel for schedule(dynamic, 1)
-      for (u64 c = 0; c < nchunks; ++c) {
-        if (count.load(std::memory_order_relaxed) >= lim) continue;
+      for (u64 c = c0; c < c1; ++c) {
+        if (count.load(std::memory_order_relaxed) >= lim || over()) continue;
         u64 found = 0, done = 0;
         Cost cc;
         const u64 r1 = std::min(f.total, (c + 1) * CHUNK);
@@ -88,6 +104,7 @@ extern "C" long long cpu_thin_port(unsigned n, unsigned n_left, const unsigned* 
         }
         tested += done;
         count += found;
+      }
       }
     }
   }

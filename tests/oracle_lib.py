@@ -251,9 +251,10 @@ PORT_PATH = os.path.join(ROOT, "oracle", "_build", "libcpuport.so")
 _port = None
 
 
-def cpu_thin_port(g, k, limit=0, threads=0):
+def cpu_thin_port(g, k, limit=0, threads=0, budget=0.0):
     """Multi-core CPU port of the thin-result method (oracle/cpu_port.cpp):
-    (results, seconds, ranks tested) over the left side's one-vertex tuples."""
+    (results, seconds, ranks tested) over the left side's one-vertex tuples,
+    stopping after about `budget` seconds when budget > 0."""
     global _port
     if _port is None:
         _port = ctypes.CDLL(PORT_PATH)
@@ -261,12 +262,13 @@ def cpu_thin_port(g, k, limit=0, threads=0):
         u32p = P(ctypes.c_uint32)
         _port.cpu_thin_port.restype = ctypes.c_longlong
         _port.cpu_thin_port.argtypes = [ctypes.c_uint32, ctypes.c_uint32, u32p, u32p, ctypes.c_uint32, ctypes.c_uint64,
-                                        ctypes.c_int, P(ctypes.c_double), P(ctypes.c_uint64)]
+                                        ctypes.c_int, ctypes.c_double, P(ctypes.c_double), P(ctypes.c_uint64)]
     off = np.ascontiguousarray(g.off, dtype=np.uint32)
     adj = np.ascontiguousarray(g.adj, dtype=np.uint32)
     sec = ctypes.c_double(0)
     rk = ctypes.c_uint64(0)
-    n = _port.cpu_thin_port(g.n, g.n_left, _p(off), _p(adj), k, limit, threads, ctypes.byref(sec), ctypes.byref(rk))
+    n = _port.cpu_thin_port(g.n, g.n_left, _p(off), _p(adj), k, limit, threads, float(budget), ctypes.byref(sec),
+                            ctypes.byref(rk))
     return int(n), float(sec.value), int(rk.value)
 
 
