@@ -107,8 +107,9 @@ typedef struct {
                                 (up to 8192 per side a state's working set is in LDS, up
                                 to 65536 in HBM) */
   uint64_t reduce_edges;     /* edges the butterfly reduction removed */
-  double ms_build;           /* seeded runs: host time building the seed partitions' vertex
-                                sets (BPPivot constructSets), inside the window `ms` */
+  double ms_build;           /* seeded runs: wall time building the seed partitions' vertex
+                                sets (BPPivot constructSets, on the device) and placing them
+                                in the partition tables, inside the window `ms` */
 } gmk_stats;
 
 /* Create a context on HIP device `device` (-d, src/main.cu:1765-1766). */
