@@ -347,6 +347,16 @@ int BBEngine::step() {
     hctl[l].cap = cap_[l];
     hctl[l].bmax = batch_max(l);
     hctl[l].fanout = (float)fanout_[l];
+    // The one-workgroup-per-state form launches a workgroup per state of the
+    // largest batch the sequence may plan.  For a stack still small at this
+    // synchronisation that largest batch is capped at a power of two above
+    // 4 times its size, so a small search does not launch the batch cap's
+    // worth of workgroups that exit at once.
+    if (l < LANE_TUNED && !(persistent_ && c_->tune.waves_per_cu[l])) {
+      u64 want = 1024;
+      while (want < 4 * top_[l] && want < hctl[l].bmax) want <<= 1;
+      hctl[l].bmax = (u32)std::min<u64>(hctl[l].bmax, want);
+    }
   }
   HIPCHK(c_, hipMemcpyAsync(dctl, hctl, sizeof(LaneCtl) * NLANE, hipMemcpyHostToDevice, s));
   ExpandArgs a{};
@@ -391,7 +401,7 @@ int BBEngine::step() {
       q.a.scratch = c_->wide_scratch.p;
     }
     // a workgroup per state of the largest batch, or the persistent grid
-    q.grid = q.waves ? q.waves : batch_max(l);
+    q.grid = q.waves ? q.waves : hctl[l].bmax;
   }
   HIPCHK(c_, hipEventRecord(c_->ev0, s));
   if ((r = launch_sequence(L, nl, active, S))) return r;
