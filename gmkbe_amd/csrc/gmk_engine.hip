@@ -89,6 +89,7 @@ extern "C" void gmk_destroy(gmk_ctx* c) {
       if (v->p) (void)hipFree(v->p);
   if (c->d_off) (void)hipFree(c->d_off);
   if (c->d_adj) (void)hipFree(c->d_adj);
+  if (c->d_deg) (void)hipFree(c->d_deg);
   if (c->h_cnt) (void)hipHostFree(c->h_cnt);
   if (c->h_ctl) (void)hipHostFree(c->h_ctl);
   if (c->h_stage) (void)hipHostFree(c->h_stage);
@@ -112,16 +113,25 @@ extern "C" int gmk_device_name(gmk_ctx* c, char* out, uint32_t cap) {
   return GMK_OK;
 }
 
+__global__ __launch_bounds__(256) void vertex_degrees(const u32* off, u32 n, u32* deg) {
+  const u32 v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v < n) deg[v] = off[v + 1] - off[v];
+}
+
 static int upload_graph(gmk_ctx* c) {
   HIPCHK(c, hipSetDevice(c->device));
   if (c->d_off) (void)hipFree(c->d_off);
   if (c->d_adj) (void)hipFree(c->d_adj);
-  c->d_off = c->d_adj = nullptr;
+  if (c->d_deg) (void)hipFree(c->d_deg);
+  c->d_off = c->d_adj = c->d_deg = nullptr;
   HIPCHK(c, hipMalloc(&c->d_off, sizeof(u32) * (c->n + 1)));
   HIPCHK(c, hipMalloc(&c->d_adj, sizeof(u32) * std::max<size_t>(1, c->adj.size())));
+  HIPCHK(c, hipMalloc(&c->d_deg, sizeof(u32) * std::max<size_t>(1, c->n)));
   HIPCHK(c, hipMemcpyAsync(c->d_off, c->off.data(), sizeof(u32) * (c->n + 1), hipMemcpyHostToDevice, c->stream));
   if (!c->adj.empty())
     HIPCHK(c, hipMemcpyAsync(c->d_adj, c->adj.data(), sizeof(u32) * c->adj.size(), hipMemcpyHostToDevice, c->stream));
+  if (c->n) hipLaunchKernelGGL(vertex_degrees, dim3((c->n + 255) / 256), dim3(256), 0, c->stream, c->d_off, c->n, c->d_deg);
+  HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return gmk::build_neighbour_hashes(c);
 }

@@ -137,6 +137,7 @@ struct gmk_ctx {
   std::vector<u32> off, adj;
   u32* d_off = nullptr;
   u32* d_adj = nullptr;
+  u32* d_deg = nullptr;  // vertex degrees, off[v + 1] - off[v] (one load per lookup)
   // per-run buffers, grown on demand and reused
   DeviceVec lane_stack[NLANE], lane_next[NLANE], lane_retry[NLANE];  // state stacks per bitset width
   DeviceVec lane_spill[NLANE];  // shallow states moved out of a nearly full stack

@@ -36,6 +36,7 @@ constexpr u32 SELECT_ITEMS = 2048;         // items per workgroup of score_selec
 struct ItemArgs {
   const u32* off;
   const u32* adj;
+  const u32* deg;   // vertex degrees
   u32 first, n;     // items: vertices first .. first + n - 1
   u32 nb;           // vertices on side B
   float k;
@@ -69,7 +70,7 @@ __device__ __forceinline__ void item_features(const ItemArgs& a, u32 u, float (&
     nb[i] = i < ns ? a.adj[o + e] : 0u;
   }
 #pragma unroll
-  for (u32 i = 0; i < SAMPLE; ++i) dv[i] = i < ns ? a.off[nb[i] + 1] - a.off[nb[i]] : 0u;
+  for (u32 i = 0; i < SAMPLE; ++i) dv[i] = i < ns ? a.deg[nb[i]] : 0u;
   // mean and variance by Welford's update (no cancellation when the degrees agree)
   float mean = 0.f, m2 = 0.f, mx = 0.f;
   u32 mn = 0xFFFFFFFFu;
@@ -106,24 +107,36 @@ __device__ __forceinline__ short bf16_rne(float x) {
 __device__ __forceinline__ float bf16_val(short h) { return __uint_as_float((u32)(uint16_t)h << 16); }
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef short s16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
+// v_mfma_f32_32x32x16_bf16: D += A B with A 32 x 16, B 16 x 32, the operands
+// given as bf16 bits
+__device__ __forceinline__ f32x16 mfma16(s16x8 a, s16x8 b, f32x16 c) {
+  return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), c, 0,
+                                                 0, 0);
+}
+
 // One lane per item: features, then (with a model) the forward pass, 64 items
-// per wave, both weight layers on the matrix cores: Z^T = W^T X^T with
-// v_mfma_f32_32x32x8_bf16 over 2 x 2 tiles of 32 (units x items), each operand
-// split into bf16 high and low parts (a b ~ ah bh + ah bl + al bh, three
-// products per step, FP32 accumulation: about 2^-15 relative error per
-// product).  Operand layout of the instruction (lane l, j = 0..3):
-// A[i = l % 32][k = 4 (l / 32) + j], B[k = 4 (l / 32) + j][n = l % 32];
-// result D[8 (r / 4) + 4 (l / 32) + r % 4][l % 32], r = 0..15.  Lane l feeds
+// per wave, both weight layers on the matrix cores as Z^T = W^T X^T over
+// tiles of 32 units x 32 items, each operand split into bf16 high and low
+// parts (a b ~ ah bh + ah bl + al bh, three products per step, FP32
+// accumulation: about 2^-15 relative error per product).  The first layer
+// (8 inputs) uses v_mfma_f32_32x32x8_bf16, the second v_mfma_f32_32x32x16_bf16.
+// Operand layout (lane l, K = 4 or 8 values per lane, j < K):
+// A[i = l % 32][k = K (l / 32) + j], B[k = K (l / 32) + j][n = l % 32]; result
+// D[8 (r / 4) + 4 (l / 32) + r % 4][l % 32], r = 0..15, for both.  Lane l feeds
 // item n = 32 t + l % 32 of tile t: its own item or that of lane l ^ 32, whose
-// features it takes by a shuffle.  The first layer's result tile (units
-// 32 m + 8 (r / 4) + 4 (l / 32) + r % 4) is, after bias and ReLU, exactly the
-// B operand of the second layer's step s = 4 m + r / 4, so no data moves
-// between the layers.  The weights sit in LDS transposed, split into their two
-// bf16 parts.  The output layer sums each item's 64 units over the lane pair
-// (l, l ^ 32).
-__global__ __launch_bounds__(256) void item_score(ItemArgs a) {
+// features it takes by a shuffle.  A first-layer result tile holds, in lane
+// half h, units 32 m + 4 h + {0..3} + 8 {0..3}; after bias and ReLU, its
+// registers 8 q .. 8 q + 7 are the B operand of a 16-unit second-layer step as
+// they stand, once the A operand (W2^T, read from LDS) takes its columns in the
+// same unit order (the instruction sums over the K pairs of A and B, whatever
+// units they stand for).  The weights sit in LDS transposed, split into their
+// two bf16 parts.  The output layer sums each item's 64 units over the lane
+// pair (l, l ^ 32).
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void item_score(ItemArgs a) {
   __shared__ float b1[H], b2[H], w3[H], mu[IN], sd[IN];
   __shared__ float b3;
   __shared__ __align__(16) short w1h[H][IN + 4], w1l[H][IN + 4];  // W1^T: [unit][input]
@@ -175,65 +188,71 @@ __global__ __launch_bounds__(256) void item_score(ItemArgs a) {
       xh[t][j] = hi;
       xl[t][j] = bf16_rne(v - bf16_val(hi));
     }
-  f32x16 z1[2][2];
-#pragma unroll
-  for (int m = 0; m < 2; ++m) {
-    const bf16x4 ah = *(const bf16x4*)&w1h[32 * m + col][4 * half];
-    const bf16x4 al = *(const bf16x4*)&w1l[32 * m + col][4 * half];
-#pragma unroll
-    for (int t = 0; t < 2; ++t) {
-      f32x16 c;
-#pragma unroll
-      for (int r = 0; r < 16; ++r) c[r] = 0.f;
-      c = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(ah, xh[t], c, 0, 0, 0);
-      c = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(ah, xl[t], c, 0, 0, 0);
-      z1[m][t] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(al, xh[t], c, 0, 0, 0);
-    }
-  }
-  f32x16 acc[2][2];
-#pragma unroll
-  for (int m = 0; m < 2; ++m)
-#pragma unroll
-    for (int t = 0; t < 2; ++t)
-#pragma unroll
-      for (int r = 0; r < 16; ++r) acc[m][t][r] = 0.f;
-#pragma unroll
-  for (int s = 0; s < H / 8; ++s) {
-    const int k0 = 8 * s + 4 * half;  // this lane's input units of step s
-    bf16x4 bh[2], bl[2];
-#pragma unroll
-    for (int t = 0; t < 2; ++t)
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        float z = z1[s / 4][t][4 * (s % 4) + j] + b1[k0 + j];
-        z = z > 0.f ? z : 0.f;
-        const short hi = bf16_rne(z);
-        bh[t][j] = hi;
-        bl[t][j] = bf16_rne(z - bf16_val(hi));
-      }
-#pragma unroll
-    for (int m = 0; m < 2; ++m) {
-      const bf16x4 ah = *(const bf16x4*)&w2h[32 * m + col][k0];
-      const bf16x4 al = *(const bf16x4*)&w2l[32 * m + col][k0];
-#pragma unroll
-      for (int t = 0; t < 2; ++t) {
-        acc[m][t] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(ah, bh[t], acc[m][t], 0, 0, 0);
-        acc[m][t] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(ah, bl[t], acc[m][t], 0, 0, 0);
-        acc[m][t] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(al, bh[t], acc[m][t], 0, 0, 0);
-      }
-    }
-  }
+  // Output tiles of the second layer one at a time (32 units each), the first
+  // layer recomputed per tile, 32 units at a time: at most two result tiles
+  // per item tile are live, so four waves fit on a SIMD.
   float part[2] = {0.f, 0.f};
 #pragma unroll
-  for (int m = 0; m < 2; ++m)
+  for (int mo = 0; mo < 2; ++mo) {
+    f32x16 acc[2];
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[t][r] = 0.f;
+#pragma unroll
+    for (int mi = 0; mi < 2; ++mi) {
+      // first layer, units 32 mi .. 32 mi + 31
+      f32x16 z1[2];
+      const bf16x4 ah1 = *(const bf16x4*)&w1h[32 * mi + col][4 * half];
+      const bf16x4 al1 = *(const bf16x4*)&w1l[32 * mi + col][4 * half];
+#pragma unroll
+      for (int t = 0; t < 2; ++t) {
+        f32x16 c;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) c[r] = 0.f;
+        c = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(ah1, xh[t], c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(ah1, xl[t], c, 0, 0, 0);
+        z1[t] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(al1, xh[t], c, 0, 0, 0);
+      }
+      // second layer, two steps of 16 input units: z1[t][8 q + j] is unit
+      // u0 + (j & 3) + 8 (j >> 2) of this lane's half, which is the k index
+      // 8 half + j of the B operand; the A operand takes the same units
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        const int u0 = 32 * mi + 16 * q + 4 * half;
+        s16x8 bh[2], bl[2];
+#pragma unroll
+        for (int t = 0; t < 2; ++t)
+#pragma unroll
+          for (int j = 0; j < 8; ++j) {
+            float z = z1[t][8 * q + j] + b1[u0 + (j & 3) + 8 * (j >> 2)];
+            z = z > 0.f ? z : 0.f;
+            const short hi = bf16_rne(z);
+            bh[t][j] = hi;
+            bl[t][j] = bf16_rne(z - bf16_val(hi));
+          }
+        const int row = 32 * mo + col;
+        const s16x8 ah = __builtin_shufflevector(*(const bf16x4*)&w2h[row][u0], *(const bf16x4*)&w2h[row][u0 + 8],
+                                                 0, 1, 2, 3, 4, 5, 6, 7);
+        const s16x8 al = __builtin_shufflevector(*(const bf16x4*)&w2l[row][u0], *(const bf16x4*)&w2l[row][u0 + 8],
+                                                 0, 1, 2, 3, 4, 5, 6, 7);
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+          acc[t] = mfma16(ah, bh[t], acc[t]);
+          acc[t] = mfma16(ah, bl[t], acc[t]);
+          acc[t] = mfma16(al, bh[t], acc[t]);
+        }
+      }
+    }
 #pragma unroll
     for (int t = 0; t < 2; ++t)
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
-        const int u = 32 * m + 8 * (r / 4) + 4 * half + (r % 4);
-        const float z = acc[m][t][r] + b2[u];
+        const int u = 32 * mo + 8 * (r / 4) + 4 * half + (r % 4);
+        const float z = acc[t][r] + b2[u];
         part[t] = fmaf(z > 0.f ? z : 0.f, w3[u], part[t]);
       }
+  }
 #pragma unroll
   for (int t = 0; t < 2; ++t) part[t] += __shfl_xor(part[t], 32, 64);
   if (!live) return;
@@ -253,10 +272,12 @@ __global__ void fill_u32(u32* p, u32 n, u32 v) {
 // largest) among the n <= 1024 SEL_PER_LANE sampled keys, by a
 // most-significant-digit-first radix select over four 8-bit digits.  Each lane
 // loads its samples once, into registers.  The keys cluster (items alike in
-// their features score alike), so a wave adds its lanes' digits to the LDS
-// histogram one distinct digit at a time (one atomic per digit, for all lanes
-// holding it) instead of one atomic per lane.  0 when every item is to be
-// gathered.
+// their features score alike), so a wave first adds its lanes' digits to the
+// LDS histogram one distinct digit at a time (one atomic per digit, for all
+// lanes holding it), for at most LEADER_ROUNDS digits; the lanes left, whose
+// digits are spread out, add theirs one atomic per lane.  0 when every item is
+// to be gathered.
+constexpr int LEADER_ROUNDS = 4;
 __global__ __launch_bounds__(1024) void select_threshold(const u32* smp, u32 n, u32 at, int all, u32* thr) {
   __shared__ u32 hist[256];
   __shared__ u32 prefix_s, at_s;
@@ -282,7 +303,7 @@ __global__ __launch_bounds__(1024) void select_threshold(const u32* smp, u32 n, 
       const u32 q = rd * blockDim.x + threadIdx.x;
       bool pend = q < n && (v[rd] & hmask) == prefix;
       const u32 dg = (v[rd] >> shift) & 255u;
-      for (;;) {
+      for (int it = 0; it < LEADER_ROUNDS; ++it) {
         const u64 m = __ballot(pend);
         if (!m) break;
         const int leader = __ffsll((unsigned long long)m) - 1;
@@ -291,17 +312,33 @@ __global__ __launch_bounds__(1024) void select_threshold(const u32* smp, u32 n, 
         if (lane == leader) atomicAdd(&hist[ld], (u32)__popcll(same));
         if (pend && dg == ld) pend = false;
       }
+      // digits still pending are spread out: one atomic per lane
+      if (pend) atomicAdd(&hist[dg], 1u);
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-      // digits from the top: the one whose keys cover rank at_s
-      u32 r = at_s, d = 255;
-      for (;; --d) {
-        if (r < hist[d] || d == 0) break;
-        r -= hist[d];
+    if (threadIdx.x < 64) {
+      // digits from the top: the one whose keys cover rank at_s.  Lane l holds
+      // digits 4 l .. 4 l + 3; a suffix sum over the lanes gives the keys above
+      // each lane's digits, and the lane whose range covers the rank (lane 0
+      // when the rank is beyond every key) picks the digit.
+      const u32 l = threadIdx.x;
+      u32 h[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) h[j] = hist[4 * l + j];
+      const u32 tot = h[0] + h[1] + h[2] + h[3];
+      u32 incl = tot;
+#pragma unroll
+      for (int off = 1; off < 64; off <<= 1) {
+        const u32 o = (u32)__shfl_down((int)incl, off, 64);
+        if (l + off < 64) incl += o;
       }
-      prefix_s = prefix | (d << shift);
-      at_s = r;
+      const u32 r = at_s, above = incl - tot;
+      if ((r >= above && r < incl) || (l == 0 && r >= incl)) {
+        u32 rr = r - above, d = 4 * l + 3;
+        for (int j = 3; j > 0 && rr >= h[j]; --j, --d) rr -= h[j];
+        prefix_s = prefix | (d << shift);
+        at_s = rr;
+      }
     }
     __syncthreads();
   }
@@ -342,23 +379,34 @@ __global__ __launch_bounds__(256) void score_select(const u32* key, u32 n, const
 constexpr u32 RANK_BLOCK = 256;
 constexpr u32 RANK_SPLIT = 8;
 constexpr u32 RANK_PER_BLOCK = RANK_BLOCK / RANK_SPLIT;
+constexpr u32 RANK_UNROLL = 8;
+static_assert(FRONT_CAP % (RANK_UNROLL * RANK_SPLIT) == 0, "the padded entries fit the LDS array");
 __global__ __launch_bounds__(RANK_BLOCK) void sort_front(const u32* cnt, const u64* sel, u64* out) {
   __shared__ u64 s[FRONT_CAP];
   const u32 m = *cnt < FRONT_CAP ? *cnt : FRONT_CAP;
   if (blockIdx.x == 0 && threadIdx.x == 0) out[0] = *cnt;
   if (blockIdx.x * RANK_PER_BLOCK >= m) return;
-  // FRONT_CAP / RANK_BLOCK loads per lane, issued together
+  // FRONT_CAP / RANK_BLOCK loads per lane, issued together; the entries are
+  // padded with zeros (above no entry) to a multiple of RANK_UNROLL RANK_SPLIT
+  const u32 mp = (m + RANK_UNROLL * RANK_SPLIT - 1) / (RANK_UNROLL * RANK_SPLIT) * (RANK_UNROLL * RANK_SPLIT);
 #pragma unroll
   for (u32 r = 0; r < FRONT_CAP / RANK_BLOCK; ++r) {
     const u32 q = r * RANK_BLOCK + threadIdx.x;
-    if (q < m) s[q] = sel[q];
+    if (q < mp) s[q] = q < m ? sel[q] : 0ull;
   }
   __syncthreads();
   const u32 e_idx = blockIdx.x * RANK_PER_BLOCK + threadIdx.x / RANK_SPLIT;
   const u32 part = threadIdx.x % RANK_SPLIT;
   const u64 e = e_idx < m ? s[e_idx] : ~0ull;
+  // RANK_UNROLL independent counts, so that that many LDS reads are in flight
+  u32 above[RANK_UNROLL] = {};
+  for (u32 q = part; q < mp; q += RANK_UNROLL * RANK_SPLIT) {
+#pragma unroll
+    for (u32 j = 0; j < RANK_UNROLL; ++j) above[j] += s[q + j * RANK_SPLIT] > e ? 1u : 0u;
+  }
   u32 rank = 0;
-  for (u32 q = part; q < m; q += RANK_SPLIT) rank += s[q] > e ? 1u : 0u;
+#pragma unroll
+  for (u32 j = 0; j < RANK_UNROLL; ++j) rank += above[j];
   // the RANK_SPLIT lanes of an entry are consecutive in the wave
 #pragma unroll
   for (u32 off = RANK_SPLIT / 2; off > 0; off >>= 1) rank += (u32)__shfl_xor((int)rank, (int)off, 64);
@@ -399,7 +447,7 @@ int gmk::learned_front(gmk_ctx* c, u32 a_lo, u32 a_hi, u32 k, u32 want, std::vec
   u64* sorted = sel + FRONT_CAP + 1;
   HIPCHK(c, hipMemcpyAsync(c->sched_p.p, P, (MLP_NPARAM + 2 * IN) * 4, hipMemcpyHostToDevice, s));
   hipLaunchKernelGGL(fill_u32, dim3(1), dim3(64), 0, s, cnt, 4u, 0u);
-  ItemArgs a{c->d_off, c->d_adj, a_lo, n, (a_lo == 0 ? c->n - c->n_left : c->n_left), (float)k,
+  ItemArgs a{c->d_off, c->d_adj, c->d_deg, a_lo, n, (a_lo == 0 ? c->n - c->n_left : c->n_left), (float)k,
              (const float*)c->sched_p.p, nullptr, key, smp, nullptr, sstep};
   hipLaunchKernelGGL(item_score, dim3((n + 255) / 256), dim3(256), 0, s, a);
   // Threshold: the sampled key ranked at about 1.5 `want` items, so that the
@@ -445,7 +493,7 @@ static int item_eval(gmk_ctx* c, uint32_t first, uint32_t count, uint32_t k, flo
     mlp_pack(c->item_model, P.data());
     for (int q = 0; q < IN; ++q) { P[MLP_NPARAM + q] = c->item_model.mu[q]; P[MLP_NPARAM + IN + q] = c->item_model.sd[q]; }
   }
-  ItemArgs a{c->d_off, c->d_adj, first, count, left ? c->n - c->n_left : c->n_left, (float)k,
+  ItemArgs a{c->d_off, c->d_adj, c->d_deg, first, count, left ? c->n - c->n_left : c->n_left, (float)k,
              predict ? (const float*)p.p : nullptr, predict ? nullptr : (float*)f.p, nullptr, nullptr,
              predict ? (float*)f.p : nullptr};
   hipError_t e = hipSuccess;
@@ -502,7 +550,7 @@ extern "C" int gmk_item_predict_static(gmk_ctx* c, uint32_t first, uint32_t coun
   for (int q = 0; q < IN; ++q) { c->h_stage[MLP_NPARAM + q] = c->item_model.mu[q]; c->h_stage[MLP_NPARAM + IN + q] = c->item_model.sd[q]; }
   HIPCHK(c, hipMemcpyAsync(c->scratch, c->h_stage, SCRATCH_PARAMS * 4, hipMemcpyHostToDevice, s));
   float* score = c->scratch + SCRATCH_PARAMS;
-  ItemArgs a{c->d_off, c->d_adj, first, count, left ? c->n - c->n_left : c->n_left, (float)k, c->scratch, nullptr,
+  ItemArgs a{c->d_off, c->d_adj, c->d_deg, first, count, left ? c->n - c->n_left : c->n_left, (float)k, c->scratch, nullptr,
              nullptr, nullptr, score};
   hipLaunchKernelGGL(item_score, dim3((count + 255) / 256), dim3(256), 0, s, a);
   HIPCHK(c, hipGetLastError());

@@ -29,7 +29,7 @@ constexpr u64 RCH = 1ull << 22;        // ranks per check launch
 constexpr u64 SHRINK_MIN = 1ull << 16;  // remaining results below which a launch takes the slot check
 constexpr u32 DEFER_BUDGET = 64;       // candidates one lane tests before a rank goes to a whole wave
 constexpr u32 LEARNED_FRONT = 4096;    // about 1.5 times this many one-vertex items are ordered by the learned schedule (the rest follow in ID order)
-constexpr u64 LEARNED_SLICE = 1u << 16; // ranks of each family per slice in learned runs
+constexpr u64 LEARNED_SLICE = 1ull << 24; // ranks of each family per slice in learned runs
 
 struct ThinRun {
   gmk_ctx* c;
@@ -311,6 +311,8 @@ int gmk_run_thin(gmk_ctx* c, const gmk_params* prm, gmk_stats* st, RunState& rs,
   // follow in blocks as below.
   if (prm->mode == GMK_MODE_LEARNED && c->has_item_model && jmin == 1) {
     T.slice = LEARNED_SLICE;
+    // GMK_LEARNED_SLICE overrides the ranks per family per slice (at least 1024)
+    if (const char* e = getenv("GMK_LEARNED_SLICE")) T.slice = std::max<u64>(1024, strtoull(e, nullptr, 10));
     LearnedOrder order;
     const auto t0 = std::chrono::steady_clock::now();
     if ((r = learned_front(c, a_lo, a_hi, k, LEARNED_FRONT, order.cand))) return r;
