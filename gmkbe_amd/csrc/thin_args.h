@@ -74,6 +74,10 @@ constexpr u32 WAVE1_MAX_TUPLES = 16384;
 // first ranks, learned runs take the families slice by slice; base 0, cap ~0:
 // whole families).
 hipError_t launch_thin_scan(const thin::Family* fams, u32 nfam, u64* prefix, u64 base, u64 cap, hipStream_t s);
+// The same with the family count read on the device (*nfam_p, at most fam_cap);
+// the sum goes to *total_out.
+hipError_t launch_thin_scan_dev(const thin::Family* fams, const u32* nfam_p, u32 fam_cap, u64* prefix, u64 base,
+                                u64 cap, u64* total_out, hipStream_t s);
 // Rank index of a check launch: ridx[b] = family of rank min(r0 + b 2^RIDX_LOG,
 // r1 - 1), b = 0 .. ceil((r1 - r0) / 2^RIDX_LOG).
 constexpr u32 RIDX_LOG = 8;

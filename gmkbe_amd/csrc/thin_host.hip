@@ -20,6 +20,7 @@ struct ThinCounters {
   u32 fam_n, tflags;
   u64 arena_used;
   u32 work, defer_n, dwork, pad;  // check kernels: chunk counter, deferred ranks, next deferred rank
+  u64 total;                      // ranks of the first slice of the launch's families
 };
 
 constexpr u32 TCH = 1u << 16;          // tuples per family-construction launch
@@ -47,47 +48,26 @@ struct ThinRun {
   std::vector<u64>* probe_ticks = nullptr;
   bool guards = true;  // guard tables for the checks (GMK_GUARDS=0: plain checks only)
   u64 count0 = 0;          // result count when the run's thin part started
-  u64 ranks_launched = 0;  // combinations launched since then
+  u64 ranks_settled = 0;   // combinations launched since then whose results the host has read
 };
 
 // Checks ranks [0, total) of the current family table (its prefix sums are
 // on the device), RCH ranks per launch: rank index, first pass, deferred pass.
+// Launches go out without waiting for the ones before them while the count
+// they may reach stays under the limit (each rank yields at most one result):
+// the host reads the counters after a launch that may reach the limit, after
+// MAX_FLIGHT launches, and at the end of the range.
+constexpr u32 MAX_FLIGHT = 4;
 int check_ranks(ThinRun& T, u64 total) {
   gmk_ctx* c = T.c;
   hipStream_t s = c->stream;
   ThinArgs& a = T.a;
   RunCounters* dc = (RunCounters*)c->counters.p;
   const u64 limit = T.rs->limit;
-  // GMK_DEDUP_REPLAY: every rank range a second time (its results repeat)
-  const u64 passes = T.rs->replay ? 2 : 1;
-  for (u64 pass = 0, r0 = 0; pass < passes; r0 = 0, ++pass)
-  for (; r0 < total; r0 = a.r1) {
-    // A launch yields at most one result per rank, so one of at most
-    // limit - count ranks cannot pass the limit and skips the per-result slot
-    // check (unlimited).  Near the limit, while the run's launches have been
-    // yielding at least half a result per rank, launches shrink to what
-    // remains (each then covers at least half of it); below SHRINK_MIN
-    // remaining results, or at a lower yield, a launch takes the slot check.
-    const u64 cnt = c->h_cnt->res_count;
-    u64 n = std::min(RCH, total - r0);
-    if (limit != ~0ull) {
-      const u64 rem = limit > cnt ? limit - cnt : 0;
-      const bool high_yield = T.ranks_launched && 2 * (cnt - T.count0) >= T.ranks_launched;
-      if (rem < n && rem >= SHRINK_MIN && high_yield) n = rem;
-    }
-    a.r0 = r0;
-    a.r1 = r0 + n;
-    a.unlimited = limit == ~0ull || cnt + n <= limit;
-    T.ranks_launched += n;
-    int rr;
-    if ((rr = dedup_reserve(c, *T.rs, c->h_cnt->res_count + (a.r1 - a.r0)))) return rr;
-    a.dedup = T.rs->dedup;
-    // work, defer_n and dwork are consecutive
-    HIPCHK(c, hipMemsetAsync(&((ThinCounters*)c->tcnt.p)->work, 0, 3 * sizeof(u32), s));
-    HIPCHK(c, hipEventRecord(c->ev0, s));
-    HIPCHK(c, launch_thin_rank_index((const u64*)c->prefix.p, a.nfam, a.r0, a.r1, (u32*)c->ridx.p, s));
-    HIPCHK(c, launch_thin_check(a, s, c->n_cu * 8, c->tune.check_form[a.k < 4 ? a.k : 3]));
-    HIPCHK(c, launch_thin_check_deferred(a, s, c->n_cu * 4));
+  u64 inflight = 0;  // ranks launched since the counters were last read
+  u32 nflight = 0;
+  // waits for the launches in flight and reads the counters
+  auto settle = [&]() -> int {
     HIPCHK(c, hipEventRecord(c->ev1, s));
     HIPCHK(c, hipMemcpyAsync(c->h_cnt, dc, offsetof(RunCounters, fp), hipMemcpyDeviceToHost, s));
     HIPCHK(c, hipStreamSynchronize(s));
@@ -95,22 +75,62 @@ int check_ranks(ThinRun& T, u64 total) {
     HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
     T.rs->ms_expand += ms;
     T.rs->ms_kernel[2] += ms;
-    T.rs->launches_kernel[2]++;
-    T.rs->launches++;
+    T.ranks_settled += inflight;
+    inflight = 0;
+    nflight = 0;
     T.rs->progress(std::min<u64>(c->h_cnt->res_count, T.rs->limit));
     T.rs->report(a.swap_sides ? "thin (tuples on the right)" : "thin (tuples on the left)",
                  std::min<u64>(c->h_cnt->res_count, T.rs->limit), 0);
     if (c->h_cnt->flags & FLAG_RES_OVERFLOW) { c->err = "result buffer overflow (raise result_capacity_words)"; return GMK_EOVERFLOW; }
     if (c->h_cnt->flags & FLAG_LIMIT) { T.stop = true; T.rs->hit_limit = true; return GMK_OK; }
     if (c->h_cnt->res_count >= limit) {
-      // an unlimited launch ended exactly at the limit: raise the flag the
+      // unlimited launches ended exactly at the limit: raise the flag the
       // kernels of later phases read
       c->h_cnt->flags |= FLAG_LIMIT;
       HIPCHK(c, hipMemcpyAsync(&dc->flags, &c->h_cnt->flags, sizeof(u32), hipMemcpyHostToDevice, s));
       HIPCHK(c, hipStreamSynchronize(s));
       T.stop = true;
       T.rs->hit_limit = true;
-      return GMK_OK;
+    }
+    return GMK_OK;
+  };
+  // GMK_DEDUP_REPLAY: every rank range a second time (its results repeat)
+  const u64 passes = T.rs->replay ? 2 : 1;
+  for (u64 pass = 0, r0 = 0; pass < passes; r0 = 0, ++pass)
+  for (; r0 < total; r0 = a.r1) {
+    // The count when this launch starts is at most cnt + inflight.  A launch
+    // that cannot pass the limit from there skips the per-result slot check
+    // (unlimited).  Near the limit, while the run's launches have been
+    // yielding at least half a result per rank, a launch shrinks to what
+    // remains (then covering at least half of it); below SHRINK_MIN remaining
+    // results, or at a lower yield, a launch takes the slot check.
+    const u64 cnt = c->h_cnt->res_count;
+    const u64 bound = cnt + inflight;
+    u64 n = std::min(RCH, total - r0);
+    if (limit != ~0ull) {
+      const u64 rem = limit > bound ? limit - bound : 0;
+      const bool high_yield = T.ranks_settled && 2 * (cnt - T.count0) >= T.ranks_settled;
+      if (rem < n && rem >= SHRINK_MIN && high_yield) n = rem;
+    }
+    a.r0 = r0;
+    a.r1 = r0 + n;
+    a.unlimited = limit == ~0ull || bound + n <= limit;
+    int rr;
+    if ((rr = dedup_reserve(c, *T.rs, bound + n))) return rr;
+    a.dedup = T.rs->dedup;
+    if (nflight == 0) HIPCHK(c, hipEventRecord(c->ev0, s));
+    // work, defer_n and dwork are consecutive
+    HIPCHK(c, hipMemsetAsync(&((ThinCounters*)c->tcnt.p)->work, 0, 3 * sizeof(u32), s));
+    HIPCHK(c, launch_thin_rank_index((const u64*)c->prefix.p, a.nfam, a.r0, a.r1, (u32*)c->ridx.p, s));
+    HIPCHK(c, launch_thin_check(a, s, c->n_cu * 8, c->tune.check_form[a.k < 4 ? a.k : 3]));
+    HIPCHK(c, launch_thin_check_deferred(a, s, c->n_cu * 4));
+    T.rs->launches_kernel[2]++;
+    T.rs->launches++;
+    inflight += n;
+    ++nflight;
+    if (!a.unlimited || nflight >= MAX_FLIGHT || a.r1 >= total) {
+      if ((rr = settle())) return rr;
+      if (T.stop) return GMK_OK;
     }
   }
   return GMK_OK;
@@ -128,6 +148,11 @@ int run_chunk(ThinRun& T, const u32* tuples, u32 count, u32 j) {
   HIPCHK(c, launch_thin_build(a, s));
   if (T.guards) HIPCHK(c, launch_thin_guard(a, s, c->n_cu * 4));
   HIPCHK(c, hipEventRecord(c->ev1, s));
+  // the first rank slice's prefix sums, over the family count on the device,
+  // so that one read brings the count and the slice's total
+  ThinCounters* dtc = (ThinCounters*)c->tcnt.p;
+  HIPCHK(c, launch_thin_scan_dev((const Family*)c->fams.p, &dtc->fam_n, FCAP, (u64*)c->prefix.p, 0,
+                                 T.slice ? T.slice : T.probe_cap, &dtc->total, s));
   ThinCounters tc;
   HIPCHK(c, hipMemcpyAsync(&tc, c->tcnt.p, sizeof(tc), hipMemcpyDeviceToHost, s));
   HIPCHK(c, hipStreamSynchronize(s));
@@ -155,11 +180,12 @@ int run_chunk(ThinRun& T, const u32* tuples, u32 count, u32 j) {
   // in turn, the families in schedule order within each slice; other runs
   // take each family whole (probes: its first probe_cap ranks) in one slice.
   for (u64 base = 0;; base += T.slice) {
-    HIPCHK(c, launch_thin_scan((const Family*)c->fams.p, nfam, (u64*)c->prefix.p, base,
-                               T.slice ? T.slice : T.probe_cap, s));
-    u64 total = 0;
-    HIPCHK(c, hipMemcpyAsync(&total, (u64*)c->prefix.p + nfam, 8, hipMemcpyDeviceToHost, s));
-    HIPCHK(c, hipStreamSynchronize(s));
+    u64 total = tc.total;
+    if (base) {
+      HIPCHK(c, launch_thin_scan((const Family*)c->fams.p, nfam, (u64*)c->prefix.p, base, T.slice, s));
+      HIPCHK(c, hipMemcpyAsync(&total, (u64*)c->prefix.p + nfam, 8, hipMemcpyDeviceToHost, s));
+      HIPCHK(c, hipStreamSynchronize(s));
+    }
     if (total == 0) break;
     a.rank_base = base;
     const int r = check_ranks(T, total);

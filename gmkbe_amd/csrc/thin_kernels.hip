@@ -308,10 +308,14 @@ __global__ __launch_bounds__(1024) void thin_scan_counts(u32* cnt, u32 n, u32* f
 // then each thread's segment from the sum of the segments before it (the
 // phases are thin::scan_phase_*).  A family whose prefix is finite keeps it
 // even when the running sum saturates inside its segment.
-__global__ __launch_bounds__(thin::TSCAN) void thin_scan_totals(const thin::Family* fams, u32 nfam, u64* prefix,
-                                                                 u64 base, u64 cap) {
+// With nfam_p the family count is read on the device (at most fam_cap), and
+// total_out receives the sum.
+__global__ __launch_bounds__(thin::TSCAN) void thin_scan_totals(const thin::Family* fams, u32 nfam, const u32* nfam_p,
+                                                                 u32 fam_cap, u64* prefix, u64 base, u64 cap,
+                                                                 u64* total_out) {
   __shared__ u64 part[thin::TSCAN];
   const u32 t = threadIdx.x;
+  if (nfam_p) nfam = *nfam_p < fam_cap ? *nfam_p : fam_cap;
   auto total = [fams, base, cap](u32 i) {
     const u64 v = fams[i].total, rest = v > base ? v - base : 0;
     return rest < cap ? rest : cap;
@@ -325,6 +329,7 @@ __global__ __launch_bounds__(thin::TSCAN) void thin_scan_totals(const thin::Fami
     __syncthreads();
   }
   thin::scan_phase_write(nfam, t, part, total, prefix);
+  if (total_out && t == thin::TSCAN - 1) *total_out = part[thin::TSCAN - 1];
 }
 
 template <int K>
@@ -893,7 +898,15 @@ hipError_t launch_thin_rank_index(const u64* prefix, u32 nfam, u64 r0, u64 r1, u
 }
 
 hipError_t launch_thin_scan(const thin::Family* fams, u32 nfam, u64* prefix, u64 base, u64 cap, hipStream_t s) {
-  hipLaunchKernelGGL(thin_scan_totals, dim3(1), dim3(thin::TSCAN), 0, s, fams, nfam, prefix, base, cap);
+  hipLaunchKernelGGL(thin_scan_totals, dim3(1), dim3(thin::TSCAN), 0, s, fams, nfam, (const u32*)nullptr, 0u, prefix,
+                     base, cap, (u64*)nullptr);
+  return hipGetLastError();
+}
+
+hipError_t launch_thin_scan_dev(const thin::Family* fams, const u32* nfam_p, u32 fam_cap, u64* prefix, u64 base,
+                                u64 cap, u64* total_out, hipStream_t s) {
+  hipLaunchKernelGGL(thin_scan_totals, dim3(1), dim3(thin::TSCAN), 0, s, fams, 0u, nfam_p, fam_cap, prefix, base, cap,
+                     total_out);
   return hipGetLastError();
 }
 
