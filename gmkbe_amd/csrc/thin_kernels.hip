@@ -416,6 +416,23 @@ __device__ __forceinline__ thin::FamRef fam_view(const ThinArgs& a, const thin::
 }
 
 // Exclusive prefix sum over the wave.
+__device__ __forceinline__ u32 wave_sum_u32(u32 v) {
+#pragma unroll
+  for (int m = 32; m > 0; m >>= 1) v += (u32)__shfl_xor((int)v, m, 64);
+  return v;
+}
+
+__device__ __forceinline__ u32 wave_excl_scan_u32(u32 v) {
+  const int lane = threadIdx.x & 63;
+  u32 inc = v;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const u32 o = (u32)__shfl_up((int)inc, (unsigned)d, 64);
+    if (lane >= d) inc += o;
+  }
+  return inc - v;
+}
+
 __device__ __forceinline__ u64 wave_excl_scan_u64(u64 v) {
   const int lane = threadIdx.x & 63;
   u64 inc = v;
@@ -554,13 +571,15 @@ __device__ __forceinline__ bool emit_wave(const ThinArgs& a, bool want, const th
 constexpr u32 STAGE_WORDS = 4096;
 
 // emit_wave for a whole workgroup of 4 waves (every thread calls it; used by
-// the lane-per-rank check, whose workgroups reach it together): result slots,
-// record space, fingerprint sums and the check statistics are taken with one
-// atomic each per workgroup, and the workgroup's records, which fill one
-// contiguous range, are assembled in LDS and written with coalesced stores.
+// the lane-per-rank check, whose workgroups reach it together, each lane with
+// at most one tested rank): result slots, record space, fingerprint sums and
+// the check statistics are taken with one atomic each per workgroup, and the
+// workgroup's records, which fill one contiguous range, are assembled in LDS
+// and written with coalesced stores.  Counts come from ballots, and the
+// record offsets and byte sums from 32-bit cross-lane steps.
 template <int K>
 __device__ __forceinline__ bool emit_block(const ThinArgs& a, bool want, const thin::Family* f, u32 (&ch)[2 * K * K],
-                                           u32 nch, u64 n_tested, thin::Cost& cost) {
+                                           u32 nch, bool tested, thin::Cost& cost) {
   __shared__ u64 s_n[4], s_w[4], s_fs[4], s_fx[4], s_st[4][3];
   __shared__ u64 s_base, s_wbase;
   __shared__ u32 s_stage[STAGE_WORDS];
@@ -591,20 +610,22 @@ __device__ __forceinline__ bool emit_block(const ThinArgs& a, bool want, const t
   const u64 slot = s_base + before + (u64)__popcll(mask & ((1ull << lane) - 1));
   if (want && !a.unlimited && slot >= a.limit) want = false;
   const u32 nl = want ? f->j : 0, nr = want ? f->nfree + nch : 0;
-  const u64 len = want && a.keep ? 2ull + nl + nr : 0;
+  const u32 len = want && a.keep ? 2u + nl + nr : 0u;
   cost.bytes += 4ull * len;
-  const u64 off = wave_excl_scan_u64(len);
-  const u64 wtot = wave_bcast_u64(off + len, 63);
+  const u32 off = wave_excl_scan_u32(len);
+  const u32 wtot = (u32)__builtin_amdgcn_readlane((int)(off + len), 63);
+  const u64 emitted = __ballot(want), tested_mask = __ballot(tested);
   const u64 fsum = wave_sum_u64(want ? fpv : 0);
   u64 fxor = want ? fpv : 0;
   for (int m = 32; m > 0; m >>= 1) fxor ^= shfl_xor_u64(fxor, m);
-  const u64 se = wave_sum_u64(want ? 1 : 0), stt = wave_sum_u64(n_tested), sb = wave_sum_u64(cost.bytes);
+  // a lane's bytes for one rank fit 32 bits
+  const u32 sb = wave_sum_u32(cost.bytes < 0xFFFFFFFFull ? (u32)cost.bytes : 0xFFFFFFFFu);
   if (lane == 0) {
     s_w[wv] = wtot;
     s_fs[wv] = fsum;
     s_fx[wv] = fxor;
-    s_st[wv][0] = stt;
-    s_st[wv][1] = se;
+    s_st[wv][0] = (u64)__popcll(tested_mask);
+    s_st[wv][1] = (u64)__popcll(emitted);
     s_st[wv][2] = sb;
   }
   __syncthreads();
@@ -741,7 +762,7 @@ __global__ __launch_bounds__(256) void thin_check(ThinArgs a) {
   bool want = false;
   const u64 t0 = a.item_log ? (u64)wall_clock64() : 0;
   if (!limit_reached(a)) want = check_rank<K>(a, r, f, ch, nch, cost, n_tested);
-  const bool em = emit_block<K>(a, want, f, ch, nch, n_tested, cost);
+  const bool em = emit_block<K>(a, want, f, ch, nch, n_tested != 0, cost);
   log_item(a, f, em, t0);
 }
 
