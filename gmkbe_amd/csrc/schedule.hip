@@ -98,13 +98,13 @@ __device__ __forceinline__ void item_features(const ItemArgs& a, u32 u, float (&
   x[7] = a.k;
 }
 
-// bf16 bits of x rounded to nearest even (x finite), and back.
-__device__ __forceinline__ short bf16_rne(float x) {
-  u32 u = __float_as_uint(x);
-  u += 0x7FFFu + ((u >> 16) & 1u);
-  return (short)(u >> 16);
+// x split into bf16 parts, x ~ hi + lo (each rounded to nearest even by the
+// hardware conversion), as bits.
+__device__ __forceinline__ void split_bf16(float x, short& hi, short& lo) {
+  const __bf16 h = (__bf16)x;
+  hi = __builtin_bit_cast(short, h);
+  lo = __builtin_bit_cast(short, (__bf16)(x - (float)h));
 }
-__device__ __forceinline__ float bf16_val(short h) { return __uint_as_float((u32)(uint16_t)h << 16); }
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef short s16x8 __attribute__((ext_vector_type(8)));
@@ -146,15 +146,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void i
     constexpr int OW1 = 0, OB1 = OW1 + IN * H, OW2 = OB1 + H, OB2 = OW2 + H * H, OW3 = OB2 + H, OB3 = OW3 + H;
     for (int i = threadIdx.x; i < IN * H; i += blockDim.x) {
       const float w = P[OW1 + i];  // w1[input i / H][unit i % H]
-      const short hi = bf16_rne(w);
-      w1h[i % H][i / H] = hi;
-      w1l[i % H][i / H] = bf16_rne(w - bf16_val(hi));
+      split_bf16(w, w1h[i % H][i / H], w1l[i % H][i / H]);
     }
     for (int i = threadIdx.x; i < H * H; i += blockDim.x) {
       const float w = P[OW2 + i];  // w2[input i / H][output i % H]
-      const short hi = bf16_rne(w);
-      w2h[i % H][i / H] = hi;
-      w2l[i % H][i / H] = bf16_rne(w - bf16_val(hi));
+      split_bf16(w, w2h[i % H][i / H], w2l[i % H][i / H]);
     }
     for (int i = threadIdx.x; i < H; i += blockDim.x) { b1[i] = P[OB1 + i]; b2[i] = P[OB2 + i]; w3[i] = P[OW3 + i]; }
     if (threadIdx.x < IN) { mu[threadIdx.x] = P[OB3 + 1 + threadIdx.x]; sd[threadIdx.x] = P[OB3 + 1 + IN + threadIdx.x]; }
@@ -184,9 +180,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void i
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const float v = t == half ? (half ? x[4 + j] : x[j]) : (half ? xp[4 + j] : xp[j]);
-      const short hi = bf16_rne(v);
+      short hi, lo;
+      split_bf16(v, hi, lo);
       xh[t][j] = hi;
-      xl[t][j] = bf16_rne(v - bf16_val(hi));
+      xl[t][j] = lo;
     }
   // Output tiles of the second layer one at a time (32 units each), the first
   // layer recomputed per tile, 32 units at a time: at most two result tiles
@@ -227,9 +224,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void i
           for (int j = 0; j < 8; ++j) {
             float z = z1[t][8 * q + j] + b1[u0 + (j & 3) + 8 * (j >> 2)];
             z = z > 0.f ? z : 0.f;
-            const short hi = bf16_rne(z);
+            short hi, lo;
+            split_bf16(z, hi, lo);
             bh[t][j] = hi;
-            bl[t][j] = bf16_rne(z - bf16_val(hi));
+            bl[t][j] = lo;
           }
         const int row = 32 * mo + col;
         const s16x8 ah = __builtin_shufflevector(*(const bf16x4*)&w2h[row][u0], *(const bf16x4*)&w2h[row][u0 + 8],
