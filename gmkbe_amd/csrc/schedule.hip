@@ -270,11 +270,12 @@ __global__ void fill_u32(u32* p, u32 n, u32 v) {
 // largest) among the n <= 1024 SEL_PER_LANE sampled keys, by a
 // most-significant-digit-first radix select over four 8-bit digits.  Each lane
 // loads its samples once, into registers.  The keys cluster (items alike in
-// their features score alike), so a wave first adds its lanes' digits to the
-// LDS histogram one distinct digit at a time (one atomic per digit, for all
-// lanes holding it), for at most LEADER_ROUNDS digits; the lanes left, whose
-// digits are spread out, add theirs one atomic per lane.  0 when every item is
-// to be gathered.
+// their features score alike), so for the top digit a wave first adds its
+// lanes' digits to the LDS histogram one distinct digit at a time (one atomic
+// per digit, for all lanes holding it), for at most LEADER_ROUNDS digits; the
+// lanes left, and every lane for the lower digits (spread out, and only under
+// the chosen prefix), add theirs one atomic per lane.  0 when every item is to
+// be gathered.
 constexpr int LEADER_ROUNDS = 4;
 __global__ __launch_bounds__(1024) void select_threshold(const u32* smp, u32 n, u32 at, int all, u32* thr) {
   __shared__ u32 hist[256];
@@ -301,7 +302,7 @@ __global__ __launch_bounds__(1024) void select_threshold(const u32* smp, u32 n, 
       const u32 q = rd * blockDim.x + threadIdx.x;
       bool pend = q < n && (v[rd] & hmask) == prefix;
       const u32 dg = (v[rd] >> shift) & 255u;
-      for (int it = 0; it < LEADER_ROUNDS; ++it) {
+      for (int it = 0; it < (shift == 24 ? LEADER_ROUNDS : 0); ++it) {
         const u64 m = __ballot(pend);
         if (!m) break;
         const int leader = __ffsll((unsigned long long)m) - 1;
