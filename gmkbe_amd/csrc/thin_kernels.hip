@@ -416,10 +416,43 @@ __device__ __forceinline__ thin::FamRef fam_view(const ThinArgs& a, const thin::
 }
 
 // Exclusive prefix sum over the wave.
+// Wave-wide reductions by DPP moves within each row of 16 lanes (quad_perm
+// [1,0,3,2] and [2,3,0,1], then row_ror 4 and 8: every lane ends with its row's
+// total) and a readlane of the four row totals: ALU work only, no LDS
+// crossbar traffic.  Every lane of the wave must be active.
+template <int CTRL>
+__device__ __forceinline__ u32 dpp_u32(u32 v) {
+  return (u32)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, 0xF, 0xF, false);
+}
+constexpr int DPP_XOR1 = 0xB1, DPP_XOR2 = 0x4E, DPP_ROR4 = 0x124, DPP_ROR8 = 0x128;
+__device__ __forceinline__ u32 rows_u32(u32 v, int r) { return (u32)__builtin_amdgcn_readlane((int)v, 16 * r); }
 __device__ __forceinline__ u32 wave_sum_u32(u32 v) {
-#pragma unroll
-  for (int m = 32; m > 0; m >>= 1) v += (u32)__shfl_xor((int)v, m, 64);
-  return v;
+  v += dpp_u32<DPP_XOR1>(v);
+  v += dpp_u32<DPP_XOR2>(v);
+  v += dpp_u32<DPP_ROR4>(v);
+  v += dpp_u32<DPP_ROR8>(v);
+  return rows_u32(v, 0) + rows_u32(v, 1) + rows_u32(v, 2) + rows_u32(v, 3);
+}
+template <int CTRL>
+__device__ __forceinline__ u64 dpp_u64(u64 v) {
+  return ((u64)dpp_u32<CTRL>((u32)(v >> 32)) << 32) | dpp_u32<CTRL>((u32)v);
+}
+__device__ __forceinline__ u64 rows_u64(u64 v, int r) {
+  return ((u64)rows_u32((u32)(v >> 32), r) << 32) | rows_u32((u32)v, r);
+}
+__device__ __forceinline__ u64 wave_sum_u64_dpp(u64 v) {
+  v += dpp_u64<DPP_XOR1>(v);
+  v += dpp_u64<DPP_XOR2>(v);
+  v += dpp_u64<DPP_ROR4>(v);
+  v += dpp_u64<DPP_ROR8>(v);
+  return rows_u64(v, 0) + rows_u64(v, 1) + rows_u64(v, 2) + rows_u64(v, 3);
+}
+__device__ __forceinline__ u64 wave_xor_u64_dpp(u64 v) {
+  v ^= dpp_u64<DPP_XOR1>(v);
+  v ^= dpp_u64<DPP_XOR2>(v);
+  v ^= dpp_u64<DPP_ROR4>(v);
+  v ^= dpp_u64<DPP_ROR8>(v);
+  return rows_u64(v, 0) ^ rows_u64(v, 1) ^ rows_u64(v, 2) ^ rows_u64(v, 3);
 }
 
 __device__ __forceinline__ u32 wave_excl_scan_u32(u32 v) {
@@ -575,8 +608,8 @@ constexpr u32 STAGE_WORDS = 4096;
 // at most one tested rank): result slots, record space, fingerprint sums and
 // the check statistics are taken with one atomic each per workgroup, and the
 // workgroup's records, which fill one contiguous range, are assembled in LDS
-// and written with coalesced stores.  Counts come from ballots, and the
-// record offsets and byte sums from 32-bit cross-lane steps.
+// and written with coalesced stores.  Counts come from ballots, sums from DPP
+// reductions, and the record offsets from a 32-bit scan.
 template <int K>
 __device__ __forceinline__ bool emit_block(const ThinArgs& a, bool want, const thin::Family* f, u32 (&ch)[2 * K * K],
                                            u32 nch, bool tested, thin::Cost& cost) {
@@ -615,9 +648,8 @@ __device__ __forceinline__ bool emit_block(const ThinArgs& a, bool want, const t
   const u32 off = wave_excl_scan_u32(len);
   const u32 wtot = (u32)__builtin_amdgcn_readlane((int)(off + len), 63);
   const u64 emitted = __ballot(want), tested_mask = __ballot(tested);
-  const u64 fsum = wave_sum_u64(want ? fpv : 0);
-  u64 fxor = want ? fpv : 0;
-  for (int m = 32; m > 0; m >>= 1) fxor ^= shfl_xor_u64(fxor, m);
+  const u64 fsum = wave_sum_u64_dpp(want ? fpv : 0);
+  const u64 fxor = wave_xor_u64_dpp(want ? fpv : 0);
   // a lane's bytes for one rank fit 32 bits
   const u32 sb = wave_sum_u32(cost.bytes < 0xFFFFFFFFull ? (u32)cost.bytes : 0xFFFFFFFFu);
   if (lane == 0) {
