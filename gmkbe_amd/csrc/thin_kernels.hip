@@ -415,7 +415,6 @@ __device__ __forceinline__ thin::FamRef fam_view(const ThinArgs& a, const thin::
   return r;
 }
 
-// Exclusive prefix sum over the wave.
 // Wave-wide reductions by DPP moves within each row of 16 lanes (quad_perm
 // [1,0,3,2] and [2,3,0,1], then row_ror 4 and 8: every lane ends with its row's
 // total) and a readlane of the four row totals: ALU work only, no LDS
@@ -455,6 +454,7 @@ __device__ __forceinline__ u64 wave_xor_u64_dpp(u64 v) {
   return rows_u64(v, 0) ^ rows_u64(v, 1) ^ rows_u64(v, 2) ^ rows_u64(v, 3);
 }
 
+// Exclusive prefix sums over the wave.
 __device__ __forceinline__ u32 wave_excl_scan_u32(u32 v) {
   const int lane = threadIdx.x & 63;
   u32 inc = v;
