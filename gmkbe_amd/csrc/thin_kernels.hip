@@ -599,9 +599,12 @@ __device__ __forceinline__ bool emit_wave(const ThinArgs& a, bool want, const th
   return true;
 }
 
-// Record words a workgroup of the lane-per-rank check stages in LDS before
-// writing them out with coalesced stores (more go straight to device memory).
-constexpr u32 STAGE_WORDS = 4096;
+// Waves per workgroup of the lane-per-rank check, and the record words a
+// workgroup stages in LDS before writing them out with coalesced stores (more
+// go straight to device memory).
+constexpr u32 CHECK_WAVES = 4;
+constexpr u32 CHECK_THREADS = 64 * CHECK_WAVES;
+constexpr u32 STAGE_WORDS = 1024 * CHECK_WAVES;
 
 // emit_wave for a whole workgroup of 4 waves (every thread calls it; used by
 // the lane-per-rank check, whose workgroups reach it together, each lane with
@@ -613,7 +616,7 @@ constexpr u32 STAGE_WORDS = 4096;
 template <int K>
 __device__ __forceinline__ bool emit_block(const ThinArgs& a, bool want, const thin::Family* f, u32 (&ch)[2 * K * K],
                                            u32 nch, bool tested, thin::Cost& cost) {
-  __shared__ u64 s_n[4], s_w[4], s_fs[4], s_fx[4], s_st[4][3];
+  __shared__ u64 s_n[CHECK_WAVES], s_w[CHECK_WAVES], s_fs[CHECK_WAVES], s_fx[CHECK_WAVES], s_st[CHECK_WAVES][3];
   __shared__ u64 s_base, s_wbase;
   __shared__ u32 s_stage[STAGE_WORDS];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -624,7 +627,9 @@ __device__ __forceinline__ bool emit_block(const ThinArgs& a, bool want, const t
   if (lane == 0) s_n[wv] = (u64)__popcll(mask);
   __syncthreads();
   if (threadIdx.x == 0) {
-    const u64 tot = s_n[0] + s_n[1] + s_n[2] + s_n[3];
+    u64 tot = 0;
+#pragma unroll
+    for (u32 w = 0; w < CHECK_WAVES; ++w) tot += s_n[w];
     u64 base = 0;
     if (tot && a.unlimited) {
       // every result of the launch fits under the limit: the count is added
@@ -639,7 +644,7 @@ __device__ __forceinline__ bool emit_block(const ThinArgs& a, bool want, const t
   __syncthreads();
   u64 before = 0;
 #pragma unroll
-  for (int w = 0; w < 4; ++w) before += w < wv ? s_n[w] : 0ull;
+  for (int w = 0; w < (int)CHECK_WAVES; ++w) before += w < wv ? s_n[w] : 0ull;
   const u64 slot = s_base + before + (u64)__popcll(mask & ((1ull << lane) - 1));
   if (want && !a.unlimited && slot >= a.limit) want = false;
   const u32 nl = want ? f->j : 0, nr = want ? f->nfree + nch : 0;
@@ -661,20 +666,29 @@ __device__ __forceinline__ bool emit_block(const ThinArgs& a, bool want, const t
     s_st[wv][2] = sb;
   }
   __syncthreads();
-  const u64 btot = s_w[0] + s_w[1] + s_w[2] + s_w[3];
+  u64 btot = 0;
+#pragma unroll
+  for (u32 w = 0; w < CHECK_WAVES; ++w) btot += s_w[w];
   if (threadIdx.x == 0) {
-    const u64 fs = s_fs[0] + s_fs[1] + s_fs[2] + s_fs[3], fx = s_fx[0] ^ s_fx[1] ^ s_fx[2] ^ s_fx[3];
+    u64 fs = 0, fx = 0, tt = 0, te = 0, tb = 0;
+#pragma unroll
+    for (u32 w = 0; w < CHECK_WAVES; ++w) {
+      fs += s_fs[w];
+      fx ^= s_fx[w];
+      tt += s_st[w][0];
+      te += s_st[w][1];
+      tb += s_st[w][2];
+    }
     if (fs | fx) {
       u64* fp = a.fp + (stripe % NSTRIPE) * 2;
       atomicAdd((unsigned long long*)&fp[0], (unsigned long long)fs);
       atomicXor((unsigned long long*)&fp[1], (unsigned long long)fx);
     }
-    const u64 tt = s_st[0][0] + s_st[1][0] + s_st[2][0] + s_st[3][0];
     if (tt) {
       u64* sp = a.stat + (stripe % NSTRIPE) * NSTAT;
       atomicAdd((unsigned long long*)&sp[1], (unsigned long long)tt);
-      atomicAdd((unsigned long long*)&sp[2], (unsigned long long)(s_st[0][1] + s_st[1][1] + s_st[2][1] + s_st[3][1]));
-      atomicAdd((unsigned long long*)&sp[5], (unsigned long long)(s_st[0][2] + s_st[1][2] + s_st[2][2] + s_st[3][2]));
+      atomicAdd((unsigned long long*)&sp[2], (unsigned long long)te);
+      atomicAdd((unsigned long long*)&sp[5], (unsigned long long)tb);
     }
     u64 wb = 0;
     if (btot) {
@@ -688,11 +702,11 @@ __device__ __forceinline__ bool emit_block(const ThinArgs& a, bool want, const t
   if (!btot || wb + btot > a.res_cap) return want;  // the same in every thread
   u64 boff = off;
 #pragma unroll
-  for (int w = 0; w < 4; ++w) boff += w < wv ? s_w[w] : 0ull;
+  for (int w = 0; w < (int)CHECK_WAVES; ++w) boff += w < wv ? s_w[w] : 0ull;
   if (btot <= STAGE_WORDS) {
     if (want) write_record<K>(s_stage + boff, a, f, ch, nch);
     __syncthreads();
-    for (u32 i = threadIdx.x; i < (u32)btot; i += 256) a.res[wb + i] = s_stage[i];
+    for (u32 i = threadIdx.x; i < (u32)btot; i += CHECK_THREADS) a.res[wb + i] = s_stage[i];
   } else if (want) {
     write_record<K>(a.res + wb + boff, a, f, ch, nch);
   }
@@ -784,7 +798,7 @@ __global__ __launch_bounds__(256) void thin_check_deferred(ThinArgs a) {
 // One lane per rank over the whole grid; a workgroup emits its results
 // together (emit_block), so every thread reaches that call.
 template <int K>
-__global__ __launch_bounds__(256) void thin_check(ThinArgs a) {
+__global__ __launch_bounds__(CHECK_THREADS) void thin_check(ThinArgs a) {
   constexpr int NM = 2 * K * K;
   const u64 r = a.r0 + (u64)blockIdx.x * blockDim.x + threadIdx.x;
   u64 n_tested = 0;
@@ -1015,11 +1029,11 @@ hipError_t launch_thin_check(const ThinArgs& a, hipStream_t s, u32 max_blocks, u
       default: return hipErrorInvalidValue;
     }
   } else {
-    const dim3 g((u32)need);
+    const dim3 g((u32)((n + CHECK_THREADS - 1) / CHECK_THREADS)), bc(CHECK_THREADS);
     switch (a.k) {
-      case 1: hipLaunchKernelGGL(thin_check<1>, g, b, 0, s, a); break;
-      case 2: hipLaunchKernelGGL(thin_check<2>, g, b, 0, s, a); break;
-      case 3: hipLaunchKernelGGL(thin_check<3>, g, b, 0, s, a); break;
+      case 1: hipLaunchKernelGGL(thin_check<1>, g, bc, 0, s, a); break;
+      case 2: hipLaunchKernelGGL(thin_check<2>, g, bc, 0, s, a); break;
+      case 3: hipLaunchKernelGGL(thin_check<3>, g, bc, 0, s, a); break;
       default: return hipErrorInvalidValue;
     }
   }
