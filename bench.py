@@ -22,6 +22,7 @@ timed on this host, one core, on a bounded sample of the same enumeration.
 from __future__ import annotations
 
 import argparse
+import gc
 import json
 import os
 import re
@@ -248,18 +249,25 @@ def train_item_model(eng, cfg, workload, g):
 
 def run_steps(eng, cfg, mode, partition, steps, warmup, sync):
     """Timed steps write every result record to device memory; the records are
-    not copied to the host inside the timed region."""
+    not copied to the host inside the timed region.  Python's cyclic garbage
+    collector is paused over the timed steps (a collection pass in the middle
+    of a millisecond step is host noise, not device work)."""
     for _ in range(warmup):
         eng.enumerate(k=cfg["k"], theta=cfg["theta"], limit=cfg["limit"], mode=mode, partition=partition, keep=True,
                       fetch=False)
     sync()
-    t0 = time.perf_counter()
-    last = None
-    for _ in range(steps):
-        last = eng.enumerate(k=cfg["k"], theta=cfg["theta"], limit=cfg["limit"], mode=mode, partition=partition,
-                             keep=True, fetch=False)
-    sync()
-    dt = time.perf_counter() - t0
+    gc.collect()
+    gc.disable()
+    try:
+        t0 = time.perf_counter()
+        last = None
+        for _ in range(steps):
+            last = eng.enumerate(k=cfg["k"], theta=cfg["theta"], limit=cfg["limit"], mode=mode, partition=partition,
+                                 keep=True, fetch=False)
+        sync()
+        dt = time.perf_counter() - t0
+    finally:
+        gc.enable()
     return dt, last
 
 
