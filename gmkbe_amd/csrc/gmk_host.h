@@ -200,8 +200,16 @@ int static_scratch(float** ptr, size_t* floats);
 // items of side A ([a_lo, a_hi)) with the highest predicted results per unit
 // of device time under the item cost model, sorted best first (ties: lower ID
 // first), as (score key << 32 | ~(item - a_lo)) entries (a sampled threshold
-// decides which are gathered; the device sorts them).
-int learned_front(gmk_ctx* c, u32 a_lo, u32 a_hi, u32 k, u32 want, std::vector<u64>& front);
+// decides which are gathered; the device sorts them).  With pre_max > 0, when
+// the front is sure to hold at least pre_max items, the first pre_max of them
+// are also written to the device's tuple buffer as one-vertex tuples (pre =
+// pre_max) and the front's copy to the host is left in flight: the caller
+// takes it with learned_front_finish after its next synchronisation of the
+// stream.  Otherwise (pre = 0) the front is in `front` on return.  ms: device
+// time of the scoring and selection kernels (known once the front is).
+int learned_front(gmk_ctx* c, u32 a_lo, u32 a_hi, u32 k, u32 want, u32 pre_max, std::vector<u64>& front, u32& pre,
+                  double& ms);
+void learned_front_finish(gmk_ctx* c, std::vector<u64>& front, double& ms);
 // Hands out the front's items in order.
 struct LearnedOrder {
   std::vector<u64> cand;

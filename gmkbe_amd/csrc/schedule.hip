@@ -414,8 +414,23 @@ __global__ __launch_bounds__(RANK_BLOCK) void sort_front(const u32* cnt, const u
 
 }  // namespace
 
-int gmk::learned_front(gmk_ctx* c, u32 a_lo, u32 a_hi, u32 k, u32 want, std::vector<u64>& front) {
+// The front's pinned staging, after the model parameters in the context's
+// pinned buffer.
+static u64* front_stage(gmk_ctx* c) { return (u64*)(c->h_stage + (MLP_NPARAM + 2 * IN + 1) / 2 * 2); }
+
+namespace {
+// First n items of the sorted front as one-vertex tuples (vertex IDs).
+__global__ void front_tuples(const u64* sorted, u32 a_lo, u32 n, u32* tuples) {
+  const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) tuples[i] = a_lo + ~(u32)(sorted[1 + i] & 0xFFFFFFFFull);
+}
+}  // namespace
+
+int gmk::learned_front(gmk_ctx* c, u32 a_lo, u32 a_hi, u32 k, u32 want, u32 pre_max, std::vector<u64>& front,
+                       u32& pre, double& ms) {
   front.clear();
+  pre = 0;
+  ms = 0;
   const u32 n = a_hi - a_lo;
   if (!c->has_item_model || n == 0 || want == 0) return GMK_OK;
   want = std::min(want, std::min(n, FRONT_CAP / 2));
@@ -435,7 +450,7 @@ int gmk::learned_front(gmk_ctx* c, u32 a_lo, u32 a_hi, u32 k, u32 want, std::vec
                     ((size_t)MLP_NPARAM + 2 * IN + GMK_STATIC_ITEMS) * 4,
                 "the pinned staging holds the parameters and the sorted front");
   float* P = c->h_stage;
-  u64* hfront = (u64*)(c->h_stage + (MLP_NPARAM + 2 * IN + 1) / 2 * 2);
+  u64* hfront = front_stage(c);
   mlp_pack(c->item_model, P);
   for (int q = 0; q < IN; ++q) { P[MLP_NPARAM + q] = c->item_model.mu[q]; P[MLP_NPARAM + IN + q] = c->item_model.sd[q]; }
   u32* cnt = (u32*)c->sched_hist.p;
@@ -445,6 +460,7 @@ int gmk::learned_front(gmk_ctx* c, u32 a_lo, u32 a_hi, u32 k, u32 want, std::vec
   u64* sel = (u64*)c->sched_sel.p;
   u64* sorted = sel + FRONT_CAP + 1;
   HIPCHK(c, hipMemcpyAsync(c->sched_p.p, P, (MLP_NPARAM + 2 * IN) * 4, hipMemcpyHostToDevice, s));
+  HIPCHK(c, hipEventRecord(c->ev2, s));
   hipLaunchKernelGGL(fill_u32, dim3(1), dim3(64), 0, s, cnt, 4u, 0u);
   ItemArgs a{c->d_off, c->d_adj, c->d_deg, a_lo, n, (a_lo == 0 ? c->n - c->n_left : c->n_left), (float)k,
              (const float*)c->sched_p.p, nullptr, key, smp, nullptr, sstep};
@@ -454,19 +470,38 @@ int gmk::learned_front(gmk_ctx* c, u32 a_lo, u32 a_hi, u32 k, u32 want, std::vec
   // and stays under FRONT_CAP; every item when there are at most 1.5 `want`.
   const u64 span = (u64)want * 3 / 2;
   const u32 at = (u32)std::min<u64>(ns - 1, (span + sstep - 1) / sstep);
-  hipLaunchKernelGGL(select_threshold, dim3(1), dim3(1024), 0, s, (const u32*)smp, ns, at, n <= span ? 1 : 0, thr);
+  const bool all = n <= span;
+  hipLaunchKernelGGL(select_threshold, dim3(1), dim3(1024), 0, s, (const u32*)smp, ns, at, all ? 1 : 0, thr);
   hipLaunchKernelGGL(score_select, dim3((n + SELECT_ITEMS - 1) / SELECT_ITEMS), dim3(256), 0, s, (const u32*)key, n,
                      (const u32*)thr, FRONT_CAP, cnt, sel);
   hipLaunchKernelGGL(sort_front, dim3(FRONT_CAP / RANK_PER_BLOCK), dim3(RANK_BLOCK), 0, s, (const u32*)cnt,
                      (const u64*)sel, sorted);
+  // The front holds at least every item (all) or the at + 1 sampled items at
+  // or above the threshold, up to FRONT_CAP: when that is pre_max or more, its
+  // first pre_max items become the first launch's tuples on the device.
+  const u64 front_min = std::min<u64>(all ? n : (u64)at + 1, FRONT_CAP);
+  if (pre_max && front_min >= pre_max) {
+    hipLaunchKernelGGL(front_tuples, dim3((pre_max + 255) / 256), dim3(256), 0, s, (const u64*)sorted, a_lo, pre_max,
+                       (u32*)c->tuples.p);
+    pre = pre_max;
+  }
   HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipEventRecord(c->ev3, s));
   HIPCHK(c, hipMemcpyAsync(hfront, sorted, (size_t)(FRONT_CAP + 1) * 8, hipMemcpyDeviceToHost, s));
+  if (pre) return GMK_OK;
   HIPCHK(c, hipStreamSynchronize(s));
+  learned_front_finish(c, front, ms);
+  return GMK_OK;
+}
+
+void gmk::learned_front_finish(gmk_ctx* c, std::vector<u64>& front, double& ms) {
+  const u64* hfront = front_stage(c);
   // entries gathered beyond FRONT_CAP (a rare sampling error) are left out of
   // the front; their items follow in ID order like every item outside it
   const u64 m = std::min<u64>(hfront[0], FRONT_CAP);
   front.assign(hfront + 1, hfront + 1 + m);
-  return GMK_OK;
+  float t = 0;
+  ms = hipEventElapsedTime(&t, c->ev2, c->ev3) == hipSuccess ? t : 0.0;
 }
 
 u32 gmk::LearnedOrder::next(u32 a_lo) { return a_lo + ~(u32)(cand[pos++] & 0xFFFFFFFFu); }

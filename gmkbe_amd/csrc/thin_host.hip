@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cstddef>
+#include <functional>
 #include <vector>
 
 #include "gmk_host.h"
@@ -136,10 +137,16 @@ int check_ranks(ThinRun& T, u64 total) {
   return GMK_OK;
 }
 
-int run_chunk(ThinRun& T, const u32* tuples, u32 count, u32 j) {
+// One launch of `count` tuples of j vertices: families, guard tables, then the
+// checks of their ranks.  With on_device the tuples are already in the
+// device's tuple buffer, and `tuples` (their host copy) is filled by
+// after_sync, which runs after the launch's first synchronisation.
+int run_chunk(ThinRun& T, const u32* tuples, u32 count, u32 j, const std::function<void()>* after_sync = nullptr,
+              bool on_device = false) {
   gmk_ctx* c = T.c;
   hipStream_t s = c->stream;
-  HIPCHK(c, hipMemcpyAsync(c->tuples.p, tuples, (size_t)count * j * 4, hipMemcpyHostToDevice, s));
+  if (!on_device)
+    HIPCHK(c, hipMemcpyAsync(c->tuples.p, tuples, (size_t)count * j * 4, hipMemcpyHostToDevice, s));
   HIPCHK(c, hipMemsetAsync(c->tcnt.p, 0, sizeof(ThinCounters), s));
   ThinArgs& a = T.a;
   a.ntuples = count;
@@ -156,6 +163,7 @@ int run_chunk(ThinRun& T, const u32* tuples, u32 count, u32 j) {
   ThinCounters tc;
   HIPCHK(c, hipMemcpyAsync(&tc, c->tcnt.p, sizeof(tc), hipMemcpyDeviceToHost, s));
   HIPCHK(c, hipStreamSynchronize(s));
+  if (after_sync) (*after_sync)();
   {
     float ms = 0;
     HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
@@ -340,11 +348,28 @@ int gmk_run_thin(gmk_ctx* c, const gmk_params* prm, gmk_stats* st, RunState& rs,
     // GMK_LEARNED_SLICE overrides the ranks per family per slice (at least 1024)
     if (const char* e = getenv("GMK_LEARNED_SLICE")) T.slice = std::max<u64>(1024, strtoull(e, nullptr, 10));
     LearnedOrder order;
-    const auto t0 = std::chrono::steady_clock::now();
-    if ((r = learned_front(c, a_lo, a_hi, k, LEARNED_FRONT, order.cand))) return r;
-    rs.ms_schedule += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-    TupleFeed feed(T, 1, chunk);
+    u32 pre = 0;
+    double ms_sched = 0;
+    if ((r = learned_front(c, a_lo, a_hi, k, LEARNED_FRONT, chunk, order.cand, pre, ms_sched))) return r;
     std::vector<u32> taken;
+    if (pre) {
+      // The front's first `chunk` items are the first launch's tuples, on the
+      // device already: the launch goes out with no host round trip, and the
+      // front reaches the host by its first synchronisation.
+      std::vector<u32> head(pre);
+      const std::function<void()> fill = [&]() {
+        learned_front_finish(c, order.cand, ms_sched);
+        for (u32 i = 0; i < pre; ++i) head[i] = order.next(a_lo);
+      };
+      r = run_chunk(T, head.data(), pre, 1, &fill, true);
+      rs.ms_schedule += ms_sched;
+      if (r || T.stop) return r;
+      taken.assign(head.begin(), head.end());
+      chunk = std::min<u32>(chunk * 4, TCH);
+    } else {
+      rs.ms_schedule += ms_sched;
+    }
+    TupleFeed feed(T, 1, chunk);
     while (!order.done()) {
       u32 u = order.next(a_lo);
       taken.push_back(u);
