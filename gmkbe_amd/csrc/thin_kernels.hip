@@ -522,11 +522,18 @@ __device__ __forceinline__ void write_record(u32* out, const ThinArgs& a, const 
                                              u32 nch) {
   constexpr int NM = 2 * K * K;
   const u32 nl = f->j, nr = f->nfree + nch;
+  // chosen vertices of one type come ascending (the far pool always, as the
+  // lists of one-vertex tuples): the network runs only when they are not
+  bool ordered = true;
 #pragma unroll
-  for (int pass = 0; pass < NM; ++pass)
+  for (int q = 0; q + 1 < NM; ++q) ordered = ordered && ((u32)(q + 1) >= nch || ch[q] <= ch[q + 1]);
+  if (!ordered) {
 #pragma unroll
-    for (int q = pass & 1; q + 1 < NM; q += 2)
-      if ((u32)(q + 1) < nch && ch[q] > ch[q + 1]) { const u32 tmp = ch[q]; ch[q] = ch[q + 1]; ch[q + 1] = tmp; }
+    for (int pass = 0; pass < NM; ++pass)
+#pragma unroll
+      for (int q = pass & 1; q + 1 < NM; q += 2)
+        if ((u32)(q + 1) < nch && ch[q] > ch[q + 1]) { const u32 tmp = ch[q]; ch[q] = ch[q + 1]; ch[q + 1] = tmp; }
+  }
   const u32* fr = f->freep;
   // tuple side and generated side, left side first
   const u32 at_t = a.swap_sides ? 1 + nr + 1 : 1;
