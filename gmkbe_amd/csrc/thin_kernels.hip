@@ -515,8 +515,9 @@ __device__ __forceinline__ bool lane_fingerprint(const ThinArgs& a, const thin::
 
 // Writes the record [|L|, L..., |R|, R...] (left side first) of family f with
 // chosen vertices ch at out (device or LDS memory); R merges the free list
-// with the chosen vertices, sorted in place by an odd-even transposition
-// network over the first nch slots.
+// with the chosen vertices, sorted in place (when they are not ascending
+// already) by an odd-even transposition network over the first nch slots.
+// ch is consumed.
 template <int K>
 __device__ __forceinline__ void write_record(u32* out, const ThinArgs& a, const thin::Family* f, u32 (&ch)[2 * K * K],
                                              u32 nch) {
@@ -542,15 +543,36 @@ __device__ __forceinline__ void write_record(u32* out, const ThinArgs& a, const 
   out[a.swap_sides ? 0 : 1 + nl] = nr;
   for (u32 i = 0; i < nl; ++i) out[at_t + i] = f->L[i];
   u32 p = 0, q = 0;
-  for (u32 i = 0; i < nr; ++i) {
-    u32 cq = 0xFFFFFFFFu;  // ch[q], or none
+  if constexpr (K <= 2) {
+    // merge: the next chosen vertex is always ch[0]; taking it shifts the
+    // rest down one slot (constant indices, so ch stays in registers)
+    u32 cq = nch ? ch[0] : 0xFFFFFFFFu;
+    for (u32 i = 0; i < nr; ++i) {
+      u32 v;
+      if (q >= nch || (p < f->nfree && fr[p] < cq)) {
+        v = fr[p++];
+      } else {
+        v = cq;
+        ++q;
 #pragma unroll
-    for (int s2 = 0; s2 < NM; ++s2)
-      if ((u32)s2 == q && q < nch) cq = ch[s2];
-    u32 v;
-    if (q >= nch || (p < f->nfree && fr[p] < cq)) v = fr[p++];
-    else { v = cq; ++q; }
-    out[at_r + i] = v;
+        for (int s2 = 0; s2 + 1 < NM; ++s2) ch[s2] = ch[s2 + 1];
+        cq = q < nch ? ch[0] : 0xFFFFFFFFu;
+      }
+      out[at_r + i] = v;
+    }
+  } else {
+    // k = 3 (18 slots): the shifts would cost a wave per SIMD in registers;
+    // ch[q] is fetched by a select over constant indices instead
+    for (u32 i = 0; i < nr; ++i) {
+      u32 cq = 0xFFFFFFFFu;  // ch[q], or none
+#pragma unroll
+      for (int s2 = 0; s2 < NM; ++s2)
+        if ((u32)s2 == q && q < nch) cq = ch[s2];
+      u32 v;
+      if (q >= nch || (p < f->nfree && fr[p] < cq)) v = fr[p++];
+      else { v = cq; ++q; }
+      out[at_r + i] = v;
+    }
   }
 }
 
