@@ -79,8 +79,11 @@ hipError_t launch_thin_scan(const thin::Family* fams, u32 nfam, u64* prefix, u64
 hipError_t launch_thin_scan_dev(const thin::Family* fams, const u32* nfam_p, u32 fam_cap, u64* prefix, u64 base,
                                 u64 cap, u64* total_out, hipStream_t s);
 // Rank index of a check launch: ridx[b] = family of rank min(r0 + b 2^RIDX_LOG,
-// r1 - 1), b = 0 .. ceil((r1 - r0) / 2^RIDX_LOG).
+// r1 - 1), b = 0 .. ceil((r1 - r0) / 2^RIDX_LOG).  It also clears the three
+// consecutive words at zero3 (the check kernels' chunk counter, deferred-rank
+// count and next deferred rank), in place of a memset launch.
 constexpr u32 RIDX_LOG = 8;
-hipError_t launch_thin_rank_index(const u64* prefix, u32 nfam, u64 r0, u64 r1, u32* ridx, hipStream_t s);
+hipError_t launch_thin_rank_index(const u64* prefix, u32 nfam, u64 r0, u64 r1, u32* ridx, u32* zero3,
+                                  hipStream_t s);
 
 }  // namespace gmk

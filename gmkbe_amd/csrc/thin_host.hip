@@ -120,9 +120,9 @@ int check_ranks(ThinRun& T, u64 total) {
     if ((rr = dedup_reserve(c, *T.rs, bound + n))) return rr;
     a.dedup = T.rs->dedup;
     if (nflight == 0) HIPCHK(c, hipEventRecord(c->ev0, s));
-    // work, defer_n and dwork are consecutive
-    HIPCHK(c, hipMemsetAsync(&((ThinCounters*)c->tcnt.p)->work, 0, 3 * sizeof(u32), s));
-    HIPCHK(c, launch_thin_rank_index((const u64*)c->prefix.p, a.nfam, a.r0, a.r1, (u32*)c->ridx.p, s));
+    // the rank index also clears work, defer_n and dwork (consecutive)
+    HIPCHK(c, launch_thin_rank_index((const u64*)c->prefix.p, a.nfam, a.r0, a.r1, (u32*)c->ridx.p,
+                                     &((ThinCounters*)c->tcnt.p)->work, s));
     HIPCHK(c, launch_thin_check(a, s, c->n_cu * 8, c->tune.check_form[a.k < 4 ? a.k : 3]));
     HIPCHK(c, launch_thin_check_deferred(a, s, c->n_cu * 4));
     T.rs->launches_kernel[2]++;

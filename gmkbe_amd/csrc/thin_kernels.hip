@@ -974,8 +974,9 @@ hipError_t launch_thin_guard(const ThinArgs& a, hipStream_t s, u32 blocks) {
 
 // One thread per index entry: binary search of the family prefix sums.
 __global__ __launch_bounds__(256) void thin_rank_index(const u64* prefix, u32 nfam, u64 r0, u64 r1, u32 nb,
-                                                       u32* ridx) {
+                                                       u32* ridx, u32* zero3) {
   const u32 b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b < 3) zero3[b] = 0;
   if (b > nb) return;
   const u64 rb = r0 + ((u64)b << RIDX_LOG);
   const u64 r = rb < r1 ? rb : r1 - 1;
@@ -987,9 +988,11 @@ __global__ __launch_bounds__(256) void thin_rank_index(const u64* prefix, u32 nf
   ridx[b] = lo;
 }
 
-hipError_t launch_thin_rank_index(const u64* prefix, u32 nfam, u64 r0, u64 r1, u32* ridx, hipStream_t s) {
+hipError_t launch_thin_rank_index(const u64* prefix, u32 nfam, u64 r0, u64 r1, u32* ridx, u32* zero3,
+                                  hipStream_t s) {
   const u32 nb = (u32)((r1 - r0 + (1ull << RIDX_LOG) - 1) >> RIDX_LOG);
-  hipLaunchKernelGGL(thin_rank_index, dim3((nb + 1 + 255) / 256), dim3(256), 0, s, prefix, nfam, r0, r1, nb, ridx);
+  hipLaunchKernelGGL(thin_rank_index, dim3((nb + 1 + 255) / 256), dim3(256), 0, s, prefix, nfam, r0, r1, nb, ridx,
+                     zero3);
   return hipGetLastError();
 }
 
