@@ -282,8 +282,8 @@ def with_fetch(eng, cfg, mode, partition):
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="cl2Mx1M", choices=sorted(WORKLOADS))
     ap.add_argument("--partition", default="seeded", choices=["auto", "global", "seeded"])
     ap.add_argument("--tune", default=os.path.join(ROOT, "gmkbe_amd", "tune", "mi355x.json"), help="tuning table (written by python -m gmkbe_amd.autotune)")
