@@ -276,7 +276,7 @@ __global__ void fill_u32(u32* p, u32 n, u32 v) {
 // lanes left, and every lane for the lower digits (spread out, and only under
 // the chosen prefix), add theirs one atomic per lane.  0 when every item is to
 // be gathered.
-constexpr int LEADER_ROUNDS = 4;
+constexpr int LEADER_ROUNDS = 2;
 __global__ __launch_bounds__(1024) void select_threshold(const u32* smp, u32 n, u32 at, int all, u32* thr) {
   __shared__ u32 hist[256];
   __shared__ u32 prefix_s, at_s;
