@@ -376,7 +376,7 @@ __global__ __launch_bounds__(256) void score_select(const u32* key, u32 n, const
 // counts by shuffles.  out[0] = entries gathered (may exceed the cap),
 // out[1 ..] = the sorted entries.
 constexpr u32 RANK_BLOCK = 256;
-constexpr u32 RANK_SPLIT = 8;
+constexpr u32 RANK_SPLIT = 16;
 constexpr u32 RANK_PER_BLOCK = RANK_BLOCK / RANK_SPLIT;
 constexpr u32 RANK_UNROLL = 8;
 static_assert(FRONT_CAP % (RANK_UNROLL * RANK_SPLIT) == 0, "the padded entries fit the LDS array");
