@@ -31,7 +31,7 @@ constexpr u32 SAMPLE = 1u << LOG2_SAMPLE;  // neighbours sampled per item for th
 constexpr u32 SSTEP_MIN = 64;              // every sstep-th item's key is sampled for the selection threshold
 constexpr u32 SEL_PER_LANE = 32;           // samples per lane of select_threshold (at most 32768 samples)
 constexpr u32 FRONT_CAP = 8192;            // gathered items, at most (sorted in LDS: 64 KiB)
-constexpr u32 SELECT_ITEMS = 2048;         // items per workgroup of score_select
+constexpr u32 SELECT_ITEMS = 4096;         // items per workgroup of score_select
 
 struct ItemArgs {
   const u32* off;
