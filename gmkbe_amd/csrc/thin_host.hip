@@ -124,7 +124,7 @@ int check_ranks(ThinRun& T, u64 total) {
     HIPCHK(c, launch_thin_rank_index((const u64*)c->prefix.p, a.nfam, a.r0, a.r1, (u32*)c->ridx.p,
                                      &((ThinCounters*)c->tcnt.p)->work, s));
     HIPCHK(c, launch_thin_check(a, s, c->n_cu * 8, c->tune.check_form[a.k < 4 ? a.k : 3]));
-    HIPCHK(c, launch_thin_check_deferred(a, s, c->n_cu * 4));
+    HIPCHK(c, launch_thin_check_deferred(a, s, c->n_cu * 2));
     T.rs->launches_kernel[2]++;
     T.rs->launches++;
     inflight += n;
